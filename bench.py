@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark: SSD cost + gradient of the spline transformation on one MI355X.
+
+Workload (the single-GPU configuration the metric is quoted on): one 3D
+cost+gradient evaluation of `image:cost=ssd` under `spline:rate=8` on 256^3
+float32 volumes (Gaussian-smoothed white noise, sigma 4, rescaled to [0, 1]),
+coefficients N(0, 1.5) clipped to +-4 voxels, cubic B-spline interpolation.
+MIA's grid formula gives 34^3 x 3 = 117,912 parameters for rate 8.
+
+A step = one evaluation (coefficient upload excluded: parameters, images and
+outputs are resident in HBM).  `--rotate` independent input sets are cycled so
+the working set (4 x ~205 MB) exceeds the 256 MB Infinity Cache.
+
+Prints one JSON line (see the repository's bench contract).
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cost+gradient Mvoxel/s (3D 256³) + batched 2D regs/sec at equal SSD"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes(shape, cs, stage):
+    """Algorithmic HBM bytes per launch of each kernel (float volumes)."""
+    nz, ny, nx = shape
+    nvox = nx * ny * nz
+    ncy = cs[1] - 2
+    part = nz * ncy * 12 * cs[0] * 4           # stage-1 partial sums (float)
+    ncoef = cs[0] * cs[1] * cs[2]
+    return {
+        "coef": ncoef * (24 + 16),
+        "warp": nvox * 8,                      # read moving, write warped
+        "ssd_force_adjoint1": nvox * 8 + part,  # read warped + fixed, write partials
+        "finalize": 0,
+        "adjoint_reduce": part + ncoef * 24,   # read partials, write gradient
+    }[stage]
+
+
+def cpu_baseline(mov, fix, cs, params, threads):
+    """The FP32 reference-faithful port of MIA's data flow (oracle/), timed on
+    the host: 1 warm-up + median of 3 with `threads`, and 1 run on 1 thread."""
+    lib_path = os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so")
+    L = ctypes.CDLL(lib_path)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    L.mcp_cost_grad_3d.argtypes = [fp, fp] + [ctypes.c_int] * 6 + [dp, ctypes.c_int, dp, dp]
+    L.mcp_cost_grad_3d.restype = ctypes.c_double
+    nz, ny, nx = mov.shape
+    g = np.zeros(params.size)
+    tm = np.zeros(7)
+
+    def run(nt):
+        t0 = time.perf_counter()
+        c = L.mcp_cost_grad_3d(mov, fix, nx, ny, nz, cs[0], cs[1], cs[2], params, nt, g, tm)
+        return time.perf_counter() - t0, c
+
+    run(threads)
+    times = []
+    for k in range(3):
+        t, cost = run(threads)
+        times.append(t)
+        log("cpu baseline run %d: %.3f s (%d threads)" % (k, t, threads))
+    t1, _ = run(1)
+    log("cpu baseline 1-thread run: %.3f s" % t1)
+    return statistics.median(times), t1, cost, g.copy(), tm.copy()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rotate", type=int, default=4)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=8.0)
+    ap.add_argument("--grad-mode", choices=["reference", "exact"], default="reference")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build targets single-GPU boxes and has no "
+              "multi-GPU path (the sharding design is documented, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    import mia_amd
+    from mia_amd import synth
+
+    n = args.size
+    shape = (n, n, n)
+    cs = tuple(mia_amd.spline_csize(n, args.rate, 3) for _ in range(3))
+    nvox = n ** 3
+    log("generating inputs %s, coefficient grid %s" % (shape, cs))
+    fix0 = synth.smoothed_noise(shape, 4.0, 1)
+    mov0 = synth.smoothed_noise(shape, 4.0, 2)
+    mode = mia_amd.GRAD_EXACT if args.grad_mode == "exact" else mia_amd.GRAD_REFERENCE
+
+    ctxs, bufs, params = [], [], []
+    for r in range(args.rotate):
+        mov = np.ascontiguousarray(np.roll(mov0, 17 * r, axis=2)) if r else mov0
+        fix = np.ascontiguousarray(np.roll(fix0, 11 * r, axis=1)) if r else fix0
+        p = synth.random_spline_params(cs, 1.5, 4.0, 3 + r)
+        c = mia_amd.HipContext(0)
+        c.setup(3, [shape], [cs], 3)
+        c.set_images(0, mov, fix)
+        c.set_cost(grad_mode=mode)
+        costs, grad = c.cost_grad(p)          # uploads p into the context's HBM buffer
+        if r == 0:
+            gpu_cost0, gpu_grad0 = costs[0], grad.copy()
+        ctxs.append(c)
+        bufs.append(c.device_buffers())
+        params.append(p)
+    log("contexts ready (%d input sets)" % args.rotate)
+
+    def step(s):
+        c = ctxs[s % args.rotate]
+        c.cost_grad_device(*bufs[s % args.rotate])
+        c.synchronize()
+
+    for s in range(args.warmup):
+        step(s)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s)
+    for c in ctxs:
+        c.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = nvox / (ms_per_step * 1e-3) / 1e6
+    log("timed %d steps: %.4f ms/step, %.1f Mvoxel/s" % (args.steps, ms_per_step, value))
+
+    # per-kernel durations with HIP events on the context streams
+    for c in ctxs:
+        c.profile_enable(True)
+    for s in range(args.steps):
+        step(s)
+    prof = {}
+    for c in ctxs:
+        for k, (ms, cnt) in c.profile_read().items():
+            a = prof.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+        c.profile_enable(False)
+    kernels = {}
+    for k, (ms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        avg_ms = ms / cnt
+        b = alg_bytes(shape, cs, k)
+        kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "launches": cnt, "alg_bytes": b,
+                      "GBs": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    kd = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4), "traffic": None}
+    tsum = sum(v["avg_us"] for v in kernels.values())
+    log("kernels: %s" % json.dumps(kernels))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        mov = mov0
+        fix = fix0
+        t_med, t_one, c_cpu, g_cpu, tm = cpu_baseline(mov, fix, cs, params[0], threads)
+        cpu = {"value": round(nvox / t_med / 1e6, 3), "unit": "Mvoxel/s", "cores": threads, "kind": "port",
+               "sample": "3 full 256^3 evaluations (median %.3f s) after 1 warm-up; 1-thread run %.3f s "
+                         "(%.3f Mvoxel/s)" % (t_med, t_one, nvox / t_one / 1e6),
+               "stage_seconds": dict(zip(["init_grid", "prefilter", "warp", "gradient", "force", "translate",
+                                          "value"], [round(float(x), 4) for x in tm]))}
+        if mode == mia_amd.GRAD_REFERENCE:
+            parity = {"cost_rel": float(abs(gpu_cost0 - c_cpu) / abs(c_cpu)),
+                      "grad_rel_inf": float(np.abs(gpu_grad0 - g_cpu).max() / np.abs(g_cpu).max())}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mvoxel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3D %d^3 single SSD cost+gradient, spline rate %g (%dx%dx%d x3 coefficients), "
+                               "cubic interpolation, %s gradient" % (n, args.rate, cs[0], cs[1], cs[2],
+                                                                     args.grad_mode),
+                   "voxels": nvox, "params": 3 * cs[0] * cs[1] * cs[2], "rotating_input_sets": args.rotate},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "kernel_time_us_per_step": round(tsum, 2),
+        "parity_vs_cpu_port": parity,
+    }
+    for c in ctxs:
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

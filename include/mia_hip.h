@@ -1,0 +1,143 @@
+/*
+ * mia_hip.h -- C ABI of the MI355X (gfx950) implementation of MIA's B-spline
+ * registration inner loop: cost + gradient of `image:cost=ssd` under the
+ * `spline` transformation, for one or many (batched) 2D/3D registrations.
+ *
+ * Plain pointers and sizes only.  Host buffers are owned by the caller; device
+ * buffers are owned by the opaque context.  One context = one HIP stream; a
+ * context must not be used from two threads at once (the reference's objects
+ * have the same restriction, mia/3d/transform/spline.cc:230,549).
+ *
+ * Every entry point returns a status: MIA_HIP_OK (0) or a negative code; the
+ * message of the last failure on the calling thread is mia_hip_last_error().
+ * The C++ layer (mia_amd/host) maps the codes back to the reference's
+ * exception types (invalid_argument, runtime_error).
+ *
+ * Layouts (the reference's, SURVEY of mia/3d/datafield.cxx:57-58):
+ *   images   : float, x fastest, idx = x + nx*(y + ny*z)
+ *   params   : double, interleaved per coefficient [cx,cy(,cz)], coefficients
+ *              x fastest (mia/3d/transform/spline.cc:289-358,
+ *              mia/2d/transform/spline.cc:292-316); a batch concatenates the
+ *              instances' parameter vectors in instance order.
+ */
+#ifndef MIA_HIP_H
+#define MIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIA_HIP_OK 0
+#define MIA_HIP_ERR_ARG -1      /* bad argument: reference throws invalid_argument */
+#define MIA_HIP_ERR_SIZE -2     /* image size mismatch: reference throws runtime_error */
+#define MIA_HIP_ERR_STATE -3    /* call order (images not set, no setup, ...) */
+#define MIA_HIP_ERR_DEVICE -4   /* HIP runtime failure */
+#define MIA_HIP_ERR_LIMIT -5    /* size beyond a documented kernel limit */
+
+/* cost flags (mia_hip_set_cost) */
+#define MIA_HIP_GRAD_REFERENCE 0 /* gradient of the warped image (what MIA computes) */
+#define MIA_HIP_GRAD_EXACT 1     /* analytic derivative of the cost (what FD measures) */
+
+typedef struct mia_hip_ctx mia_hip_ctx;
+
+/* Thread-local message of the last failed call. */
+const char *mia_hip_last_error(void);
+
+/* Coefficient grid size along one axis for `spline:rate=<rate>` with the
+ * default cubic kernel; replaces the size computation in the
+ * C2D/C3DSplineTransformation constructors (mia/3d/transform/spline.cc:130-143,
+ * mia/2d/transform/spline.cc:103-116).  Returns -1 where the reference throws
+ * invalid_argument (rate too large for the image). No device use. */
+int mia_hip_spline_csize(uint32_t n, float rate, int dim);
+
+/* Create / destroy a context bound to HIP device `device`. */
+int mia_hip_ctx_create(int device, mia_hip_ctx **out);
+int mia_hip_ctx_destroy(mia_hip_ctx *ctx);
+
+/* Describe the registration problem(s): `n_inst` independent instances of
+ * dimension `dim` (2 or 3) with image sizes sizes[3*i..3*i+2] (x,y,z; z = 1 in
+ * 2D) and coefficient grid sizes csizes[3*i..] (including the spline's
+ * 2-coefficient enlargement, as get_coeff_size() returns them,
+ * mia/3d/transform/spline.hh:38-101).  `degree` is the image interpolation
+ * kernel, 3 (`imgkernel=bspline:d=3`, default) or 1 (`bspline:d=1`); boundary
+ * is mirror (mia/template/transformfactory.cxx:58-59).  Replaces the grid
+ * set-up of C?DSplineTransformation::reinit (3D spline.cc:224-275, 2D
+ * :170-203).  Allocates all device buffers. */
+int mia_hip_setup(mia_hip_ctx *ctx, int dim, int n_inst, const uint32_t *sizes,
+                  const uint32_t *csizes, int degree);
+
+/* Upload instance `inst`'s moving (source) and fixed (reference) images from
+ * host memory, prefilter the moving image and record its clamp range.
+ * Replaces C?DImageFullCost::do_reinit/set_reference + the interpolator
+ * construction that the reference repeats on every warp
+ * (mia/3d/fullcost/image.cc:112-128, mia/3d/interpolator.cxx:98-164). */
+int mia_hip_set_images(mia_hip_ctx *ctx, int inst, const float *moving, const float *fixed);
+
+/* Same with device pointers (float, HBM-resident, same layout). */
+int mia_hip_set_images_device(mia_hip_ctx *ctx, int inst, const void *d_moving, const void *d_fixed);
+
+/* Cost options: ssd `norm` (default false), `autothresh` (0 = off), the
+ * full-cost `weight` (mia/3d/fullcost/image.cc:155-165,
+ * mia/template/fullcost.cxx:36-45) and the gradient mode
+ * (MIA_HIP_GRAD_REFERENCE / MIA_HIP_GRAD_EXACT).  Plugin parameters of
+ * mia/template/ssd.hh:217-231. */
+int mia_hip_set_cost(mia_hip_ctx *ctx, int norm, float autothresh, double weight, int grad_mode);
+
+/* Number of parameters of instance `inst` and its offset in the concatenated
+ * parameter vector (degrees_of_freedom(), spline.cc:414-418). */
+int mia_hip_n_params(mia_hip_ctx *ctx, int inst, size_t *n, size_t *offset);
+
+/* Cost and gradient of every instance for host parameter vector `params`
+ * (all instances concatenated).  Writes grad (same layout, OVERWRITTEN, i.e.
+ * the weighted gradient of this cost alone) and costs[n_inst].  Replaces
+ * C?DImageFullCost::do_evaluate + TFullCost::evaluate
+ * (mia/3d/fullcost/image.cc:67-78, mia/2d/fullcost/image.cc:81-105,
+ * mia/template/fullcost.cxx:36-45).  Synchronous. */
+int mia_hip_cost_grad(mia_hip_ctx *ctx, const double *params, double *grad, double *costs);
+
+/* Same with device pointers; enqueued on the context stream, asynchronous. */
+int mia_hip_cost_grad_device(mia_hip_ctx *ctx, const void *d_params, void *d_grad, void *d_costs);
+
+/* The context's own device buffers for parameters, gradient (n_params
+ * doubles each) and costs (n_inst doubles), for callers that keep the
+ * optimizer state in HBM and call mia_hip_cost_grad_device on them.  Valid
+ * until the next mia_hip_setup or mia_hip_ctx_destroy. */
+int mia_hip_device_buffers(mia_hip_ctx *ctx, void **d_params, void **d_grad, void **d_costs);
+
+/* Warped moving image of instance `inst` for its parameters `params_inst`
+ * (transformation operator()(image), mia/core/transformation.hh:151-160,
+ * mia/3d/transform.cc:280-323).  Synchronous, host output. */
+int mia_hip_warp(mia_hip_ctx *ctx, int inst, const double *params_inst, float *out);
+
+/* Capture one cost+gradient iteration (host params -> device -> kernels ->
+ * host grad/costs, all through pinned staging buffers) as a HIP graph.  After
+ * this, mia_hip_graph_params() returns the pinned parameter buffer to fill and
+ * mia_hip_graph_run() replays the iteration and waits; the results appear in
+ * the pinned buffers returned by mia_hip_graph_outputs(). */
+int mia_hip_graph_capture(mia_hip_ctx *ctx);
+int mia_hip_graph_params(mia_hip_ctx *ctx, double **params);
+int mia_hip_graph_outputs(mia_hip_ctx *ctx, double **grad, double **costs);
+int mia_hip_graph_run(mia_hip_ctx *ctx);
+
+/* Wait for all work on the context stream. */
+int mia_hip_synchronize(mia_hip_ctx *ctx);
+
+/* The context's hipStream_t (as void*), for callers that enqueue around it. */
+int mia_hip_get_stream(mia_hip_ctx *ctx, void **stream);
+
+/* Per-kernel timing with HIP events on the context stream.  While enabled,
+ * each kernel launch is bracketed by events; mia_hip_profile_read returns, per
+ * stage id (see mia_hip_stage_name), the summed milliseconds and the number of
+ * launches since the last read, then resets the counters. */
+int mia_hip_profile_enable(mia_hip_ctx *ctx, int enable);
+int mia_hip_profile_read(mia_hip_ctx *ctx, double *ms, int *launches, int max_stages, int *n_stages);
+const char *mia_hip_stage_name(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

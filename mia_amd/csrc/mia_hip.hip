@@ -1,0 +1,820 @@
+// C-ABI implementation (include/mia_hip.h): context, device buffers, spline
+// axis tables and the launch sequence of one cost + gradient evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mia_hip.h"
+#include "kernels.hh"
+
+using namespace mia_hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(MIA_HIP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+#define LAUNCH_CHECK(name)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = hipGetLastError();                                                              \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(MIA_HIP_ERR_DEVICE, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum Stage { ST_COEF = 0, ST_WARP, ST_STAGE1, ST_FINALIZE, ST_REDUCE, ST_COUNT };
+const char *kStageNames[ST_COUNT] = {"coef", "warp", "ssd_force_adjoint1", "finalize", "adjoint_reduce"};
+
+// Spline axis table with the reference's semantics
+// (mia/3d/transform/spline.cc:237-273, mia/2d/transform/spline.cc:181-200):
+// float scale (csize - 3) / (n - 1), float position i*scale + 1, start index
+// fastfloor(pos) - 1 (16.16 rounding, mia/core/splinekernel.cc:164-192), cubic
+// weights at pos - fastfloor(pos) in double, the last sample's fourth weight
+// dropped.  Band table: cells[c] = first sample whose start index is >= c.
+struct AxisTable {
+    std::vector<float> w; // 4 per sample
+    std::vector<int> start;
+    std::vector<int> cells; // csize + 1
+    int nbands = 0;
+};
+
+int make_axis_table(int n, int cs, AxisTable &t, const char *axis)
+{
+    if (n < 2)
+        return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": image size must be >= 2");
+    if (cs < 4)
+        return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": coefficient grid must be >= 4 (2 + enlarge)");
+    if (cs - 3 > n - 1)
+        return fail(MIA_HIP_ERR_LIMIT, std::string("axis ") + axis +
+                                           ": coefficient grid denser than the image (scale > 1) is not supported");
+    const float scale = (float)(cs - 3) / (float)(n - 1);
+    t.w.assign(4 * (size_t)n, 0.0f);
+    t.start.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        const float si = (float)i * scale;
+        const float pos = si + 1.0f;
+        const int f = (int)std::floor(std::nearbyint((double)pos * 65536.0) / 65536.0);
+        const double x = (double)pos - f;
+        const double xm1 = 1.0 - x;
+        const double w3 = (1.0 / 6.0) * x * x * x;
+        const double w0 = (1.0 / 6.0) * xm1 * xm1 * xm1;
+        const double w2 = x - 2.0 * w3 + w0;
+        const double w1 = xm1 - 2.0 * w0 + w3;
+        t.start[i] = f - 1;
+        t.w[4 * i + 0] = (float)w0;
+        t.w[4 * i + 1] = (float)w1;
+        t.w[4 * i + 2] = (float)w2;
+        t.w[4 * i + 3] = (i == n - 1) ? 0.0f : (float)w3;
+    }
+    if (t.start[0] != 0)
+        return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": unexpected first start index");
+    for (int i = 1; i < n; ++i) {
+        const int dlt = t.start[i] - t.start[i - 1];
+        if (dlt < 0 || dlt > 1)
+            return fail(MIA_HIP_ERR_LIMIT, std::string("axis ") + axis + ": non-unit knot step");
+    }
+    if (t.start[n - 1] + 3 > cs || (n > 1 && t.start[n - 2] + 3 > cs - 1))
+        return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": knot range exceeds the coefficient grid");
+    t.cells.assign(cs + 1, n);
+    int c = 0;
+    for (int i = 0; i < n; ++i)
+        while (c <= t.start[i]) {
+            t.cells[c] = i;
+            ++c;
+        }
+    t.nbands = t.start[n - 1] + 1;
+    return MIA_HIP_OK;
+}
+
+long long align64(long long v)
+{
+    return (v + 63) & ~63LL;
+}
+
+} // namespace
+
+struct mia_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int dim = 0, n_inst = 0, degree = 3;
+    int norm = 0, grad_mode = MIA_HIP_GRAD_REFERENCE;
+    float thr = 0.0f;
+    double weight = 1.0;
+
+    std::vector<InstDesc> inst;
+    std::vector<int> warp_prefix, s1_prefix, red_prefix;
+    std::vector<char> images_set;
+    long long n_img = 0, n_par = 0, n_coef = 0, n_part = 0, max_vox = 0;
+    size_t s1_lds = 0;
+
+    InstDesc *d_inst = nullptr;
+    int *d_warp_prefix = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr;
+    float *d_M = nullptr, *d_R = nullptr, *d_W = nullptr, *d_P = nullptr;
+    float2 *d_clamp = nullptr, *d_mm = nullptr;
+    void *d_coef = nullptr;
+    float4 *d_tw = nullptr;
+    int *d_ts = nullptr, *d_cells = nullptr;
+    double *d_ssd = nullptr, *d_gscale = nullptr, *d_params = nullptr, *d_grad = nullptr, *d_costs = nullptr;
+    double *d_scratch = nullptr;
+    unsigned *d_cnt = nullptr;
+
+    double *h_params = nullptr, *h_grad = nullptr, *h_costs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    bool profile = false;
+    struct Ev {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[ST_COUNT] = {0};
+    int prof_n[ST_COUNT] = {0};
+};
+
+namespace {
+
+void free_buffers(mia_hip_ctx *c)
+{
+    void *ptrs[] = {c->d_inst, c->d_warp_prefix, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
+                    c->d_P, c->d_clamp, c->d_mm, c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
+                    c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    if (c->h_params)
+        (void)hipHostFree(c->h_params);
+    if (c->h_grad)
+        (void)hipHostFree(c->h_grad);
+    if (c->h_costs)
+        (void)hipHostFree(c->h_costs);
+    if (c->gexec)
+        (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph)
+        (void)hipGraphDestroy(c->graph);
+    c->d_inst = nullptr;
+    c->d_warp_prefix = c->d_s1_prefix = c->d_red_prefix = nullptr;
+    c->d_M = c->d_R = c->d_W = c->d_P = nullptr;
+    c->d_clamp = c->d_mm = nullptr;
+    c->d_coef = nullptr;
+    c->d_tw = nullptr;
+    c->d_ts = c->d_cells = nullptr;
+    c->d_ssd = c->d_gscale = c->d_params = c->d_grad = c->d_costs = c->d_scratch = nullptr;
+    c->d_cnt = nullptr;
+    c->h_params = c->h_grad = c->h_costs = nullptr;
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+hipEvent_t get_event(mia_hip_ctx *c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct StageTimer {
+    mia_hip_ctx *c;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageTimer(mia_hip_ctx *ctx, int st, bool on) : c(ctx), stage(st)
+    {
+        if (on) {
+            a = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~StageTimer()
+    {
+        if (a) {
+            hipEvent_t b = get_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({stage, a, b});
+        }
+    }
+};
+
+template <typename T>
+int dev_alloc(T **p, size_t count)
+{
+    if (count == 0)
+        count = 1;
+    HIP_TRY(hipMalloc((void **)p, sizeof(T) * count));
+    return MIA_HIP_OK;
+}
+
+// Enqueue coefficient conversion, warp (reference mode), stage 1, finalize and
+// reduce for all instances.  No memsets: every output element is written.
+int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
+{
+    const int n = c->n_inst;
+    {
+        StageTimer t(c, ST_COEF, timed);
+        const long long blocks = (c->n_coef + kBlock - 1) / kBlock;
+        if (c->dim == 3)
+            hipLaunchKernelGGL(k_coef<3>, dim3(blocks), dim3(kBlock), 0, c->stream, d_params, c->d_coef, 0LL,
+                               c->n_coef);
+        else
+            hipLaunchKernelGGL(k_coef<2>, dim3(blocks), dim3(kBlock), 0, c->stream, d_params, c->d_coef, 0LL,
+                               c->n_coef);
+        LAUNCH_CHECK("coef");
+    }
+    if (c->grad_mode == MIA_HIP_GRAD_REFERENCE) {
+        StageTimer t(c, ST_WARP, timed);
+        const int nb = c->warp_prefix[n];
+        if (c->dim == 3) {
+            auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_warp_prefix, 0, nb,
+                               (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+        } else {
+            auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_warp_prefix, 0, nb,
+                               (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+        }
+        LAUNCH_CHECK("warp");
+    }
+    {
+        StageTimer t(c, ST_STAGE1, timed);
+        const int nb = c->s1_prefix[n];
+        const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
+        if (c->dim == 3) {
+            auto kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
+                                       : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), c->s1_lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
+                               (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W,
+                               c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+        } else {
+            auto kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
+                                       : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), c->s1_lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
+                               (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W,
+                               c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+        }
+        LAUNCH_CHECK("stage1");
+    }
+    {
+        StageTimer t(c, ST_FINALIZE, timed);
+        hipLaunchKernelGGL(k_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->d_inst, c->d_s1_prefix, c->d_ssd,
+                           c->d_cnt, c->norm, c->thr, c->weight, d_costs, c->d_gscale);
+        LAUNCH_CHECK("finalize");
+    }
+    {
+        StageTimer t(c, ST_REDUCE, timed);
+        const int nb = c->red_prefix[n];
+        if (c->dim == 3)
+            hipLaunchKernelGGL(k_reduce3d, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix,
+                               c->d_P, c->d_tw, c->d_ts, c->d_cells, c->d_gscale, d_grad);
+        else
+            hipLaunchKernelGGL(k_reduce2d, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix,
+                               c->d_P, c->d_gscale, d_grad);
+        LAUNCH_CHECK("reduce");
+    }
+    return MIA_HIP_OK;
+}
+
+int check_ready(mia_hip_ctx *c)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    for (int i = 0; i < c->n_inst; ++i)
+        if (!c->images_set[i])
+            return fail(MIA_HIP_ERR_STATE, "images of instance " + std::to_string(i) + " are not set");
+    return MIA_HIP_OK;
+}
+
+int prefilter_instance(mia_hip_ctx *c, int i)
+{
+    const InstDesc &d = c->inst[i];
+    float *img = c->d_M + d.img_off;
+    const long long nvox = (long long)d.nx * d.ny * d.nz;
+    const int nbm = (int)std::min<long long>(1024, (nvox + kBlock * 16 - 1) / (kBlock * 16));
+    hipLaunchKernelGGL(k_minmax_partial, dim3(nbm), dim3(kBlock), 0, c->stream, img, nvox, c->d_mm);
+    LAUNCH_CHECK("minmax");
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, c->stream, c->d_mm, nbm, c->d_clamp + i);
+    LAUNCH_CHECK("minmax_final");
+    if (c->degree != 3)
+        return MIA_HIP_OK;
+    struct Pass {
+        long long nlines;
+        int n;
+        long long stride;
+        int mode;
+    } passes[3] = {{(long long)d.ny * d.nz, d.nx, 1, 0},
+                   {(long long)d.nx * d.nz, d.ny, d.nx, 1},
+                   {(long long)d.nx * d.ny, d.nz, (long long)d.nx * d.ny, 2}};
+    const int npass = c->dim == 3 ? 3 : 2;
+    for (int k = 0; k < npass; ++k) {
+        const Pass &p = passes[k];
+        const long long blocks = (p.nlines + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_prefilter_lines, dim3(blocks), dim3(kBlock), 0, c->stream, img, p.nlines, p.n, p.stride,
+                           p.mode, d.nx, d.ny, c->d_scratch);
+        LAUNCH_CHECK("prefilter");
+    }
+    return MIA_HIP_OK;
+}
+
+int collect_profile(mia_hip_ctx *c)
+{
+    if (c->pending.empty())
+        return MIA_HIP_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &e : c->pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.a, e.b));
+        c->prof_ms[e.stage] += ms;
+        c->prof_n[e.stage] += 1;
+        c->pool.push_back(e.a);
+        c->pool.push_back(e.b);
+    }
+    c->pending.clear();
+    return MIA_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *mia_hip_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+int mia_hip_spline_csize(uint32_t n, float rate, int dim)
+{
+    if (rate < 1.0f || n < 1)
+        return -1;
+    float v;
+    if (dim == 3)
+        v = ((float)(n - 1) + rate) / rate;
+    else
+        v = (((float)n + rate) - 1.0f) / rate;
+    const int cs = (int)v;
+    return cs <= 1 ? -1 : cs + 2;
+}
+
+int mia_hip_ctx_create(int device, mia_hip_ctx **out)
+{
+    if (!out)
+        return fail(MIA_HIP_ERR_ARG, "null output pointer");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(MIA_HIP_ERR_ARG, "no HIP device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library targets gfx950");
+    mia_hip_ctx *c = new mia_hip_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_ctx_destroy(mia_hip_ctx *c)
+{
+    if (!c)
+        return MIA_HIP_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_buffers(c);
+    for (auto &e : c->pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto e : c->pool)
+        (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const uint32_t *csizes, int degree)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (dim != 2 && dim != 3)
+        return fail(MIA_HIP_ERR_ARG, "dim must be 2 or 3");
+    if (n_inst < 1 || !sizes || !csizes)
+        return fail(MIA_HIP_ERR_ARG, "need at least one instance with sizes and coefficient sizes");
+    if (degree != 1 && degree != 3)
+        return fail(MIA_HIP_ERR_ARG, "image interpolation kernel must be bspline:d=1 or bspline:d=3");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_buffers(c);
+    c->dim = dim;
+    c->degree = degree;
+    c->n_inst = 0;
+    c->inst.assign(n_inst, InstDesc());
+    c->warp_prefix.assign(n_inst + 1, 0);
+    c->s1_prefix.assign(n_inst + 1, 0);
+    c->red_prefix.assign(n_inst + 1, 0);
+    c->images_set.assign(n_inst, 0);
+    std::vector<float> tw;
+    std::vector<int> ts, cells;
+    long long img = 0, par = 0, coef = 0, part = 0, maxv = 0;
+    int max_nx = 0, max_cx = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        InstDesc &d = c->inst[i];
+        d.nx = (int)sizes[3 * i];
+        d.ny = (int)sizes[3 * i + 1];
+        d.nz = dim == 3 ? (int)sizes[3 * i + 2] : 1;
+        d.cx = (int)csizes[3 * i];
+        d.cy = (int)csizes[3 * i + 1];
+        d.cz = dim == 3 ? (int)csizes[3 * i + 2] : 1;
+        if (d.nx > kMaxRow)
+            return fail(MIA_HIP_ERR_LIMIT, "image x size " + std::to_string(d.nx) + " exceeds the limit " +
+                                               std::to_string(kMaxRow));
+        AxisTable ax, ay, az;
+        int st = make_axis_table(d.nx, d.cx, ax, "x");
+        if (st)
+            return st;
+        st = make_axis_table(d.ny, d.cy, ay, "y");
+        if (st)
+            return st;
+        if (dim == 3) {
+            st = make_axis_table(d.nz, d.cz, az, "z");
+            if (st)
+                return st;
+        } else {
+            az.w = {1.0f, 0.0f, 0.0f, 0.0f};
+            az.start = {0};
+            az.cells = {0, 1};
+            az.nbands = 1;
+        }
+        d.ncy = ay.nbands;
+        d.ncz = az.nbands;
+        AxisTable *axes[3] = {&ax, &ay, &az};
+        int *toff[3] = {&d.tab_x, &d.tab_y, &d.tab_z};
+        int *coff[3] = {&d.cell_x, &d.cell_y, &d.cell_z};
+        for (int a = 0; a < 3; ++a) {
+            *toff[a] = (int)ts.size();
+            ts.insert(ts.end(), axes[a]->start.begin(), axes[a]->start.end());
+            tw.insert(tw.end(), axes[a]->w.begin(), axes[a]->w.end());
+            *coff[a] = (int)cells.size();
+            cells.insert(cells.end(), axes[a]->cells.begin(), axes[a]->cells.end());
+        }
+        const long long nvox = (long long)d.nx * d.ny * d.nz;
+        const long long ncoef = (long long)d.cx * d.cy * d.cz;
+        d.img_off = img;
+        img += align64(nvox);
+        d.par_off = par;
+        par += dim * ncoef;
+        d.coef_off = coef;
+        coef += ncoef;
+        d.part_off = part;
+        part += dim == 3 ? (long long)d.nz * d.ncy * 12 * d.cx : (long long)d.ncy * 8 * d.cx;
+        maxv = std::max(maxv, nvox);
+        max_nx = std::max(max_nx, d.nx);
+        max_cx = std::max(max_cx, d.cx);
+        const int tiles = ((d.nx + kWarpTileX - 1) / kWarpTileX) * ((d.ny + kWarpTileY - 1) / kWarpTileY) * d.nz;
+        c->warp_prefix[i + 1] = c->warp_prefix[i] + tiles;
+        c->s1_prefix[i + 1] = c->s1_prefix[i] + (dim == 3 ? d.nz * d.ncy : d.ncy);
+        c->red_prefix[i + 1] = c->red_prefix[i] + (int)((ncoef + kBlock - 1) / kBlock);
+    }
+    c->n_img = img;
+    c->n_par = par;
+    c->n_coef = coef;
+    c->n_part = part;
+    c->max_vox = maxv;
+    c->s1_lds = dim == 3 ? (size_t)(12 * max_nx) * sizeof(float) + (size_t)4 * (max_cx + 4) * sizeof(float4)
+                         : (size_t)(8 * max_nx) * sizeof(float) + (size_t)4 * (max_cx + 4) * sizeof(float2);
+    int st;
+    if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_warp_prefix, n_inst + 1)) ||
+        (st = dev_alloc(&c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_red_prefix, n_inst + 1)) ||
+        (st = dev_alloc(&c->d_M, img)) || (st = dev_alloc(&c->d_R, img)) || (st = dev_alloc(&c->d_W, img)) ||
+        (st = dev_alloc(&c->d_P, part)) || (st = dev_alloc(&c->d_clamp, n_inst)) || (st = dev_alloc(&c->d_mm, 1024)) ||
+        (st = dev_alloc((float4 **)&c->d_coef, coef)) || (st = dev_alloc(&c->d_tw, tw.size() / 4)) ||
+        (st = dev_alloc(&c->d_ts, ts.size())) || (st = dev_alloc(&c->d_cells, cells.size())) ||
+        (st = dev_alloc(&c->d_ssd, c->s1_prefix[n_inst])) || (st = dev_alloc(&c->d_cnt, c->s1_prefix[n_inst])) ||
+        (st = dev_alloc(&c->d_gscale, n_inst)) || (st = dev_alloc(&c->d_params, par)) ||
+        (st = dev_alloc(&c->d_grad, par)) || (st = dev_alloc(&c->d_costs, n_inst)) ||
+        (st = dev_alloc(&c->d_scratch, maxv))) {
+        free_buffers(c);
+        return st;
+    }
+    HIP_TRY(hipHostMalloc((void **)&c->h_params, sizeof(double) * std::max<long long>(par, 1), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_grad, sizeof(double) * std::max<long long>(par, 1), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_costs, sizeof(double) * n_inst, hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(c->d_inst, c->inst.data(), sizeof(InstDesc) * n_inst, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_warp_prefix, c->warp_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_s1_prefix, c->s1_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_red_prefix, c->red_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_tw, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_ts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_cells, cells.data(), sizeof(int) * cells.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->n_inst = n_inst;
+    return MIA_HIP_OK;
+}
+
+static int set_images_impl(mia_hip_ctx *c, int inst, const void *moving, const void *fixed, hipMemcpyKind kind)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    if (!moving || !fixed)
+        return fail(MIA_HIP_ERR_ARG, "null image pointer");
+    HIP_TRY(hipSetDevice(c->device));
+    const InstDesc &d = c->inst[inst];
+    const size_t bytes = sizeof(float) * (size_t)d.nx * d.ny * d.nz;
+    HIP_TRY(hipMemcpyAsync(c->d_M + d.img_off, moving, bytes, kind, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_R + d.img_off, fixed, bytes, kind, c->stream));
+    int st = prefilter_instance(c, inst);
+    if (st)
+        return st;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->images_set[inst] = 1;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_set_images(mia_hip_ctx *c, int inst, const float *moving, const float *fixed)
+{
+    return set_images_impl(c, inst, moving, fixed, hipMemcpyHostToDevice);
+}
+
+int mia_hip_set_images_device(mia_hip_ctx *c, int inst, const void *d_moving, const void *d_fixed)
+{
+    return set_images_impl(c, inst, d_moving, d_fixed, hipMemcpyDeviceToDevice);
+}
+
+int mia_hip_set_cost(mia_hip_ctx *c, int norm, float autothresh, double weight, int grad_mode)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (!(autothresh >= 0.0f && autothresh <= 1000.0f))
+        return fail(MIA_HIP_ERR_ARG, "autothresh must be in [0, 1000]");
+    if (grad_mode != MIA_HIP_GRAD_REFERENCE && grad_mode != MIA_HIP_GRAD_EXACT)
+        return fail(MIA_HIP_ERR_ARG, "unknown gradient mode");
+    c->norm = norm ? 1 : 0;
+    c->thr = autothresh;
+    c->weight = weight;
+    c->grad_mode = grad_mode;
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    if (c->graph) {
+        (void)hipGraphDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return MIA_HIP_OK;
+}
+
+int mia_hip_n_params(mia_hip_ctx *c, int inst, size_t *n, size_t *offset)
+{
+    if (!c || c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    const InstDesc &d = c->inst[inst];
+    if (n)
+        *n = (size_t)c->dim * d.cx * d.cy * d.cz;
+    if (offset)
+        *offset = (size_t)d.par_off;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_cost_grad(mia_hip_ctx *c, const double *params, double *grad, double *costs)
+{
+    int st = check_ready(c);
+    if (st)
+        return st;
+    if (!params || !grad || !costs)
+        return fail(MIA_HIP_ERR_ARG, "null pointer");
+    HIP_TRY(hipSetDevice(c->device));
+    std::memcpy(c->h_params, params, sizeof(double) * c->n_par);
+    HIP_TRY(hipMemcpyAsync(c->d_params, c->h_params, sizeof(double) * c->n_par, hipMemcpyHostToDevice, c->stream));
+    if ((st = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, c->profile)))
+        return st;
+    HIP_TRY(hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->n_par, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_costs, c->d_costs, sizeof(double) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(grad, c->h_grad, sizeof(double) * c->n_par);
+    std::memcpy(costs, c->h_costs, sizeof(double) * c->n_inst);
+    return MIA_HIP_OK;
+}
+
+int mia_hip_cost_grad_device(mia_hip_ctx *c, const void *d_params, void *d_grad, void *d_costs)
+{
+    int st = check_ready(c);
+    if (st)
+        return st;
+    if (!d_params || !d_grad || !d_costs)
+        return fail(MIA_HIP_ERR_ARG, "null pointer");
+    HIP_TRY(hipSetDevice(c->device));
+    return enqueue_eval(c, (const double *)d_params, (double *)d_grad, (double *)d_costs, c->profile);
+}
+
+int mia_hip_device_buffers(mia_hip_ctx *c, void **d_params, void **d_grad, void **d_costs)
+{
+    if (!c || c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (d_params)
+        *d_params = c->d_params;
+    if (d_grad)
+        *d_grad = c->d_grad;
+    if (d_costs)
+        *d_costs = c->d_costs;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_warp(mia_hip_ctx *c, int inst, const double *params_inst, float *out)
+{
+    if (!c || c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    if (!c->images_set[inst])
+        return fail(MIA_HIP_ERR_STATE, "images of instance " + std::to_string(inst) + " are not set");
+    if (!params_inst || !out)
+        return fail(MIA_HIP_ERR_ARG, "null pointer");
+    HIP_TRY(hipSetDevice(c->device));
+    const InstDesc &d = c->inst[inst];
+    const long long ncoef = (long long)d.cx * d.cy * d.cz;
+    std::memcpy(c->h_params + d.par_off, params_inst, sizeof(double) * c->dim * ncoef);
+    HIP_TRY(hipMemcpyAsync(c->d_params + d.par_off, c->h_params + d.par_off, sizeof(double) * c->dim * ncoef,
+                           hipMemcpyHostToDevice, c->stream));
+    const long long blocks = (ncoef + kBlock - 1) / kBlock;
+    if (c->dim == 3)
+        hipLaunchKernelGGL(k_coef<3>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_params, c->d_coef, d.coef_off,
+                           ncoef);
+    else
+        hipLaunchKernelGGL(k_coef<2>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_params, c->d_coef, d.coef_off,
+                           ncoef);
+    LAUNCH_CHECK("coef");
+    const int boff = c->warp_prefix[inst];
+    const int nb = c->warp_prefix[inst + 1] - boff;
+    if (c->dim == 3) {
+        auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff, nb,
+                           (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+    } else {
+        auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff, nb,
+                           (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+    }
+    LAUNCH_CHECK("warp");
+    HIP_TRY(hipMemcpyAsync(out, c->d_W + d.img_off, sizeof(float) * (size_t)d.nx * d.ny * d.nz, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MIA_HIP_OK;
+}
+
+int mia_hip_graph_capture(mia_hip_ctx *c)
+{
+    int st = check_ready(c);
+    if (st)
+        return st;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    if (c->graph) {
+        (void)hipGraphDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(c->d_params, c->h_params, sizeof(double) * c->n_par, hipMemcpyHostToDevice,
+                                  c->stream);
+    int rc = e == hipSuccess ? enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false)
+                             : fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+    if (rc == MIA_HIP_OK) {
+        e = hipMemcpyAsync(c->h_grad, c->d_grad, sizeof(double) * c->n_par, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->h_costs, c->d_costs, sizeof(double) * c->n_inst, hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess)
+            rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc != MIA_HIP_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e2 != hipSuccess)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    c->graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return MIA_HIP_OK;
+}
+
+int mia_hip_graph_params(mia_hip_ctx *c, double **params)
+{
+    if (!c || !c->gexec)
+        return fail(MIA_HIP_ERR_STATE, "no captured graph");
+    *params = c->h_params;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_graph_outputs(mia_hip_ctx *c, double **grad, double **costs)
+{
+    if (!c || !c->gexec)
+        return fail(MIA_HIP_ERR_STATE, "no captured graph");
+    *grad = c->h_grad;
+    *costs = c->h_costs;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_graph_run(mia_hip_ctx *c)
+{
+    if (!c || !c->gexec)
+        return fail(MIA_HIP_ERR_STATE, "no captured graph");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MIA_HIP_OK;
+}
+
+int mia_hip_synchronize(mia_hip_ctx *c)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MIA_HIP_OK;
+}
+
+int mia_hip_get_stream(mia_hip_ctx *c, void **stream)
+{
+    if (!c || !stream)
+        return fail(MIA_HIP_ERR_ARG, "null pointer");
+    *stream = (void *)c->stream;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_profile_enable(mia_hip_ctx *c, int enable)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    c->profile = enable != 0;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_profile_read(mia_hip_ctx *c, double *ms, int *launches, int max_stages, int *n_stages)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    int st = collect_profile(c);
+    if (st)
+        return st;
+    const int n = std::min<int>(max_stages, ST_COUNT);
+    for (int k = 0; k < n; ++k) {
+        if (ms)
+            ms[k] = c->prof_ms[k];
+        if (launches)
+            launches[k] = c->prof_n[k];
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    if (n_stages)
+        *n_stages = ST_COUNT;
+    return MIA_HIP_OK;
+}
+
+const char *mia_hip_stage_name(int stage)
+{
+    if (stage < 0 || stage >= ST_COUNT)
+        return "";
+    return kStageNames[stage];
+}
+
+} // extern "C"

@@ -1,0 +1,410 @@
+// mia_cpu_port.cc -- FP32 CPU restatement of MIA's own data flow for one
+// image:cost=ssd + spline cost/gradient evaluation, used ONLY as the bench's
+// CPU baseline (timed beside the GPU, never a product path).  The reference
+// itself cannot be built here (Boost, GSL and NLopt are absent), so this port
+// keeps its structure and its thread decomposition:
+//   * C3DSplineTransformation::set_parameters -> init_grid: separable x, y, z
+//     passes over slices, single-threaded (mia/3d/transform/spline.cc:344-358,
+//     541-653)
+//   * the interpolator is constructed (copy + 3-axis prefilter, pfor over z
+//     and y) on every warp (mia/3d/transform.cc:306, interpolator.cxx:98-164)
+//   * warp: pfor over z slabs, per-voxel 4x4x4 float contraction with clamp
+//     (mia/3d/transform.cc:308-353, mia/3d/interpolator.cxx:250-287)
+//   * get_gradient, SSD force, translate and the second SSD pass serial
+//     (mia/3d/image.cc:347-477, mia/template/ssd.hh:54-181,
+//     mia/3d/transform/spline.cc:731-791, mia/3d/fullcost/image.cc:67-78)
+// pfor uses `nthreads` std::threads (the reference uses hardware_concurrency
+// tasks, mia/core/parallelcxx11.cc:25-31).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+
+template <typename F>
+void pfor(int n, int nthreads, F f)
+{
+    if (nthreads <= 1 || n < 2) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int chunk = (n + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        const int b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e)
+            break;
+        th.emplace_back([=]() { f(b, e); });
+    }
+    for (auto &t : th)
+        t.join();
+}
+
+int fastfloor(double v)
+{
+    return (int)std::floor(std::nearbyint(v * 65536.0) / 65536.0);
+}
+
+void bspline3(double x, double w[4])
+{
+    const double xm1 = 1.0 - x;
+    w[3] = x * x * x / 6.0;
+    w[0] = xm1 * xm1 * xm1 / 6.0;
+    w[2] = x - 2.0 * w[3] + w[0];
+    w[1] = xm1 - 2.0 * w[0] + w[3];
+}
+
+double bspline3_value(double x)
+{
+    x = std::fabs(x);
+    if (x <= 1.0)
+        return 2.0 / 3.0 - x * x * (1 - 0.5 * x);
+    if (x < 2.0) {
+        const double d = 2.0 - x;
+        return d * d * d / 6.0;
+    }
+    return 0.0;
+}
+
+int mirror(int i, int n)
+{
+    if (n == 1)
+        return 0;
+    const int n2 = 2 * n - 2;
+    int idx = i < 0 ? -i : i;
+    idx = idx < n2 ? idx : idx % n2;
+    if (idx >= n)
+        idx = n2 - idx;
+    return idx;
+}
+
+void filter_line(std::vector<double> &c)
+{
+    const int n = (int)c.size();
+    if (n < 2)
+        return;
+    const double p = std::sqrt(3.0) - 2.0;
+    const double lambda = 2.0 - p - 1.0 / p;
+    for (auto &v : c)
+        v *= lambda;
+    double zn = p, iz = 1.0 / p, z2n = std::pow(p, (double)(n - 1));
+    double sum = c[0] + z2n * c[n - 1];
+    z2n *= z2n * iz;
+    for (int k = 1; k < n - 1; ++k) {
+        sum += (zn + z2n) * c[k];
+        zn *= p;
+        z2n *= iz;
+    }
+    c[0] = sum / (1.0 - zn * zn);
+    for (int k = 1; k < n; ++k)
+        c[k] += p * c[k - 1];
+    c[n - 1] = (p / (p * p - 1.0)) * (p * c[n - 2] + c[n - 1]);
+    for (int k = n - 2; k >= 0; --k)
+        c[k] = p * (c[k + 1] - c[k]);
+}
+
+struct Axis {
+    std::vector<int> start;
+    std::vector<std::vector<double>> w;
+    std::vector<int> row_first;
+    std::vector<std::vector<double>> row;
+};
+
+Axis make_axis(int n, int cs)
+{
+    Axis a;
+    const float scale = (float)(cs - 3) / (float)(n - 1);
+    a.start.resize(n);
+    a.w.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const float pos = (float)i * scale + 1.0f;
+        const int f = fastfloor(pos);
+        double w[4];
+        bspline3((double)pos - f, w);
+        a.start[i] = f - 1;
+        a.w[i].assign(w, w + (i == n - 1 ? 3 : 4));
+    }
+    // adjoint rows, get_derivative_row (spline.cc:697-729)
+    a.row_first.assign(cs, -1);
+    a.row.resize(cs);
+    for (int o = 0; o < cs; ++o)
+        for (int i = 0; i < n; ++i) {
+            const double x = i * (double)scale + 1.0 - o;
+            if (std::fabs(x) < 2.0) {
+                if (a.row_first[o] < 0)
+                    a.row_first[o] = i;
+                const double y = bspline3_value(x);
+                if (y > 0)
+                    a.row[o].push_back(y);
+                else
+                    break;
+            }
+        }
+    return a;
+}
+
+void saxpy(int n, float a, const float *x, float *y)
+{
+    for (int i = 0; i < n; ++i)
+        y[i] += a * x[i];
+}
+
+} // namespace
+
+extern "C" {
+
+// One full evaluation; returns the cost, writes grad (3 * cx*cy*cz doubles).
+// timings (optional, 7 entries, seconds): init_grid, prefilter, warp,
+// gradient, force, translate, value.
+double mcp_cost_grad_3d(const float *mov, const float *fix, int nx, int ny, int nz, int cx, int cy, int cz,
+                        const double *params, int nthreads, double *grad, double *timings)
+{
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    const size_t N = (size_t)nx * ny * nz;
+    auto t0 = clk::now();
+    // --- set_parameters + init_grid -------------------------------------------
+    std::vector<V3> coef((size_t)cx * cy * cz);
+    for (size_t k = 0; k < coef.size(); ++k)
+        coef[k] = {(float)params[3 * k], (float)params[3 * k + 1], (float)params[3 * k + 2]};
+    Axis ax = make_axis(nx, cx), ay = make_axis(ny, cy), az = make_axis(nz, cz);
+    // x pass: tmp(x, ky, kz) = sum_i wx_i coef(sx+i, ky, kz)
+    std::vector<V3> tmp((size_t)nx * cy * cz);
+    {
+        std::vector<float> slice((size_t)cy * cz * 3), out((size_t)cy * cz * 3);
+        for (int x = 0; x < nx; ++x) {
+            std::fill(out.begin(), out.end(), 0.0f);
+            for (size_t i = 0; i < ax.w[x].size(); ++i) {
+                const int kx = ax.start[x] + (int)i;
+                if (kx >= cx)
+                    break;
+                for (int kz = 0; kz < cz; ++kz)
+                    for (int ky = 0; ky < cy; ++ky) {
+                        const V3 &c = coef[((size_t)kz * cy + ky) * cx + kx];
+                        float *o = &slice[3 * ((size_t)kz * cy + ky)];
+                        o[0] = c.x;
+                        o[1] = c.y;
+                        o[2] = c.z;
+                    }
+                saxpy((int)out.size(), (float)ax.w[x][i], slice.data(), out.data());
+            }
+            for (int kz = 0; kz < cz; ++kz)
+                for (int ky = 0; ky < cy; ++ky) {
+                    const float *o = &out[3 * ((size_t)kz * cy + ky)];
+                    tmp[((size_t)kz * cy + ky) * nx + x] = {o[0], o[1], o[2]};
+                }
+        }
+    }
+    // y pass: tmp2(x, y, kz)
+    std::vector<V3> tmp2((size_t)nx * ny * cz);
+    {
+        std::vector<float> out((size_t)nx * cz * 3), slice((size_t)nx * cz * 3);
+        for (int y = 0; y < ny; ++y) {
+            std::fill(out.begin(), out.end(), 0.0f);
+            for (size_t i = 0; i < ay.w[y].size(); ++i) {
+                const int ky = ay.start[y] + (int)i;
+                if (ky >= cy)
+                    break;
+                for (int kz = 0; kz < cz; ++kz)
+                    std::memcpy(&slice[3 * (size_t)kz * nx], &tmp[((size_t)kz * cy + ky) * nx], sizeof(V3) * nx);
+                saxpy((int)out.size(), (float)ay.w[y][i], slice.data(), out.data());
+            }
+            for (int kz = 0; kz < cz; ++kz)
+                std::memcpy(&tmp2[((size_t)kz * ny + y) * nx], &out[3 * (size_t)kz * nx], sizeof(V3) * nx);
+        }
+    }
+    // z pass into the position field T(x) = x - u(x)
+    std::vector<V3> grid(N);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+                grid[((size_t)z * ny + y) * nx + x] = {(float)x, (float)y, (float)z};
+    for (int z = 0; z < nz; ++z)
+        for (size_t i = 0; i < az.w[z].size(); ++i) {
+            const int kz = az.start[z] + (int)i;
+            if (kz >= cz)
+                break;
+            saxpy((int)(3 * (size_t)nx * ny), -(float)az.w[z][i], &tmp2[(size_t)kz * nx * ny].x,
+                  &grid[(size_t)z * nx * ny].x);
+        }
+    auto t1 = clk::now();
+    // --- interpolator construction (every warp) --------------------------------
+    std::vector<float> cf(mov, mov + N);
+    float vmin = mov[0], vmax = mov[0];
+    for (size_t i = 0; i < N; ++i) {
+        vmin = std::min(vmin, mov[i]);
+        vmax = std::max(vmax, mov[i]);
+    }
+    if (0.0f < vmin)
+        vmin = 0.0f;
+    pfor(nz, nthreads, [&](int zb, int ze) {
+        std::vector<double> buf(nx);
+        for (int z = zb; z < ze; ++z)
+            for (int y = 0; y < ny; ++y) {
+                float *row = &cf[((size_t)z * ny + y) * nx];
+                for (int x = 0; x < nx; ++x)
+                    buf[x] = row[x];
+                filter_line(buf);
+                for (int x = 0; x < nx; ++x)
+                    row[x] = (float)buf[x];
+            }
+    });
+    pfor(nz, nthreads, [&](int zb, int ze) {
+        std::vector<double> buf(ny);
+        for (int z = zb; z < ze; ++z)
+            for (int x = 0; x < nx; ++x) {
+                for (int y = 0; y < ny; ++y)
+                    buf[y] = cf[((size_t)z * ny + y) * nx + x];
+                filter_line(buf);
+                for (int y = 0; y < ny; ++y)
+                    cf[((size_t)z * ny + y) * nx + x] = (float)buf[y];
+            }
+    });
+    pfor(ny, nthreads, [&](int yb, int ye) {
+        std::vector<double> buf(nz);
+        for (int y = yb; y < ye; ++y)
+            for (int x = 0; x < nx; ++x) {
+                for (int z = 0; z < nz; ++z)
+                    buf[z] = cf[((size_t)z * ny + y) * nx + x];
+                filter_line(buf);
+                for (int z = 0; z < nz; ++z)
+                    cf[((size_t)z * ny + y) * nx + x] = (float)buf[z];
+            }
+    });
+    auto t2 = clk::now();
+    // --- warp, pfor over z ----------------------------------------------------------
+    std::vector<float> W(N);
+    pfor(nz, nthreads, [&](int zb, int ze) {
+        for (int z = zb; z < ze; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const size_t idx = ((size_t)z * ny + y) * nx + x;
+                    const V3 p = grid[idx];
+                    int s[3];
+                    double wd[3][4];
+                    const float pc[3] = {p.x, p.y, p.z};
+                    for (int a = 0; a < 3; ++a) {
+                        const int f = fastfloor(pc[a]);
+                        bspline3((double)pc[a] - f, wd[a]);
+                        s[a] = f - 1;
+                    }
+                    int ix[4], iy[4], iz[4];
+                    for (int k = 0; k < 4; ++k) {
+                        ix[k] = mirror(s[0] + k, nx);
+                        iy[k] = mirror(s[1] + k, ny);
+                        iz[k] = mirror(s[2] + k, nz);
+                    }
+                    float r = 0.0f;
+                    for (int c = 0; c < 4; ++c) {
+                        float ry = 0.0f;
+                        for (int b = 0; b < 4; ++b) {
+                            const float *row = &cf[((size_t)iz[c] * ny + iy[b]) * nx];
+                            float rx = 0.0f;
+                            for (int a = 0; a < 4; ++a)
+                                rx += (float)wd[0][a] * row[ix[a]];
+                            ry += (float)wd[1][b] * rx;
+                        }
+                        r += (float)wd[2][c] * ry;
+                    }
+                    W[idx] = r >= vmin ? (r <= vmax ? r : vmax) : vmin;
+                }
+    });
+    auto t3 = clk::now();
+    // --- get_gradient (serial) --------------------------------------------------------
+    std::vector<V3> g(N);
+    const size_t sy = nx, sz = (size_t)nx * ny;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t i = ((size_t)z * ny + y) * nx + x;
+                g[i].x = (x > 0 && x < nx - 1) ? 0.5f * (W[i + 1] - W[i - 1]) : 0.0f;
+                g[i].y = (y > 0 && y < ny - 1) ? 0.5f * (W[i + sy] - W[i - sy]) : 0.0f;
+                g[i].z = (z > 0 && z < nz - 1) ? 0.5f * (W[i + sz] - W[i - sz]) : 0.0f;
+            }
+    auto t4 = clk::now();
+    // --- force (serial, float) ---------------------------------------------------------
+    std::vector<V3> force(N);
+    float fcost = 0.0f;
+    for (size_t i = 0; i < N; ++i) {
+        const float d = W[i] - fix[i];
+        force[i] = {g[i].x * d, g[i].y * d, g[i].z * d};
+        fcost += d * d;
+    }
+    (void)fcost;
+    auto t5 = clk::now();
+    // --- translate (serial, separable z -> y -> x) --------------------------------------
+    std::vector<V3> t1v((size_t)nx * ny * cz, V3{0, 0, 0});
+    for (int oz = 0; oz < cz; ++oz) {
+        int i = az.row_first[oz];
+        for (double w : az.row[oz]) {
+            saxpy((int)(3 * (size_t)nx * ny), (float)w, &force[(size_t)i * nx * ny].x, &t1v[(size_t)oz * nx * ny].x);
+            ++i;
+        }
+    }
+    std::vector<V3> t2v((size_t)nx * cy * cz);
+    {
+        std::vector<V3> in(ny);
+        for (int oz = 0; oz < cz; ++oz)
+            for (int x = 0; x < nx; ++x) {
+                for (int y = 0; y < ny; ++y)
+                    in[y] = t1v[((size_t)oz * ny + y) * nx + x];
+                for (int oy = 0; oy < cy; ++oy) {
+                    V3 acc{0, 0, 0};
+                    int j = ay.row_first[oy];
+                    for (double w : ay.row[oy]) {
+                        acc.x += (float)w * in[j].x;
+                        acc.y += (float)w * in[j].y;
+                        acc.z += (float)w * in[j].z;
+                        ++j;
+                    }
+                    t2v[((size_t)oz * cy + oy) * nx + x] = acc;
+                }
+            }
+    }
+    for (int oz = 0; oz < cz; ++oz)
+        for (int oy = 0; oy < cy; ++oy) {
+            const V3 *in = &t2v[((size_t)oz * cy + oy) * nx];
+            for (int ox = 0; ox < cx; ++ox) {
+                V3 acc{0, 0, 0};
+                int j = ax.row_first[ox];
+                for (double w : ax.row[ox]) {
+                    acc.x += (float)w * in[j].x;
+                    acc.y += (float)w * in[j].y;
+                    acc.z += (float)w * in[j].z;
+                    ++j;
+                }
+                double *r = grad + 3 * (((size_t)oz * cy + oy) * cx + ox);
+                r[0] = -acc.x;
+                r[1] = -acc.y;
+                r[2] = -acc.z;
+            }
+        }
+    auto t6 = clk::now();
+    // --- value (second pass, double) ---------------------------------------------------
+    double sum = 0.0;
+    for (size_t i = 0; i < N; ++i) {
+        const double d = (double)W[i] - (double)fix[i];
+        sum += d * d;
+    }
+    auto t7 = clk::now();
+    if (timings) {
+        timings[0] = secs(t0, t1);
+        timings[1] = secs(t1, t2);
+        timings[2] = secs(t2, t3);
+        timings[3] = secs(t3, t4);
+        timings[4] = secs(t4, t5);
+        timings[5] = secs(t5, t6);
+        timings[6] = secs(t6, t7);
+    }
+    return 0.5 * sum;
+}
+
+} // extern "C"
