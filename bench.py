@@ -79,8 +79,28 @@ def cpu_baseline(mov, fix, cs, params, threads):
     return statistics.median(times), t1, cost, g.copy(), tm.copy()
 
 
+def copy_calibration(mib, reps):
+    """Device-to-device copies of `mib` MiB (the HIP runtime's blit kernel): a
+    transfer of known size against which rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters are scaled."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -92,6 +112,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 3)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build targets single-GPU boxes and has no "
               "multi-GPU path (the sharding design is documented, not implemented).", file=sys.stderr)

@@ -1,19 +1,26 @@
 // Device kernels of the B-spline SSD cost + gradient path for gfx950.
 //
-// Data flow of one evaluation (all instances of a batch in each launch):
-//   k_coef      params (double, interleaved) -> coefficient vectors (float)
-//   k_warp      displacement u(x) (separable through an LDS coefficient
-//               window) and warp W(x) = M~(x - u(x)) of the prefiltered moving
-//               image with mirror indices and clamp              [reference mode]
-//   k_stage1    per (z slice, y knot band): SSD residual, force, FP64 SSD
-//               partial with a wave64 shuffle reduction, contraction of the
-//               force with the y weights (registers) and the x weights (LDS)
-//               -> per-band partial sums on the coefficient grid; in exact mode
-//               it also does the warp and the analytic image gradient inline
-//   k_finalize  per instance: fixed-order FP64 sum of the SSD partials, cost,
-//               gradient scale (norm / autothresh / weight)
-//   k_reduce    per coefficient: fixed-order FP64 sum over the four y bands
-//               and the z support -> gradient (no atomics: deterministic)
+// Data flow of one evaluation (every launch covers all instances of a batch):
+//   k_warp      per 32x8x8 (3D) / 64x32 (2D) voxel tile: displacement u(x) by a
+//               separable contraction of the double parameters through an LDS
+//               coefficient window, sample positions x - u(x) kept in
+//               registers, the tile's bounding box of interpolation taps
+//               loaded once into LDS (mirror indices applied on the load), then
+//               the 4^3 (4^2) taps per voxel read from LDS, clamp, store W.
+//               A tile whose bounding box does not fit LDS gathers from global
+//               memory instead (same arithmetic).                  [reference mode]
+//   k_stage1    per (z slice, y knot band): residual d = W - R, force
+//               f = grad * d, FP64 SSD partial (wave64 shuffle + fixed-order
+//               block sum), contraction of f with the band's y weights in
+//               registers and with the x weights through LDS -> partial sums
+//               P[z][band][t][kx][c] on the coefficient grid.  In exact mode it
+//               computes the warp and the analytic image gradient inline.
+//   k_finalize  (only with autothresh) per instance: SSD sum, mask count, cost
+//               and gradient scale.
+//   k_reduce    per coefficient component: fixed-order FP64 sum over the four y
+//               bands and the z support -> gradient; without autothresh its
+//               first n_inst blocks also do the finalize work.  No atomics: the
+//               result is deterministic and independent of the batch layout.
 //
 // Reference algorithm: mia/3d/transform/spline.cc:541-653 (init_grid),
 // mia/3d/interpolator.cxx:250-287 (interpolation), mia/3d/image.cc:347-477
@@ -28,18 +35,27 @@
 namespace mia_hip {
 
 constexpr int kBlock = 256;
-constexpr int kWarpTileX = 64;   // warp kernel tile: 64 (x) by 16 (y) voxels
-constexpr int kWarpTileY = 16;
-constexpr int kMaxRow = 512;     // longest x row the stage-1 kernel keeps in LDS
+constexpr int kMaxRow = 512;       // longest x row the stage-1 kernel keeps in LDS
+// 3D warp tile and LDS budgets
+constexpr int kW3TX = 32, kW3TY = 8, kW3TZ = 8;
+// The coefficient window (phase 1-2) and the box of moving-image taps
+// (phase 3-4) share one LDS buffer; 9984 floats keep a block under 40 KiB so
+// four blocks (16 waves) fit a CU.
+constexpr int kW3Buf = 9984;
+constexpr int kW3CWin = kW3Buf / 4;                 // coefficient window entries (float4)
+constexpr int kW3Region = kW3Buf;                   // tap box (floats)
+// 2D warp tile and LDS budgets
+constexpr int kW2TX = 64, kW2TY = 32;
+constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2
+constexpr int kW2Region = 8192;                     // floats
 
 // Per-instance description, identical on host and device.
 struct InstDesc {
     int nx, ny, nz;           // image size (nz = 1 in 2D)
     int cx, cy, cz;           // coefficient grid (cz = 1 in 2D)
-    int ncy, ncz;             // number of y / z knot bands (max start index + 1)
+    int ncx, ncy, ncz;        // number of x / y / z knot bands (max start index + 1)
     long long img_off;        // offset of the instance's images (floats)
     long long par_off;        // offset of its parameters / gradient (doubles)
-    long long coef_off;       // offset of its coefficient vectors (float4 / float2)
     long long part_off;       // offset of its stage-1 partials (floats)
     int tab_x, tab_y, tab_z;  // offsets into the axis tables (one entry per voxel)
     int cell_x, cell_y, cell_z; // offsets into the band tables (csize + 1 entries)
@@ -80,6 +96,14 @@ __device__ __forceinline__ int fastfloor16(float v)
     return (int)floorf(rintf(v * 65536.0f) * (1.0f / 65536.0f));
 }
 
+// Sample positions are kept in [-1e6, 1e6]: the float -> int conversion and
+// the mirror arithmetic stay in range for any input (NaN maps to the lower
+// bound because fmaxf drops a NaN operand).
+__device__ __forceinline__ float clamp_pos(float p)
+{
+    return fminf(fmaxf(p, -1.0e6f), 1.0e6f);
+}
+
 // mirror boundary index map (mia/core/splinebc/bc.cc:52-64)
 __device__ __forceinline__ int mirror_idx(int i, int n)
 {
@@ -99,9 +123,6 @@ __device__ __forceinline__ int mirror_idx(int i, int n)
 template <int DEG, bool GRAD>
 __device__ __forceinline__ int kweights(float p, float *w, float *dw)
 {
-    // Keep the float -> int conversion and the mirror arithmetic in range for
-    // any input (NaN and +-inf map to the bounds; fmaxf drops a NaN operand).
-    p = fminf(fmaxf(p, -1.0e6f), 1.0e6f);
     const int fl = fastfloor16(p);
     const float f = p - (float)fl;
     if constexpr (DEG == 3) {
@@ -143,8 +164,15 @@ __device__ __forceinline__ void axis_index(int s, int n, int *idx)
     }
 }
 
-// Interpolation of the prefiltered image at p with clamp to [vmin, vmax]
-// (mia/3d/interpolator.cxx:250-287, z->y->x contraction as add_3d :187-210).
+__device__ __forceinline__ float clamp_val(float v, float vmin, float vmax)
+{
+    return v >= vmin ? (v <= vmax ? v : vmax) : vmin;
+}
+
+// Interpolation of the prefiltered image at p from global memory with mirror
+// indices and clamp to [vmin, vmax] (mia/3d/interpolator.cxx:250-287,
+// z->y->x contraction as add_3d :187-210); GRAD adds the analytic gradient,
+// zero where the value is clamped.
 template <int DEG, bool GRAD>
 __device__ __forceinline__ float interp3(const float *__restrict__ M, int nx, int ny, int nz, float px,
                                          float py, float pz, float vmin, float vmax, float *g)
@@ -193,7 +221,7 @@ __device__ __forceinline__ float interp3(const float *__restrict__ M, int nx, in
         g[1] = clamped ? 0.0f : gy;
         g[2] = clamped ? 0.0f : gz;
     }
-    return v >= vmin ? (v <= vmax ? v : vmax) : vmin;
+    return clamp_val(v, vmin, vmax);
 }
 
 // 2D interpolation (mia/2d/interpolator.cxx:277-311)
@@ -231,7 +259,110 @@ __device__ __forceinline__ float interp2(const float *__restrict__ M, int nx, in
         g[0] = clamped ? 0.0f : gx;
         g[1] = clamped ? 0.0f : gy;
     }
-    return v >= vmin ? (v <= vmax ? v : vmax) : vmin;
+    return clamp_val(v, vmin, vmax);
+}
+
+// The same contractions over an LDS box that already holds the mirrored taps
+// (local start indices lx/ly/lz inside an RX x RY x . box); same arithmetic
+// order as the global versions.
+template <int DEG>
+__device__ __forceinline__ float contract_lds3(const float *R, int RX, int RY, int lx, int ly, int lz,
+                                               const float *wx, const float *wy, const float *wz)
+{
+    constexpr int NS = DEG == 3 ? 4 : 2;
+    float v = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        const float *slab = R + (lz + a) * RX * RY;
+        float vy = 0.0f;
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            const float *row = slab + (ly + b) * RX + lx;
+            float vx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                vx = fmaf(wx[k], row[k], vx);
+            vy = fmaf(wy[b], vx, vy);
+        }
+        v = fmaf(wz[a], vy, v);
+    }
+    return v;
+}
+
+template <int DEG>
+__device__ __forceinline__ float contract_lds2(const float *R, int RX, int lx, int ly, const float *wx,
+                                               const float *wy)
+{
+    constexpr int NS = DEG == 3 ? 4 : 2;
+    float v = 0.0f;
+#pragma unroll
+    for (int b = 0; b < NS; ++b) {
+        const float *row = R + (ly + b) * RX + lx;
+        float vx = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            vx = fmaf(wx[k], row[k], vx);
+        v = fmaf(wy[b], vx, v);
+    }
+    return v;
+}
+
+// Fill an RX x RY x RZ box of LDS with the image values at (rx0 + ix, ...),
+// mirrored at the image borders.  Each thread walks the flat box index in
+// steps of the block size, updating (ix, iy, iz) incrementally: the step is
+// split into (kBlock % RX, row step % RY, row step / RY), so each coordinate
+// needs at most one carry per step, for any box shape.
+__device__ __forceinline__ void fill_box3(float *box, const float *__restrict__ img, int nx, int ny, int nz,
+                                          int rx0, int ry0, int rz0, int RX, int RY, int RZ)
+{
+    const int vol = RX * RY * RZ;
+    const int stx = kBlock % RX, str = kBlock / RX; // step: str rows and stx columns
+    int ix = threadIdx.x % RX;
+    int row = threadIdx.x / RX;
+    int iy = row % RY, iz = row / RY;
+    const int sty = str % RY, stz = str / RY;
+    const bool xin = rx0 >= 0 && rx0 + RX <= nx;
+    const bool yin = ry0 >= 0 && ry0 + RY <= ny;
+    const bool zin = rz0 >= 0 && rz0 + RZ <= nz;
+    for (int e = threadIdx.x; e < vol; e += kBlock) {
+        const int gx = xin ? rx0 + ix : mirror_idx(rx0 + ix, nx);
+        const int gy = yin ? ry0 + iy : mirror_idx(ry0 + iy, ny);
+        const int gz = zin ? rz0 + iz : mirror_idx(rz0 + iz, nz);
+        box[e] = img[((long long)gz * ny + gy) * nx + gx];
+        ix += stx;
+        iy += sty;
+        iz += stz;
+        if (ix >= RX) {
+            ix -= RX;
+            ++iy;
+        }
+        if (iy >= RY) {
+            iy -= RY;
+            ++iz;
+        }
+    }
+}
+
+__device__ __forceinline__ void fill_box2(float *box, const float *__restrict__ img, int nx, int ny, int rx0,
+                                          int ry0, int RX, int RY)
+{
+    const int vol = RX * RY;
+    const int stx = kBlock % RX, sty = kBlock / RX;
+    int ix = threadIdx.x % RX;
+    int iy = threadIdx.x / RX;
+    const bool xin = rx0 >= 0 && rx0 + RX <= nx;
+    const bool yin = ry0 >= 0 && ry0 + RY <= ny;
+    for (int e = threadIdx.x; e < vol; e += kBlock) {
+        const int gx = xin ? rx0 + ix : mirror_idx(rx0 + ix, nx);
+        const int gy = yin ? ry0 + iy : mirror_idx(ry0 + iy, ny);
+        box[e] = img[(long long)gy * nx + gx];
+        ix += stx;
+        iy += sty;
+        if (ix >= RX) {
+            ix -= RX;
+            ++iy;
+        }
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -250,7 +381,24 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v)
     return v;
 }
 
-// Deterministic block sum (fixed shuffle tree per wave, then waves in order).
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Deterministic block sum (fixed shuffle tree per wave, then waves in order);
+// the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
     v = wave_sum(v);
@@ -281,6 +429,20 @@ __device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned *red)
     return s;
 }
 
+// Spline coefficient k of an instance, converted to float as the reference
+// stores it (C3DFVector / C2DFVector, spline.cc:344-358).
+__device__ __forceinline__ float3 coef3(const double *__restrict__ p, long long k)
+{
+    const double *q = p + 3 * k;
+    return make_float3((float)q[0], (float)q[1], (float)q[2]);
+}
+
+__device__ __forceinline__ float2 coef2(const double *__restrict__ p, long long k)
+{
+    const double *q = p + 2 * k;
+    return make_float2((float)q[0], (float)q[1]);
+}
+
 // ---------------------------------------------------------------------------
 // set-up kernels (once per level): clamp range and prefilter
 // ---------------------------------------------------------------------------
@@ -294,10 +456,8 @@ __global__ void __launch_bounds__(kBlock) k_minmax_partial(const float *__restri
         mn = fminf(mn, v);
         mx = fmaxf(mx, v);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, off, 64));
-        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
     __shared__ float2 red[kBlock / 64];
     if ((threadIdx.x & 63) == 0)
         red[threadIdx.x >> 6] = make_float2(mn, mx);
@@ -373,159 +533,259 @@ __global__ void __launch_bounds__(kBlock) k_prefilter_lines(float *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// per-evaluation kernels
+// warp
 // ---------------------------------------------------------------------------
 
-// double parameters -> float coefficient vectors (the reference stores
-// C3DFVector / C2DFVector coefficients, spline.cc:344-358)
-template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_coef(const double *__restrict__ params, void *__restrict__ coef,
-                                                 long long first, long long count)
-{
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= count)
-        return;
-    const long long k = first + i;
-    if constexpr (DIM == 3) {
-        const double *p = params + 3 * k;
-        ((float4 *)coef)[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.0f);
-    } else {
-        const double *p = params + 2 * k;
-        ((float2 *)coef)[k] = make_float2((float)p[0], (float)p[1]);
-    }
-}
-
-// Warp kernel, 3D.  Tile = 64 x 16 voxels of one z slice; 256 threads, four
-// rows each.  The coefficients are first contracted along z into an LDS
-// window covering the tile's x/y knot range (zero padded beyond the grid),
-// then each voxel contracts 4x4 window entries with its y/x table weights.
+// 3D warp.  Block = 32 x 8 x 8 voxels, thread = one (x, y) column of 8 voxels.
+// Per z slice the tile's coefficient window is contracted along z into LDS
+// (zero beyond the grid), each voxel contracts 4 x 4 window entries with its
+// y / x table weights and keeps its sample position.  The block then reduces
+// the positions to the bounding box of all interpolation taps; if that box
+// fits kW3Region it is loaded once (mirrored) and every tap is an LDS read.
 template <int DEG>
 __global__ void __launch_bounds__(kBlock)
     k_warp3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int boff, int nblocks,
-             const float4 *__restrict__ coef, const float4 *__restrict__ tw, const int *__restrict__ ts,
+             const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
-    constexpr int WX = kWarpTileX + 4 + 4, WY = kWarpTileY + 4 + 4;
-    __shared__ float4 win[WX * WY];
+    constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
+    __shared__ float4 buf4[kW3Buf / 4];
+    __shared__ float red[6][kBlock / 64];
+    __shared__ float4 wz_s[kW3TZ];
+    __shared__ int sz_s[kW3TZ];
+    float4 *cwin = buf4;
+    float *region = (float *)buf4;
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
     const int lb = b - prefix[i];
-    const int tiles_x = (d.nx + kWarpTileX - 1) / kWarpTileX;
-    const int tiles_y = (d.ny + kWarpTileY - 1) / kWarpTileY;
+    const int tiles_x = (d.nx + kW3TX - 1) / kW3TX;
+    const int tiles_y = (d.ny + kW3TY - 1) / kW3TY;
     const int tx = lb % tiles_x;
     const int rest = lb / tiles_x;
-    const int ty = rest % tiles_y;
-    const int z = rest / tiles_y;
-    const int x0 = tx * kWarpTileX, y0 = ty * kWarpTileY;
-    const int x1 = min(x0 + kWarpTileX, d.nx) - 1, y1 = min(y0 + kWarpTileY, d.ny) - 1;
+    const int x0 = tx * kW3TX, y0 = (rest % tiles_y) * kW3TY, z0 = (rest / tiles_y) * kW3TZ;
+    const int x = x0 + (threadIdx.x & (kW3TX - 1)), y = y0 + threadIdx.x / kW3TX;
+    const bool valid = x < d.nx && y < d.ny;
+    const int xc = min(x, d.nx - 1), yc = min(y, d.ny - 1);
+    const int nzl = min(kW3TZ, d.nz - z0);
     const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
-    const int wxn = ts[d.tab_x + x1] + 4 - kx0;
-    const int wyn = ts[d.tab_y + y1] + 4 - ky0;
-    const int sz = ts[d.tab_z + z];
-    const float4 wz = tw[d.tab_z + z];
-    const float wzs[4] = {wz.x, wz.y, wz.z, wz.w};
-    const float4 *cbase = coef + d.coef_off;
-    for (int e = threadIdx.x; e < wxn * wyn; e += kBlock) {
-        const int ky = ky0 + e / wxn, kx = kx0 + e % wxn;
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        if (kx < d.cx && ky < d.cy) {
+    const int wxn = ts[d.tab_x + min(x0 + kW3TX, d.nx) - 1] + 4 - kx0;
+    const int wyn = ts[d.tab_y + min(y0 + kW3TY, d.ny) - 1] + 4 - ky0;
+    const int sx = ts[d.tab_x + xc] - kx0, sy = ts[d.tab_y + yc] - ky0;
+    const float4 wxv = tw[d.tab_x + xc], wyv = tw[d.tab_y + yc];
+    const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
+    const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+    const double *P = params + d.par_off;
+    // phase 1: the tile's coefficient window (x fastest, then y, then z) and
+    // the z table of its slices
+    if ((int)threadIdx.x < nzl) {
+        wz_s[threadIdx.x] = tw[d.tab_z + z0 + threadIdx.x];
+        sz_s[threadIdx.x] = ts[d.tab_z + z0 + threadIdx.x];
+    }
+    const int kz0 = ts[d.tab_z + z0];
+    const int wzn = ts[d.tab_z + z0 + nzl - 1] + 4 - kz0;
+    const bool win_lds = wxn * wyn * wzn <= kW3CWin; // block-uniform
+    if (win_lds) {
+        for (int e = threadIdx.x; e < wxn * wyn * wzn; e += kBlock) {
+            const int kx = kx0 + e % wxn;
+            const int t = e / wxn;
+            const int ky = ky0 + t % wyn, kz = kz0 + t / wyn;
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (kx < d.cx && ky < d.cy && kz < d.cz) {
+                const float3 c = coef3(P, ((long long)kz * d.cy + ky) * d.cx + kx);
+                q = make_float4(c.x, c.y, c.z, 0.0f);
+            }
+            cwin[e] = q;
+        }
+    }
+    __syncthreads();
+    // phase 2: displacement of the thread's 8 voxels, contracted along x, then
+    // y, then z (the order of init_grid, spline.cc:573-650)
+    float ux[kW3TZ], uy[kW3TZ], uz[kW3TZ];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int kz = sz + c;
-                if (kz < d.cz) {
-                    const float4 q = cbase[((long long)kz * d.cy + ky) * d.cx + kx];
-                    ax = fmaf(wzs[c], q.x, ax);
-                    ay = fmaf(wzs[c], q.y, ay);
-                    az = fmaf(wzs[c], q.z, az);
+    for (int zl = 0; zl < kW3TZ; ++zl)
+        ux[zl] = uy[zl] = uz[zl] = 0.0f;
+    for (int kzl = 0; kzl < wzn; ++kzl) {
+        float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+        if (win_lds) {
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+                float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+                const float4 *wr = cwin + (kzl * wyn + sy + bb) * wxn + sx;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const float4 q = wr[a];
+                    hx = fmaf(wxs[a], q.x, hx);
+                    hy = fmaf(wxs[a], q.y, hy);
+                    hz = fmaf(wxs[a], q.z, hz);
+                }
+                vx = fmaf(wys[bb], hx, vx);
+                vy = fmaf(wys[bb], hy, vy);
+                vz = fmaf(wys[bb], hz, vz);
+            }
+        } else {
+            const int kz = kz0 + kzl;
+            if (kz < d.cz) {
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const int ky = ky0 + sy + bb;
+                    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int kx = kx0 + sx + a;
+                        if (kx < d.cx && ky < d.cy) {
+                            const float3 q = coef3(P, ((long long)kz * d.cy + ky) * d.cx + kx);
+                            hx = fmaf(wxs[a], q.x, hx);
+                            hy = fmaf(wxs[a], q.y, hy);
+                            hz = fmaf(wxs[a], q.z, hz);
+                        }
+                    }
+                    vx = fmaf(wys[bb], hx, vx);
+                    vy = fmaf(wys[bb], hy, vy);
+                    vz = fmaf(wys[bb], hz, vz);
                 }
             }
         }
-        win[e] = make_float4(ax, ay, az, 0.0f);
+#pragma unroll
+        for (int zl = 0; zl < kW3TZ; ++zl) {
+            const int c = kz0 + kzl - (zl < nzl ? sz_s[zl] : 0);
+            if (zl < nzl && c >= 0 && c < 4) {
+                const float w = ((const float *)&wz_s[zl])[c];
+                ux[zl] = fmaf(w, vx, ux[zl]);
+                uy[zl] = fmaf(w, vy, uy[zl]);
+                uz[zl] = fmaf(w, vz, uz[zl]);
+            }
+        }
+    }
+    float px[kW3TZ], py[kW3TZ], pz[kW3TZ];
+    float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+#pragma unroll
+    for (int zl = 0; zl < kW3TZ; ++zl) {
+        px[zl] = py[zl] = pz[zl] = 0.0f;
+        if (zl < nzl) {
+            const int z = z0 + zl;
+            px[zl] = clamp_pos((float)x - ux[zl]);
+            py[zl] = clamp_pos((float)y - uy[zl]);
+            pz[zl] = clamp_pos((float)z - uz[zl]);
+            if (valid) {
+                mnx = fminf(mnx, px[zl]);
+                mxx = fmaxf(mxx, px[zl]);
+                mny = fminf(mny, py[zl]);
+                mxy = fmaxf(mxy, py[zl]);
+                mnz = fminf(mnz, pz[zl]);
+                mxz = fmaxf(mxz, pz[zl]);
+            }
+        }
+    }
+    // bounding box of the taps of all valid voxels of the tile
+    mnx = wave_min(mnx);
+    mny = wave_min(mny);
+    mnz = wave_min(mnz);
+    mxx = wave_max(mxx);
+    mxy = wave_max(mxy);
+    mxz = wave_max(mxz);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wid] = mnx;
+        red[1][wid] = mny;
+        red[2][wid] = mnz;
+        red[3][wid] = mxx;
+        red[4][wid] = mxy;
+        red[5][wid] = mxz;
     }
     __syncthreads();
-    const int lx = threadIdx.x & 63;
-    const int x = x0 + lx;
-    if (x >= d.nx)
-        return;
-    const int sx = ts[d.tab_x + x] - kx0;
-    const float4 wxv = tw[d.tab_x + x];
-    const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
-    const float2 cl = clamp[i];
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+        mnx = fminf(mnx, red[0][k]);
+        mny = fminf(mny, red[1][k]);
+        mnz = fminf(mnz, red[2][k]);
+        mxx = fmaxf(mxx, red[3][k]);
+        mxy = fmaxf(mxy, red[4][k]);
+        mxz = fmaxf(mxz, red[5][k]);
+    }
+    const int rx0 = fastfloor16(mnx) - LO, ry0 = fastfloor16(mny) - LO, rz0 = fastfloor16(mnz) - LO;
+    const int RX = fastfloor16(mxx) - LO + NS - rx0;
+    const int RY = fastfloor16(mxy) - LO + NS - ry0;
+    const int RZ = fastfloor16(mxz) - LO + NS - rz0;
+    const bool use_lds = (long long)RX * RY * RZ <= kW3Region;
     const float *Mi = M + d.img_off;
-    float *Wi = W + d.img_off;
-    for (int ly = threadIdx.x >> 6; ly < kWarpTileY; ly += kBlock / 64) {
-        const int y = y0 + ly;
-        if (y >= d.ny)
-            break;
-        const int sy = ts[d.tab_y + y] - ky0;
-        const float4 wyv = tw[d.tab_y + y];
-        const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
-        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (use_lds) {
+        // phase 3: flat fill of the box, mirror indices applied here; the box
+        // coordinates advance incrementally (no division in the loop)
+        fill_box3(region, Mi, d.nx, d.ny, d.nz, rx0, ry0, rz0, RX, RY, RZ);
+        __syncthreads();
+    }
+    if (!valid)
+        return;
+    const float2 cl = clamp[i];
+    float *Wi = W + d.img_off + ((long long)z0 * d.ny + y) * d.nx + x;
+    const long long plane = (long long)d.nx * d.ny;
+    // phase 4 (two separate loops so the LDS one unrolls and overlaps the
+    // latency of consecutive voxels' tap reads)
+    if (use_lds) {
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-            const float4 *wr = win + (sy + bb) * wxn + sx;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float4 q = wr[a];
-                hx = fmaf(wxs[a], q.x, hx);
-                hy = fmaf(wxs[a], q.y, hy);
-                hz = fmaf(wxs[a], q.z, hz);
+        for (int zl = 0; zl < kW3TZ; ++zl) {
+            if (zl < nzl) {
+                float wa[NS], wb[NS], wc[NS];
+                const int s0 = kweights<DEG, false>(px[zl], wa, nullptr);
+                const int s1 = kweights<DEG, false>(py[zl], wb, nullptr);
+                const int s2 = kweights<DEG, false>(pz[zl], wc, nullptr);
+                Wi[zl * plane] = clamp_val(
+                    contract_lds3<DEG>(region, RX, RY, s0 - rx0, s1 - ry0, s2 - rz0, wa, wb, wc), cl.x, cl.y);
             }
-            ux = fmaf(wys[bb], hx, ux);
-            uy = fmaf(wys[bb], hy, uy);
-            uz = fmaf(wys[bb], hz, uz);
         }
-        const float v = interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, (float)x - ux, (float)y - uy, (float)z - uz,
-                                            cl.x, cl.y, nullptr);
-        Wi[((long long)z * d.ny + y) * d.nx + x] = v;
+    } else {
+#pragma unroll
+        for (int zl = 0; zl < kW3TZ; ++zl)
+            if (zl < nzl)
+                Wi[zl * plane] =
+                    interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, px[zl], py[zl], pz[zl], cl.x, cl.y, nullptr);
     }
 }
 
-// Warp kernel, 2D (same tiling, no z contraction).
+// 2D warp.  Block = 64 x 32 pixels, thread = 8 pixels of one x column
+// (rows y0 + ly + 4k).  One coefficient window per tile.
 template <int DEG>
 __global__ void __launch_bounds__(kBlock)
     k_warp2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int boff, int nblocks,
-             const float2 *__restrict__ coef, const float4 *__restrict__ tw, const int *__restrict__ ts,
+             const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
-    constexpr int WX = kWarpTileX + 4 + 4, WY = kWarpTileY + 4 + 4;
-    __shared__ float2 win[WX * WY];
+    constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
+    constexpr int RPT = kW2TY / (kBlock / kW2TX);   // rows per thread (8)
+    __shared__ float2 win[kW2Win];
+    __shared__ float region[kW2Region];
+    __shared__ float red[4][kBlock / 64];
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
     const int lb = b - prefix[i];
-    const int tiles_x = (d.nx + kWarpTileX - 1) / kWarpTileX;
-    const int tx = lb % tiles_x;
-    const int ty = lb / tiles_x;
-    const int x0 = tx * kWarpTileX, y0 = ty * kWarpTileY;
-    const int x1 = min(x0 + kWarpTileX, d.nx) - 1, y1 = min(y0 + kWarpTileY, d.ny) - 1;
+    const int tiles_x = (d.nx + kW2TX - 1) / kW2TX;
+    const int x0 = (lb % tiles_x) * kW2TX, y0 = (lb / tiles_x) * kW2TY;
+    const int x = x0 + (threadIdx.x & (kW2TX - 1));
+    const int ly = threadIdx.x / kW2TX;
+    const bool xvalid = x < d.nx;
+    const int xc = min(x, d.nx - 1);
     const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
-    const int wxn = ts[d.tab_x + x1] + 4 - kx0;
-    const int wyn = ts[d.tab_y + y1] + 4 - ky0;
-    const float2 *cbase = coef + d.coef_off;
+    const int wxn = ts[d.tab_x + min(x0 + kW2TX, d.nx) - 1] + 4 - kx0;
+    const int wyn = ts[d.tab_y + min(y0 + kW2TY, d.ny) - 1] + 4 - ky0;
+    const double *P = params + d.par_off;
     for (int e = threadIdx.x; e < wxn * wyn; e += kBlock) {
         const int ky = ky0 + e / wxn, kx = kx0 + e % wxn;
-        win[e] = (kx < d.cx && ky < d.cy) ? cbase[(long long)ky * d.cx + kx] : make_float2(0.0f, 0.0f);
+        win[e] = (kx < d.cx && ky < d.cy) ? coef2(P, (long long)ky * d.cx + kx) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    const int lx = threadIdx.x & 63;
-    const int x = x0 + lx;
-    if (x >= d.nx)
-        return;
-    const int sx = ts[d.tab_x + x] - kx0;
-    const float4 wxv = tw[d.tab_x + x];
+    const int sx = ts[d.tab_x + xc] - kx0;
+    const float4 wxv = tw[d.tab_x + xc];
     const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
-    const float2 cl = clamp[i];
-    const float *Mi = M + d.img_off;
-    float *Wi = W + d.img_off;
-    for (int ly = threadIdx.x >> 6; ly < kWarpTileY; ly += kBlock / 64) {
-        const int y = y0 + ly;
-        if (y >= d.ny)
-            break;
-        const int sy = ts[d.tab_y + y] - ky0;
-        const float4 wyv = tw[d.tab_y + y];
+    float px[RPT], py[RPT];
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int y = y0 + ly + k * (kBlock / kW2TX);
+        const int yc = min(y, d.ny - 1);
+        const int sy = ts[d.tab_y + yc] - ky0;
+        const float4 wyv = tw[d.tab_y + yc];
         const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
         float ux = 0.0f, uy = 0.0f;
 #pragma unroll
@@ -541,25 +801,158 @@ __global__ void __launch_bounds__(kBlock)
             ux = fmaf(wys[bb], hx, ux);
             uy = fmaf(wys[bb], hy, uy);
         }
-        Wi[(long long)y * d.nx + x] =
-            interp2<DEG, false>(Mi, d.nx, d.ny, (float)x - ux, (float)y - uy, cl.x, cl.y, nullptr);
+        px[k] = clamp_pos((float)x - ux);
+        py[k] = clamp_pos((float)y - uy);
+        if (xvalid && y < d.ny) {
+            mnx = fminf(mnx, px[k]);
+            mxx = fmaxf(mxx, px[k]);
+            mny = fminf(mny, py[k]);
+            mxy = fmaxf(mxy, py[k]);
+        }
+    }
+    mnx = wave_min(mnx);
+    mny = wave_min(mny);
+    mxx = wave_max(mxx);
+    mxy = wave_max(mxy);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wid] = mnx;
+        red[1][wid] = mny;
+        red[2][wid] = mxx;
+        red[3][wid] = mxy;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+        mnx = fminf(mnx, red[0][k]);
+        mny = fminf(mny, red[1][k]);
+        mxx = fmaxf(mxx, red[2][k]);
+        mxy = fmaxf(mxy, red[3][k]);
+    }
+    const int rx0 = fastfloor16(mnx) - LO, ry0 = fastfloor16(mny) - LO;
+    const int RX = fastfloor16(mxx) - LO + NS - rx0;
+    const int RY = fastfloor16(mxy) - LO + NS - ry0;
+    const bool use_lds = (long long)RX * RY <= kW2Region;
+    const float *Mi = M + d.img_off;
+    if (use_lds) {
+        fill_box2(region, Mi, d.nx, d.ny, rx0, ry0, RX, RY);
+        __syncthreads();
+    }
+    if (!xvalid)
+        return;
+    const float2 cl = clamp[i];
+    float *Wi = W + d.img_off;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int y = y0 + ly + k * (kBlock / kW2TX);
+        if (y < d.ny) {
+            float v;
+            if (use_lds) {
+                float wa[NS], wb[NS];
+                const int s0 = kweights<DEG, false>(px[k], wa, nullptr);
+                const int s1 = kweights<DEG, false>(py[k], wb, nullptr);
+                v = clamp_val(contract_lds2<DEG>(region, RX, s0 - rx0, s1 - ry0, wa, wb), cl.x, cl.y);
+            } else {
+                v = interp2<DEG, false>(Mi, d.nx, d.ny, px[k], py[k], cl.x, cl.y, nullptr);
+            }
+            Wi[(long long)y * d.nx + x] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage 1: residual, force, SSD partial, first contraction of the adjoint
+// ---------------------------------------------------------------------------
+
+// Dynamic LDS layout of the stage-1 kernels (offsets in floats):
+//   acc  [nx][4 * DIM + 1]  per-column band sums (t, component); the odd row
+//                           stride keeps the column-parallel stores and the
+//                           cell-parallel loads free of bank conflicts
+//   xw   [nx] float4        x table weights
+//   xs   [nx] int           x table start indices
+//   cells[cx + 1] int       x band table
+//   S    [ncx][4][4 * DIM]  per x band partial sums for each of the 4 weights
+//   win  [4][cx + 4]        exact mode: coefficient window of the band (float4 / float2)
+struct Stage1Layout {
+    int acc_stride;
+    size_t acc, xw, xs, cl, S, win, bytes;
+};
+
+__host__ __device__ inline Stage1Layout stage1_layout(int dim, int nx, int cx, int ncx, bool exact)
+{
+    Stage1Layout L;
+    const int T = 4 * dim;
+    L.acc_stride = T + 1;
+    size_t f = 0;
+    L.acc = f;
+    f += (size_t)nx * L.acc_stride;
+    f = (f + 3) & ~(size_t)3;
+    L.xw = f;
+    f += (size_t)4 * nx;
+    L.xs = f;
+    f += (size_t)nx;
+    L.cl = f;
+    f += (size_t)(cx + 1);
+    L.S = f;
+    f += (size_t)ncx * 4 * T;
+    f = (f + 3) & ~(size_t)3;
+    L.win = f;
+    if (exact)
+        f += (size_t)4 * (cx + 4) * (dim == 3 ? 4 : 2);
+    L.bytes = f * 4;
+    return L;
+}
+
+// x contraction of the per-column band sums into the block's partial-sum row:
+// P[(t * cx + kx) * DIM + comp] = sum_x wx_{kx - start(x)}(x) * acc[x][t*DIM + comp].
+// Two fixed-order steps: for each x band c and weight index a,
+// S[c][a][tc] = sum over the band's columns in ascending x; then
+// P = sum_{a = 0..3} S[kx - a][a][tc].
+template <int DIM>
+__device__ __forceinline__ void contract_x(const float *acc_s, int AS, const float4 *xw_s, const int *cl_s,
+                                           float *S_s, int ncx, int cx, float *__restrict__ Pout)
+{
+    constexpr int T = 4 * DIM;
+    for (int o2 = threadIdx.x; o2 < ncx * 4 * T; o2 += kBlock) {
+        const int cell = o2 / (4 * T);
+        const int r = o2 - cell * 4 * T;
+        const int a = r / T, tc = r - a * T;
+        const int xb = cl_s[cell], xe = cl_s[cell + 1];
+        float sum = 0.0f;
+        for (int xx = xb; xx < xe; ++xx)
+            sum = fmaf(((const float *)&xw_s[xx])[a], acc_s[AS * xx + tc], sum);
+        S_s[o2] = sum;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < T * cx; o += kBlock) {
+        const int comp = o % DIM;
+        const int q = o / DIM;
+        const int kx = q % cx;
+        const int tc = (q / cx) * DIM + comp;
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int cell = kx - a;
+            if (cell >= 0 && cell < ncx)
+                sum += S_s[(cell * 4 + a) * T + tc];
+        }
+        Pout[o] = sum;
     }
 }
 
 // Stage 1, 3D: block = (z slice, y knot band).  Each thread owns x columns;
-// for every row j of the band it forms the residual d = W - R, the force
-// f = grad * d (reference mode: central differences of W with zero border
-// components, mia/3d/image.cc:347-477; exact mode: analytic gradient of the
-// interpolated moving image at x - u(x)), accumulates d^2 in FP64 and
-// f * wy_t(j) for the band's four y coefficients in registers.  The rows are
-// then contracted along x with the x table weights through LDS and written as
-// partial sums P[z][band][t][kx][c] (t = ky - band).
+// rows of the band are walked with the y neighbours rotated through registers
+// (reference mode).  Force f = grad * d with the reference's central
+// differences of W, zero border components (mia/3d/image.cc:347-477) or, in
+// exact mode, the analytic gradient of the interpolated moving image at
+// x - u(x).  f * wy_t(j) accumulates in registers, the x contraction runs
+// through LDS with the x table staged there.
 template <int DEG, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_stage1_3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int nblocks,
-                const float4 *__restrict__ coef, const float4 *__restrict__ tw, const int *__restrict__ ts,
+                const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
                 const int *__restrict__ cells, const float *__restrict__ M, const float2 *__restrict__ clamp,
-                const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ P,
+                const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ Pout_all,
                 double *__restrict__ ssd_part, unsigned *__restrict__ cnt_part)
 {
     extern __shared__ float smem[];
@@ -572,24 +965,36 @@ __global__ void __launch_bounds__(kBlock)
     const int band = lb % d.ncy;
     const int z = lb / d.ncy;
     const int j0 = cells[d.cell_y + band], j1 = cells[d.cell_y + band + 1];
-    const int nx = d.nx, ny = d.ny, nz = d.nz;
-    float *acc_s = smem;                          // nx * 12 floats
-    float4 *win = (float4 *)(smem + 12 * nx);     // exact mode: 4 x (cx + 4) window
-    const int wxn = d.cx + 4;
+    const int nx = d.nx, ny = d.ny, nz = d.nz, cx = d.cx;
+    const Stage1Layout L = stage1_layout(3, nx, cx, d.ncx, MODE == 1);
+    const int AS = L.acc_stride;
+    float *acc_s = smem + L.acc;
+    float4 *xw_s = (float4 *)(smem + L.xw);
+    int *xs_s = (int *)(smem + L.xs);
+    int *cl_s = (int *)(smem + L.cl);
+    float *S_s = smem + L.S;
+    float4 *win = (float4 *)(smem + L.win);
+    const int wxn = cx + 4;
+    for (int x = threadIdx.x; x < nx; x += kBlock) {
+        xw_s[x] = tw[d.tab_x + x];
+        xs_s[x] = ts[d.tab_x + x];
+    }
+    for (int k = threadIdx.x; k <= cx; k += kBlock)
+        cl_s[k] = cells[d.cell_x + k];
     if constexpr (MODE == 1) {
         const int sz = ts[d.tab_z + z];
         const float4 wz = tw[d.tab_z + z];
         const float wzs[4] = {wz.x, wz.y, wz.z, wz.w};
-        const float4 *cbase = coef + d.coef_off;
+        const double *P = params + d.par_off;
         for (int e = threadIdx.x; e < 4 * wxn; e += kBlock) {
             const int ky = band + e / wxn, kx = e % wxn;
             float ax = 0.0f, ay = 0.0f, az = 0.0f;
-            if (kx < d.cx && ky < d.cy) {
+            if (kx < cx && ky < d.cy) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int kz = sz + c;
                     if (kz < d.cz) {
-                        const float4 q = cbase[((long long)kz * d.cy + ky) * d.cx + kx];
+                        const float3 q = coef3(P, ((long long)kz * d.cy + ky) * cx + kx);
                         ax = fmaf(wzs[c], q.x, ax);
                         ay = fmaf(wzs[c], q.y, ay);
                         az = fmaf(wzs[c], q.z, az);
@@ -598,13 +1003,14 @@ __global__ void __launch_bounds__(kBlock)
             }
             win[e] = make_float4(ax, ay, az, 0.0f);
         }
-        __syncthreads();
     }
+    __syncthreads();
     const long long plane = (long long)nx * ny;
     const float *Wi = W + d.img_off;
     const float *Ri = R + d.img_off;
     const float *Mi = M + d.img_off;
     const float2 cl = clamp[i];
+    const bool iz = z > 0 && z < nz - 1;
     double ssd = 0.0;
     unsigned cnt = 0;
     for (int x = threadIdx.x; x < nx; x += kBlock) {
@@ -612,24 +1018,34 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int k = 0; k < 12; ++k)
             acc[k] = 0.0f;
-        const int sx = ts[d.tab_x + x];
-        const float4 wxv = tw[d.tab_x + x];
+        const int sx = xs_s[x];
+        const float4 wxv = xw_s[x];
         const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
+        const bool ix = x > 0 && x < nx - 1;
+        const int xm = ix ? x - 1 : x, xp = ix ? x + 1 : x;
+        const long long zoff = (long long)z * plane;
+        const long long zm = iz ? zoff - plane : zoff, zp = iz ? zoff + plane : zoff;
+        // rows j-1, j, j+1 of column x (reference mode)
+        float wm = 0.0f, wc = 0.0f, wp = 0.0f;
+        if constexpr (MODE == 0) {
+            wc = Wi[zoff + (long long)j0 * nx + x];
+            wm = Wi[zoff + (long long)max(j0 - 1, 0) * nx + x];
+        }
+#pragma unroll(MODE == 0 ? 4 : 1)
         for (int j = j0; j < j1; ++j) {
-            const long long idx = (long long)z * plane + (long long)j * nx + x;
+            const long long idx = zoff + (long long)j * nx + x;
             const float4 wyv = tw[d.tab_y + j];
             const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
             float w, gx, gy, gz;
             if constexpr (MODE == 0) {
-                // neighbour indices stay inside the image on the border
-                const bool ix = x > 0 && x < nx - 1, iy = j > 0 && j < ny - 1, iz = z > 0 && z < nz - 1;
-                const long long xm = ix ? idx - 1 : idx, xp = ix ? idx + 1 : idx;
-                const long long ym = iy ? idx - nx : idx, yp = iy ? idx + nx : idx;
-                const long long zm = iz ? idx - plane : idx, zp = iz ? idx + plane : idx;
-                w = Wi[idx];
-                gx = ix ? 0.5f * (Wi[xp] - Wi[xm]) : 0.0f;
-                gy = iy ? 0.5f * (Wi[yp] - Wi[ym]) : 0.0f;
-                gz = iz ? 0.5f * (Wi[zp] - Wi[zm]) : 0.0f;
+                const bool iy = j > 0 && j < ny - 1;
+                wp = Wi[zoff + (long long)min(j + 1, ny - 1) * nx + x];
+                w = wc;
+                gx = ix ? 0.5f * (Wi[zoff + (long long)j * nx + xp] - Wi[zoff + (long long)j * nx + xm]) : 0.0f;
+                gy = iy ? 0.5f * (wp - wm) : 0.0f;
+                gz = iz ? 0.5f * (Wi[zp + (long long)j * nx + x] - Wi[zm + (long long)j * nx + x]) : 0.0f;
+                wm = wc;
+                wc = wp;
             } else {
                 float ux = 0.0f, uy = 0.0f, uz = 0.0f;
 #pragma unroll
@@ -648,7 +1064,8 @@ __global__ void __launch_bounds__(kBlock)
                     uz = fmaf(wys[bb], hz, uz);
                 }
                 float g[3];
-                w = interp3<DEG, true>(Mi, nx, ny, nz, (float)x - ux, (float)j - uy, (float)z - uz, cl.x, cl.y, g);
+                w = interp3<DEG, true>(Mi, nx, ny, nz, clamp_pos((float)x - ux), clamp_pos((float)j - uy),
+                                       clamp_pos((float)z - uz), cl.x, cl.y, g);
                 gx = g[0];
                 gy = g[1];
                 gz = g[2];
@@ -668,7 +1085,7 @@ __global__ void __launch_bounds__(kBlock)
         }
 #pragma unroll
         for (int k = 0; k < 12; ++k)
-            acc_s[12 * x + k] = acc[k];
+            acc_s[AS * x + k] = acc[k];
     }
     const double s = block_sum(ssd, red);
     const unsigned c = block_sum_u(cnt, redu);
@@ -677,32 +1094,17 @@ __global__ void __launch_bounds__(kBlock)
         cnt_part[b] = c;
     }
     __syncthreads();
-    // x contraction: output o = (t * cx + kx) * 3 + comp
-    const int cx = d.cx;
-    const float *twf = (const float *)tw;
-    float *Pout = P + d.part_off + ((long long)z * d.ncy + band) * 12 * cx;
-    for (int o = threadIdx.x; o < 12 * cx; o += kBlock) {
-        const int comp = o % 3;
-        const int q = o / 3;
-        const int kx = q % cx;
-        const int t = q / cx;
-        const int xb = cells[d.cell_x + max(kx - 3, 0)], xe = cells[d.cell_x + kx + 1];
-        float sum = 0.0f;
-        for (int x = xb; x < xe; ++x) {
-            const int k = kx - ts[d.tab_x + x];
-            sum = fmaf(twf[4 * (d.tab_x + x) + k], acc_s[12 * x + 3 * t + comp], sum);
-        }
-        Pout[o] = sum;
-    }
+    contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx,
+                  Pout_all + d.part_off + ((long long)z * d.ncy + band) * 12 * cx);
 }
 
 // Stage 1, 2D: block = y knot band.
 template <int DEG, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_stage1_2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int nblocks,
-                const float2 *__restrict__ coef, const float4 *__restrict__ tw, const int *__restrict__ ts,
+                const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
                 const int *__restrict__ cells, const float *__restrict__ M, const float2 *__restrict__ clamp,
-                const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ P,
+                const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ Pout_all,
                 double *__restrict__ ssd_part, unsigned *__restrict__ cnt_part)
 {
     extern __shared__ float smem[];
@@ -713,18 +1115,30 @@ __global__ void __launch_bounds__(kBlock)
     const InstDesc d = inst[i];
     const int band = b - prefix[i];
     const int j0 = cells[d.cell_y + band], j1 = cells[d.cell_y + band + 1];
-    const int nx = d.nx, ny = d.ny;
-    float *acc_s = smem;                       // nx * 8 floats
-    float2 *win = (float2 *)(smem + 8 * nx);   // exact mode: 4 x (cx + 4)
-    const int wxn = d.cx + 4;
+    const int nx = d.nx, ny = d.ny, cx = d.cx;
+    const Stage1Layout L = stage1_layout(2, nx, cx, d.ncx, MODE == 1);
+    const int AS = L.acc_stride;
+    float *acc_s = smem + L.acc;
+    float4 *xw_s = (float4 *)(smem + L.xw);
+    int *xs_s = (int *)(smem + L.xs);
+    int *cl_s = (int *)(smem + L.cl);
+    float *S_s = smem + L.S;
+    float2 *win = (float2 *)(smem + L.win);
+    const int wxn = cx + 4;
+    for (int x = threadIdx.x; x < nx; x += kBlock) {
+        xw_s[x] = tw[d.tab_x + x];
+        xs_s[x] = ts[d.tab_x + x];
+    }
+    for (int k = threadIdx.x; k <= cx; k += kBlock)
+        cl_s[k] = cells[d.cell_x + k];
     if constexpr (MODE == 1) {
-        const float2 *cbase = coef + d.coef_off;
+        const double *P = params + d.par_off;
         for (int e = threadIdx.x; e < 4 * wxn; e += kBlock) {
             const int ky = band + e / wxn, kx = e % wxn;
-            win[e] = (kx < d.cx && ky < d.cy) ? cbase[(long long)ky * d.cx + kx] : make_float2(0.0f, 0.0f);
+            win[e] = (kx < cx && ky < d.cy) ? coef2(P, (long long)ky * cx + kx) : make_float2(0.0f, 0.0f);
         }
-        __syncthreads();
     }
+    __syncthreads();
     const float *Wi = W + d.img_off;
     const float *Ri = R + d.img_off;
     const float *Mi = M + d.img_off;
@@ -736,21 +1150,30 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             acc[k] = 0.0f;
-        const int sx = ts[d.tab_x + x];
-        const float4 wxv = tw[d.tab_x + x];
+        const int sx = xs_s[x];
+        const float4 wxv = xw_s[x];
         const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
+        const bool ix = x > 0 && x < nx - 1;
+        const int xm = ix ? x - 1 : x, xp = ix ? x + 1 : x;
+        float wm = 0.0f, wc = 0.0f, wp = 0.0f;
+        if constexpr (MODE == 0) {
+            wc = Wi[(long long)j0 * nx + x];
+            wm = Wi[(long long)max(j0 - 1, 0) * nx + x];
+        }
+#pragma unroll(MODE == 0 ? 4 : 1)
         for (int j = j0; j < j1; ++j) {
             const long long idx = (long long)j * nx + x;
             const float4 wyv = tw[d.tab_y + j];
             const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
             float w, gx, gy;
             if constexpr (MODE == 0) {
-                const bool ix = x > 0 && x < nx - 1, iy = j > 0 && j < ny - 1;
-                const long long xm = ix ? idx - 1 : idx, xp = ix ? idx + 1 : idx;
-                const long long ym = iy ? idx - nx : idx, yp = iy ? idx + nx : idx;
-                w = Wi[idx];
-                gx = ix ? 0.5f * (Wi[xp] - Wi[xm]) : 0.0f;
-                gy = iy ? 0.5f * (Wi[yp] - Wi[ym]) : 0.0f;
+                const bool iy = j > 0 && j < ny - 1;
+                wp = Wi[(long long)min(j + 1, ny - 1) * nx + x];
+                w = wc;
+                gx = ix ? 0.5f * (Wi[(long long)j * nx + xp] - Wi[(long long)j * nx + xm]) : 0.0f;
+                gy = iy ? 0.5f * (wp - wm) : 0.0f;
+                wm = wc;
+                wc = wp;
             } else {
                 float ux = 0.0f, uy = 0.0f;
 #pragma unroll
@@ -767,7 +1190,8 @@ __global__ void __launch_bounds__(kBlock)
                     uy = fmaf(wys[bb], hy, uy);
                 }
                 float g[2];
-                w = interp2<DEG, true>(Mi, nx, ny, (float)x - ux, (float)j - uy, cl.x, cl.y, g);
+                w = interp2<DEG, true>(Mi, nx, ny, clamp_pos((float)x - ux), clamp_pos((float)j - uy), cl.x, cl.y,
+                                       g);
                 gx = g[0];
                 gy = g[1];
             }
@@ -785,7 +1209,7 @@ __global__ void __launch_bounds__(kBlock)
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            acc_s[8 * x + k] = acc[k];
+            acc_s[AS * x + k] = acc[k];
     }
     const double s = block_sum(ssd, red);
     const unsigned c = block_sum_u(cnt, redu);
@@ -794,34 +1218,24 @@ __global__ void __launch_bounds__(kBlock)
         cnt_part[b] = c;
     }
     __syncthreads();
-    const int cx = d.cx;
-    const float *twf = (const float *)tw;
-    float *Pout = P + d.part_off + (long long)band * 8 * cx;
-    for (int o = threadIdx.x; o < 8 * cx; o += kBlock) {
-        const int comp = o & 1;
-        const int q = o >> 1;
-        const int kx = q % cx;
-        const int t = q / cx;
-        const int xb = cells[d.cell_x + max(kx - 3, 0)], xe = cells[d.cell_x + kx + 1];
-        float sum = 0.0f;
-        for (int x = xb; x < xe; ++x) {
-            const int k = kx - ts[d.tab_x + x];
-            sum = fmaf(twf[4 * (d.tab_x + x) + k], acc_s[8 * x + 2 * t + comp], sum);
-        }
-        Pout[o] = sum;
-    }
+    contract_x<2>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pout_all + d.part_off + (long long)band * 8 * cx);
 }
 
-// Per instance: SSD sum in a fixed order, cost and gradient scale
-// (mia/template/ssd.hh:54-99,125-181; weight: mia/template/fullcost.cxx:36-45).
-__global__ void __launch_bounds__(kBlock)
-    k_finalize(const InstDesc *__restrict__ inst, const int *__restrict__ prefix, const double *__restrict__ ssd_part,
-               const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
-               double *__restrict__ gscale)
+// ---------------------------------------------------------------------------
+// finalize and reduce
+// ---------------------------------------------------------------------------
+
+// Per instance: SSD sum and mask count in a fixed order, cost and gradient
+// scale (mia/template/ssd.hh:54-99,125-181; weight:
+// mia/template/fullcost.cxx:36-45).  Run by thread block `i` of either kernel.
+__device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, const int *__restrict__ prefix,
+                                                  const double *__restrict__ ssd_part,
+                                                  const unsigned *__restrict__ cnt_part, int norm, float thr,
+                                                  double weight, double *__restrict__ costs,
+                                                  double *__restrict__ gscale)
 {
     __shared__ double red[kBlock / 64];
     __shared__ unsigned redu[kBlock / 64];
-    const int i = blockIdx.x;
     const int beg = prefix[i], end = prefix[i + 1];
     double s = 0.0;
     unsigned c = 0;
@@ -832,7 +1246,6 @@ __global__ void __launch_bounds__(kBlock)
     s = block_sum(s, red);
     c = block_sum_u(c, redu);
     if (threadIdx.x == 0) {
-        const InstDesc d = inst[i];
         const double n = (double)d.nx * d.ny * d.nz;
         double cost, sc;
         if (thr == 0.0f) {
@@ -846,81 +1259,102 @@ __global__ void __launch_bounds__(kBlock)
             cost = (double)FLT_MAX;
         }
         costs[i] = weight * cost;
-        gscale[i] = -sc * weight;
+        if (gscale)
+            gscale[i] = -sc * weight;
     }
 }
 
-// Per coefficient: gradient = scale * sum_z wz(z) sum_t P[z][ky - t][t][kx][.]
-// in FP64, fixed order (adjoint of spline.cc:731-791 completed along y and z).
 __global__ void __launch_bounds__(kBlock)
-    k_reduce3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
-               const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
-               const int *__restrict__ cells, const double *__restrict__ gscale, double *__restrict__ grad)
+    k_finalize(const InstDesc *__restrict__ inst, const int *__restrict__ prefix, const double *__restrict__ ssd_part,
+               const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
+               double *__restrict__ gscale)
 {
-    const int b = blockIdx.x;
+    const int i = blockIdx.x;
+    finalize_instance(inst[i], i, prefix, ssd_part, cnt_part, norm, thr, weight, costs, gscale);
+}
+
+// Per coefficient component: gradient = scale * sum_z wz(z) sum_t
+// P[z][ky - t][t][kx][comp] in FP64, fixed order (the adjoint of
+// spline.cc:731-791 completed along y and z).  FUSED: the first n_inst blocks
+// do the finalize work and the scale is -weight * (norm ? 1/N : 1), known
+// without the mask count (no autothresh); otherwise the scale comes from
+// k_finalize.
+template <int DIM, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+    k_reduce(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
+             const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
+             const int *__restrict__ cells, const int *__restrict__ s1_prefix, const double *__restrict__ ssd_part,
+             const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
+             const double *__restrict__ gscale, double *__restrict__ grad)
+{
+    int b = blockIdx.x;
+    if constexpr (FUSED) {
+        if (b < n_inst) {
+            finalize_instance(inst[b], b, s1_prefix, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
+            return;
+        }
+        b -= n_inst;
+    }
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
-    const long long e = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
     const long long ncoef = (long long)d.cx * d.cy * d.cz;
-    if (e >= ncoef)
+    double sc;
+    if constexpr (FUSED)
+        sc = -weight * (norm ? 1.0 / ((double)d.nx * d.ny * d.nz) : 1.0);
+    else
+        sc = gscale[i];
+    if constexpr (DIM == 3) {
+        // 64 outputs per block; wave g sums the g-th quarter of the z support,
+        // the quarters are added in order through LDS
+        __shared__ double part[kBlock / 64][64];
+        const int lane = threadIdx.x & 63, g4 = threadIdx.x >> 6;
+        const long long e3 = (long long)(b - prefix[i]) * 64 + lane;
+        double g = 0.0;
+        if (e3 < 3 * ncoef) {
+            const int comp = (int)(e3 % 3);
+            const long long e = e3 / 3;
+            const int kx = (int)(e % d.cx);
+            const int ky = (int)((e / d.cx) % d.cy);
+            const int kz = (int)(e / ((long long)d.cx * d.cy));
+            const int zb = cells[d.cell_z + max(kz - 3, 0)], ze = cells[d.cell_z + kz + 1];
+            const int len = ze - zb;
+            const int za = zb + (len * g4) / 4, zc = zb + (len * (g4 + 1)) / 4;
+            const float *twf = (const float *)tw;
+            const long long row = 12LL * d.cx;
+            for (int z = za; z < zc; ++z) {
+                const double wz = twf[4 * (d.tab_z + z) + (kz - ts[d.tab_z + z])];
+                double s = 0.0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int band = ky - t;
+                    if (band >= 0 && band < d.ncy)
+                        s += P[d.part_off + ((long long)z * d.ncy + band) * row + (t * d.cx + kx) * 3 + comp];
+                }
+                g += wz * s;
+            }
+        }
+        part[g4][lane] = g;
+        __syncthreads();
+        if (g4 == 0 && e3 < 3 * ncoef)
+            grad[d.par_off + e3] = sc * (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
         return;
-    const int kx = (int)(e % d.cx);
-    const int ky = (int)((e / d.cx) % d.cy);
-    const int kz = (int)(e / ((long long)d.cx * d.cy));
-    const int zb = cells[d.cell_z + max(kz - 3, 0)], ze = cells[d.cell_z + kz + 1];
-    const float *twf = (const float *)tw;
-    const long long row = 12LL * d.cx;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int z = zb; z < ze; ++z) {
-        const double wz = twf[4 * (d.tab_z + z) + (kz - ts[d.tab_z + z])];
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    } else {
+        const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
+        if (e3 >= 2 * ncoef)
+            return;
+        const int comp = (int)(e3 % 2);
+        const long long e = e3 / 2;
+        const int kx = (int)(e % d.cx);
+        const int ky = (int)(e / d.cx);
+        double g = 0.0;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int band = ky - t;
-            if (band < 0 || band >= d.ncy)
-                continue;
-            const float *p = P + d.part_off + ((long long)z * d.ncy + band) * row + (t * d.cx + kx) * 3;
-            s0 += p[0];
-            s1 += p[1];
-            s2 += p[2];
+            if (band >= 0 && band < d.ncy)
+                g += P[d.part_off + (long long)band * 8 * d.cx + (t * d.cx + kx) * 2 + comp];
         }
-        g0 += wz * s0;
-        g1 += wz * s1;
-        g2 += wz * s2;
+        grad[d.par_off + e3] = sc * g;
     }
-    const double sc = gscale[i];
-    double *out = grad + d.par_off + 3 * e;
-    out[0] = sc * g0;
-    out[1] = sc * g1;
-    out[2] = sc * g2;
-}
-
-__global__ void __launch_bounds__(kBlock)
-    k_reduce2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
-               const float *__restrict__ P, const double *__restrict__ gscale, double *__restrict__ grad)
-{
-    const int b = blockIdx.x;
-    const int i = find_inst(prefix, n_inst, b);
-    const InstDesc d = inst[i];
-    const long long e = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
-    if (e >= (long long)d.cx * d.cy)
-        return;
-    const int kx = (int)(e % d.cx);
-    const int ky = (int)(e / d.cx);
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int band = ky - t;
-        if (band < 0 || band >= d.ncy)
-            continue;
-        const float *p = P + d.part_off + (long long)band * 8 * d.cx + (t * d.cx + kx) * 2;
-        s0 += p[0];
-        s1 += p[1];
-    }
-    const double sc = gscale[i];
-    double *out = grad + d.par_off + 2 * e;
-    out[0] = sc * s0;
-    out[1] = sc * s1;
 }
 
 } // namespace mia_hip

@@ -38,8 +38,8 @@ int fail(int code, const std::string &msg)
             return fail(MIA_HIP_ERR_DEVICE, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Stage { ST_COEF = 0, ST_WARP, ST_STAGE1, ST_FINALIZE, ST_REDUCE, ST_COUNT };
-const char *kStageNames[ST_COUNT] = {"coef", "warp", "ssd_force_adjoint1", "finalize", "adjoint_reduce"};
+enum Stage { ST_WARP = 0, ST_STAGE1, ST_FINALIZE, ST_REDUCE, ST_COUNT };
+const char *kStageNames[ST_COUNT] = {"warp", "ssd_force_adjoint1", "finalize", "adjoint_reduce"};
 
 // Spline axis table with the reference's semantics
 // (mia/3d/transform/spline.cc:237-273, mia/2d/transform/spline.cc:181-200):
@@ -120,14 +120,13 @@ struct mia_hip_ctx {
     std::vector<InstDesc> inst;
     std::vector<int> warp_prefix, s1_prefix, red_prefix;
     std::vector<char> images_set;
-    long long n_img = 0, n_par = 0, n_coef = 0, n_part = 0, max_vox = 0;
-    size_t s1_lds = 0;
+    long long n_img = 0, n_par = 0, n_part = 0, max_vox = 0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0;
 
     InstDesc *d_inst = nullptr;
     int *d_warp_prefix = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr;
     float *d_M = nullptr, *d_R = nullptr, *d_W = nullptr, *d_P = nullptr;
     float2 *d_clamp = nullptr, *d_mm = nullptr;
-    void *d_coef = nullptr;
     float4 *d_tw = nullptr;
     int *d_ts = nullptr, *d_cells = nullptr;
     double *d_ssd = nullptr, *d_gscale = nullptr, *d_params = nullptr, *d_grad = nullptr, *d_costs = nullptr;
@@ -154,7 +153,7 @@ namespace {
 void free_buffers(mia_hip_ctx *c)
 {
     void *ptrs[] = {c->d_inst, c->d_warp_prefix, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
-                    c->d_P, c->d_clamp, c->d_mm, c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
+                    c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt};
     for (void *p : ptrs)
         if (p)
@@ -173,7 +172,6 @@ void free_buffers(mia_hip_ctx *c)
     c->d_warp_prefix = c->d_s1_prefix = c->d_red_prefix = nullptr;
     c->d_M = c->d_R = c->d_W = c->d_P = nullptr;
     c->d_clamp = c->d_mm = nullptr;
-    c->d_coef = nullptr;
     c->d_tw = nullptr;
     c->d_ts = c->d_cells = nullptr;
     c->d_ssd = c->d_gscale = c->d_params = c->d_grad = c->d_costs = c->d_scratch = nullptr;
@@ -225,56 +223,59 @@ int dev_alloc(T **p, size_t count)
     return MIA_HIP_OK;
 }
 
-// Enqueue coefficient conversion, warp (reference mode), stage 1, finalize and
+size_t stage1_lds(const mia_hip_ctx *c, bool exact)
+{
+    return stage1_layout(c->dim, c->max_nx, c->max_cx, c->max_ncx, exact).bytes;
+}
+
+int launch_warp(mia_hip_ctx *c, const double *d_params, int boff, int nb)
+{
+    if (c->dim == 3) {
+        auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff,
+                           nb, d_params, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+    } else {
+        auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff,
+                           nb, d_params, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+    }
+    LAUNCH_CHECK("warp");
+    return MIA_HIP_OK;
+}
+
+// Enqueue warp (reference mode), stage 1, finalize (only with autothresh) and
 // reduce for all instances.  No memsets: every output element is written.
 int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
 {
     const int n = c->n_inst;
-    {
-        StageTimer t(c, ST_COEF, timed);
-        const long long blocks = (c->n_coef + kBlock - 1) / kBlock;
-        if (c->dim == 3)
-            hipLaunchKernelGGL(k_coef<3>, dim3(blocks), dim3(kBlock), 0, c->stream, d_params, c->d_coef, 0LL,
-                               c->n_coef);
-        else
-            hipLaunchKernelGGL(k_coef<2>, dim3(blocks), dim3(kBlock), 0, c->stream, d_params, c->d_coef, 0LL,
-                               c->n_coef);
-        LAUNCH_CHECK("coef");
-    }
-    if (c->grad_mode == MIA_HIP_GRAD_REFERENCE) {
+    const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
+    if (!exact) {
         StageTimer t(c, ST_WARP, timed);
-        const int nb = c->warp_prefix[n];
-        if (c->dim == 3) {
-            auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_warp_prefix, 0, nb,
-                               (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
-        } else {
-            auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_warp_prefix, 0, nb,
-                               (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
-        }
-        LAUNCH_CHECK("warp");
+        int st = launch_warp(c, d_params, 0, c->warp_prefix[n]);
+        if (st)
+            return st;
     }
     {
         StageTimer t(c, ST_STAGE1, timed);
         const int nb = c->s1_prefix[n];
-        const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
+        const size_t lds = stage1_lds(c, exact);
         if (c->dim == 3) {
             auto kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
                                        : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), c->s1_lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
-                               (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W,
-                               c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
+                               d_params, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W, c->d_R, c->thr,
+                               c->d_P, c->d_ssd, c->d_cnt);
         } else {
             auto kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
                                        : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), c->s1_lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
-                               (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W,
-                               c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
+                               d_params, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W, c->d_R, c->thr,
+                               c->d_P, c->d_ssd, c->d_cnt);
         }
         LAUNCH_CHECK("stage1");
     }
-    {
+    const bool fused = c->thr == 0.0f;
+    if (!fused) {
         StageTimer t(c, ST_FINALIZE, timed);
         hipLaunchKernelGGL(k_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->d_inst, c->d_s1_prefix, c->d_ssd,
                            c->d_cnt, c->norm, c->thr, c->weight, d_costs, c->d_gscale);
@@ -282,13 +283,15 @@ int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double 
     }
     {
         StageTimer t(c, ST_REDUCE, timed);
-        const int nb = c->red_prefix[n];
+        const int nb = c->red_prefix[n] + (fused ? n : 0);
+        decltype(&k_reduce<3, true>) kern;
         if (c->dim == 3)
-            hipLaunchKernelGGL(k_reduce3d, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix,
-                               c->d_P, c->d_tw, c->d_ts, c->d_cells, c->d_gscale, d_grad);
+            kern = fused ? k_reduce<3, true> : k_reduce<3, false>;
         else
-            hipLaunchKernelGGL(k_reduce2d, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix,
-                               c->d_P, c->d_gscale, d_grad);
+            kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix, c->d_P, c->d_tw,
+                           c->d_ts, c->d_cells, c->d_s1_prefix, c->d_ssd, c->d_cnt, c->norm, c->thr, c->weight,
+                           d_costs, c->d_gscale, d_grad);
         LAUNCH_CHECK("reduce");
     }
     return MIA_HIP_OK;
@@ -441,8 +444,8 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->images_set.assign(n_inst, 0);
     std::vector<float> tw;
     std::vector<int> ts, cells;
-    long long img = 0, par = 0, coef = 0, part = 0, maxv = 0;
-    int max_nx = 0, max_cx = 0;
+    long long img = 0, par = 0, part = 0, maxv = 0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0;
     for (int i = 0; i < n_inst; ++i) {
         InstDesc &d = c->inst[i];
         d.nx = (int)sizes[3 * i];
@@ -471,6 +474,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
             az.cells = {0, 1};
             az.nbands = 1;
         }
+        d.ncx = ax.nbands;
         d.ncy = ay.nbands;
         d.ncz = az.nbands;
         AxisTable *axes[3] = {&ax, &ay, &az};
@@ -489,31 +493,50 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         img += align64(nvox);
         d.par_off = par;
         par += dim * ncoef;
-        d.coef_off = coef;
-        coef += ncoef;
         d.part_off = part;
         part += dim == 3 ? (long long)d.nz * d.ncy * 12 * d.cx : (long long)d.ncy * 8 * d.cx;
         maxv = std::max(maxv, nvox);
         max_nx = std::max(max_nx, d.nx);
         max_cx = std::max(max_cx, d.cx);
-        const int tiles = ((d.nx + kWarpTileX - 1) / kWarpTileX) * ((d.ny + kWarpTileY - 1) / kWarpTileY) * d.nz;
+        max_ncx = std::max(max_ncx, d.ncx);
+        const int tiles = dim == 3 ? ((d.nx + kW3TX - 1) / kW3TX) * ((d.ny + kW3TY - 1) / kW3TY) *
+                                         ((d.nz + kW3TZ - 1) / kW3TZ)
+                                   : ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
         c->warp_prefix[i + 1] = c->warp_prefix[i] + tiles;
         c->s1_prefix[i + 1] = c->s1_prefix[i] + (dim == 3 ? d.nz * d.ncy : d.ncy);
-        c->red_prefix[i + 1] = c->red_prefix[i] + (int)((ncoef + kBlock - 1) / kBlock);
+        // 3D reduce: 64 gradient components per block (4 waves split the z
+        // support); 2D: 256 per block
+        const int per_block = dim == 3 ? 64 : kBlock;
+        c->red_prefix[i + 1] = c->red_prefix[i] + (int)((dim * ncoef + per_block - 1) / per_block);
     }
     c->n_img = img;
     c->n_par = par;
-    c->n_coef = coef;
     c->n_part = part;
     c->max_vox = maxv;
-    c->s1_lds = dim == 3 ? (size_t)(12 * max_nx) * sizeof(float) + (size_t)4 * (max_cx + 4) * sizeof(float4)
-                         : (size_t)(8 * max_nx) * sizeof(float) + (size_t)4 * (max_cx + 4) * sizeof(float2);
+    c->max_nx = max_nx;
+    c->max_cx = max_cx;
+    c->max_ncx = max_ncx;
+    // stage 1 keeps one x row (plus tables and, in exact mode, a coefficient
+    // window) in dynamic LDS; allow more than the 64 KiB default where needed
+    for (int exact = 0; exact < 2; ++exact) {
+        const size_t lds = stage1_layout(dim, max_nx, max_cx, max_ncx, exact != 0).bytes;
+        if (lds > 160 * 1024)
+            return fail(MIA_HIP_ERR_LIMIT, "stage-1 LDS requirement exceeds 160 KiB");
+        const void *fn[2][2] = {{(const void *)k_stage1_3d<3, 0>, (const void *)k_stage1_3d<1, 0>},
+                                {(const void *)k_stage1_3d<3, 1>, (const void *)k_stage1_3d<1, 1>}};
+        const void *fn2[2][2] = {{(const void *)k_stage1_2d<3, 0>, (const void *)k_stage1_2d<1, 0>},
+                                 {(const void *)k_stage1_2d<3, 1>, (const void *)k_stage1_2d<1, 1>}};
+        if (lds > 64 * 1024)
+            for (int k = 0; k < 2; ++k)
+                HIP_TRY(hipFuncSetAttribute(dim == 3 ? fn[exact][k] : fn2[exact][k],
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     int st;
     if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_warp_prefix, n_inst + 1)) ||
         (st = dev_alloc(&c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_red_prefix, n_inst + 1)) ||
         (st = dev_alloc(&c->d_M, img)) || (st = dev_alloc(&c->d_R, img)) || (st = dev_alloc(&c->d_W, img)) ||
         (st = dev_alloc(&c->d_P, part)) || (st = dev_alloc(&c->d_clamp, n_inst)) || (st = dev_alloc(&c->d_mm, 1024)) ||
-        (st = dev_alloc((float4 **)&c->d_coef, coef)) || (st = dev_alloc(&c->d_tw, tw.size() / 4)) ||
+        (st = dev_alloc(&c->d_tw, tw.size() / 4)) ||
         (st = dev_alloc(&c->d_ts, ts.size())) || (st = dev_alloc(&c->d_cells, cells.size())) ||
         (st = dev_alloc(&c->d_ssd, c->s1_prefix[n_inst])) || (st = dev_alloc(&c->d_cnt, c->s1_prefix[n_inst])) ||
         (st = dev_alloc(&c->d_gscale, n_inst)) || (st = dev_alloc(&c->d_params, par)) ||
@@ -670,26 +693,10 @@ int mia_hip_warp(mia_hip_ctx *c, int inst, const double *params_inst, float *out
     std::memcpy(c->h_params + d.par_off, params_inst, sizeof(double) * c->dim * ncoef);
     HIP_TRY(hipMemcpyAsync(c->d_params + d.par_off, c->h_params + d.par_off, sizeof(double) * c->dim * ncoef,
                            hipMemcpyHostToDevice, c->stream));
-    const long long blocks = (ncoef + kBlock - 1) / kBlock;
-    if (c->dim == 3)
-        hipLaunchKernelGGL(k_coef<3>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_params, c->d_coef, d.coef_off,
-                           ncoef);
-    else
-        hipLaunchKernelGGL(k_coef<2>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_params, c->d_coef, d.coef_off,
-                           ncoef);
-    LAUNCH_CHECK("coef");
     const int boff = c->warp_prefix[inst];
-    const int nb = c->warp_prefix[inst + 1] - boff;
-    if (c->dim == 3) {
-        auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff, nb,
-                           (const float4 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
-    } else {
-        auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff, nb,
-                           (const float2 *)c->d_coef, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
-    }
-    LAUNCH_CHECK("warp");
+    int st = launch_warp(c, c->d_params, boff, c->warp_prefix[inst + 1] - boff);
+    if (st)
+        return st;
     HIP_TRY(hipMemcpyAsync(out, c->d_W + d.img_off, sizeof(float) * (size_t)d.nx * d.ny * d.nz, hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -85,6 +85,27 @@ def test_warp_vs_oracle(mia, shape, rate, degree):
     assert np.abs(W - W_ref).max() < WARP_ATOL * (mov.max() - mov.min())
 
 
+@pytest.mark.parametrize("shape,rate,sd", [((40, 44, 48), 8.0, 12.0), ((120, 150), 16.0, 25.0),
+                                           ((33, 41, 37), 4.0, 0.2), ((90, 77), 3.0, 0.3)])
+def test_warp_large_and_small_displacement(mia, shape, rate, sd):
+    """Tiles whose tap bounding box exceeds the LDS budget (|u| up to ~30
+    voxels) take the global-gather path; tiny displacements keep every tile in
+    LDS.  Both must match the oracle, including mirrored taps far outside."""
+    mov, fix, cs, _ = _inputs(shape, rate, 8)
+    from mia_amd import synth
+    p = synth.random_spline_params(cs, sd, 3 * sd, 9)
+    with _ctx(mia, [shape], [cs]) as c:
+        c.set_images(0, mov, fix)
+        W = c.warp(0, p)
+        costs, g = c.cost_grad(p)
+    u = O.displacement(shape, cs, p)
+    W_ref = O.warp(mov, u, 3)
+    assert np.abs(W - W_ref).max() < 5e-5
+    c_ref, g_ref = O.cost_grad(mov, fix, cs, p)
+    assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
+    assert _rel_inf(g, g_ref) < 1e-4
+
+
 def test_shift_warp_known_answer(mia, kat):
     """mia/2d/transform/test_spline.cc:497-545 through the GPU warp."""
     k = kat["spline_shift_warp_2d"]
