@@ -79,6 +79,33 @@ def cpu_baseline(mov, fix, cs, params, threads):
     return statistics.median(times), t1, cost, g.copy(), tm.copy()
 
 
+def series_2d(n_frames, minimizer):
+    """Batched 2D series: n_frames frames with sides uniform in [96, 224],
+    spline rate 8, ssd, registered from the identity.  Timed: the optimisation
+    loop with one captured graph per iteration for all frames, against one
+    context per frame with eager launches (one launch sequence per frame per
+    evaluation).  Equal final SSD is checked bitwise."""
+    from mia_amd import series, synth
+    frames = synth.series_frames_2d(n_frames, (96, 224), 8.0, 4.0, seed=4)
+    px = sum(f[0].size for f in frames)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        s.run(minimizer.replace("maxiter=100", "maxiter=3"), series.BATCHED_GRAPH)  # warm-up
+        pb, cb, eb, tb = s.run(minimizer, series.BATCHED_GRAPH)
+        pe, ce, ee, te = s.run(minimizer, series.BATCHED)
+        pf, cf, ef, tf = s.run(minimizer, series.PER_FRAME)
+    log("series: batched graph %.4f s, batched eager %.4f s, per frame %.4f s" % (tb, te, tf))
+    equal = bool(np.array_equal(cb, cf) and np.array_equal(pb, pf) and np.array_equal(cb, ce))
+    return {"frames": n_frames, "pixels": int(px), "minimizer": minimizer,
+            "regs_per_sec": round(n_frames / tb, 2),
+            "regs_per_sec_batched_eager": round(n_frames / te, 2),
+            "regs_per_sec_per_frame": round(n_frames / tf, 2),
+            "speedup_vs_per_frame": round(tf / tb, 2),
+            "equal_final_ssd": equal,
+            "final_cost_mean": float(np.mean(cb)),
+            "evaluations_mean": float(np.mean(eb)),
+            "seconds_batched_graph": round(tb, 5), "seconds_per_frame": round(tf, 5)}
+
+
 def copy_calibration(mib, reps):
     """Device-to-device copies of `mib` MiB (the HIP runtime's blit kernel): a
     transfer of known size against which rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -110,6 +137,8 @@ def main():
     ap.add_argument("--grad-mode", choices=["reference", "exact"], default="reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-series", action="store_true", help="skip the batched 2D series measurement")
+    ap.add_argument("--series-frames", type=int, default=64)
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -211,6 +240,10 @@ def main():
             parity = {"cost_rel": float(abs(gpu_cost0 - c_cpu) / abs(c_cpu)),
                       "grad_rel_inf": float(np.abs(gpu_grad0 - g_cpu).max() / np.abs(g_cpu).max())}
 
+    ser = None
+    if not args.no_series:
+        ser = series_2d(args.series_frames, "lbfgs:maxiter=100")
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -233,6 +266,7 @@ def main():
         "kernels": kernels,
         "kernel_time_us_per_step": round(tsum, 2),
         "parity_vs_cpu_port": parity,
+        "series_2d": ser,
     }
     for c in ctxs:
         c.close()

@@ -1,0 +1,62 @@
+/*
+ * mia_host.h -- C ABI of the host-side registration layer (libmia_host.so)
+ * built on libmia_hip: batched series registration of independent frames,
+ * the MI355X counterpart of `mia-2dmany2one-nonrigid`'s pfor over frames
+ * (src/2dmany2one-nonrigid.cc:102-118,185) with the nonrigid driver's
+ * single-level loop (mia/template/nonrigidregister.cxx:192-296).
+ *
+ * Status codes are those of mia_hip.h; mia_host_last_error() gives the
+ * message of the last failure on the calling thread.
+ */
+#ifndef MIA_HOST_H
+#define MIA_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* run modes */
+#define MIA_HOST_BATCHED_GRAPH 0 /* one context, one captured graph per iteration for all frames */
+#define MIA_HOST_BATCHED 1       /* one context, eager launches per iteration for all frames */
+#define MIA_HOST_PER_FRAME 2     /* one context per frame, frames registered one after the other */
+
+typedef struct mia_host_series mia_host_series;
+
+const char *mia_host_last_error(void);
+
+/* n_frames independent registrations of dimension dim (2 or 3), frame sizes
+ * sizes[3*i..] (x,y,z), transformation `spline:rate=<rate>` (cubic kernel),
+ * image interpolation bspline:d=<degree>, cost image:cost=ssd (norm=false).
+ * Coefficient grids follow the reference's formula
+ * (mia/2d/transform/spline.cc:103-116, mia/3d/transform/spline.cc:130-143). */
+int mia_host_series_create(int device, int dim, int n_frames, const uint32_t *sizes, float rate, int degree,
+                           mia_host_series **out);
+
+/* Frame i's moving (src) and fixed (ref) images.  Both are mapped together to
+ * mean 0 and standard deviation 1 before registration, as the reference's
+ * FScaleFilterCreator does (mia/template/nonrigidregister.cxx:147-184,
+ * 204-210). */
+int mia_host_series_set_frame(mia_host_series *s, int frame, const float *src, const float *ref);
+
+/* Number of parameters of frame i and its offset in the concatenated vector. */
+int mia_host_series_n_params(mia_host_series *s, int frame, size_t *n, size_t *offset);
+
+/* Register every frame from the identity with the minimizer `minimizer`
+ * ("lbfgs:maxiter=100[,memory=10,gtol=0,ftolr=0,step=1]" or MIA's
+ * "gdsq:maxiter=100[,step=0.1,scale=2,xtola=0,gtola=0,ftolr=0]") in run mode
+ * `mode`.  Outputs: params_out (concatenated best parameters), costs_out
+ * (best cost per frame), evals_out (evaluations per frame), seconds (wall time
+ * of the optimisation loop, set-up excluded).  Any output may be NULL. */
+int mia_host_series_run(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
+                        int *evals_out, double *seconds);
+
+int mia_host_series_destroy(mia_host_series *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -903,6 +903,12 @@ __host__ __device__ inline Stage1Layout stage1_layout(int dim, int nx, int cx, i
     return L;
 }
 
+// rows of a band walked per iteration of the stage-1 loop: unrolled in
+// reference mode (independent loads of consecutive rows in flight), not in
+// exact mode (the interpolation body is large)
+template <int MODE>
+constexpr int kRowUnroll = MODE == 0 ? 4 : 1;
+
 // x contraction of the per-column band sums into the block's partial-sum row:
 // P[(t * cx + kx) * DIM + comp] = sum_x wx_{kx - start(x)}(x) * acc[x][t*DIM + comp].
 // Two fixed-order steps: for each x band c and weight index a,
@@ -1031,7 +1037,7 @@ __global__ void __launch_bounds__(kBlock)
             wc = Wi[zoff + (long long)j0 * nx + x];
             wm = Wi[zoff + (long long)max(j0 - 1, 0) * nx + x];
         }
-#pragma unroll(MODE == 0 ? 4 : 1)
+#pragma unroll kRowUnroll<MODE>
         for (int j = j0; j < j1; ++j) {
             const long long idx = zoff + (long long)j * nx + x;
             const float4 wyv = tw[d.tab_y + j];
@@ -1160,7 +1166,7 @@ __global__ void __launch_bounds__(kBlock)
             wc = Wi[(long long)j0 * nx + x];
             wm = Wi[(long long)max(j0 - 1, 0) * nx + x];
         }
-#pragma unroll(MODE == 0 ? 4 : 1)
+#pragma unroll kRowUnroll<MODE>
         for (int j = j0; j < j1; ++j) {
             const long long idx = (long long)j * nx + x;
             const float4 wyv = tw[d.tab_y + j];

@@ -88,6 +88,35 @@ def spline_warp(image, csize, params):
     return out.astype(np.float32)
 
 
+def spline_csize(n, rate, dim):
+    """Coefficient grid size of `spline:rate=<rate>` (float arithmetic of
+    mia/2d/transform/spline.cc:103-116 and mia/3d/transform/spline.cc:130-143)."""
+    r = np.float32(rate)
+    if dim == 3:
+        v = (np.float32(n - 1) + r) / r
+    else:
+        v = ((np.float32(n) + r) - np.float32(1.0)) / r
+    return int(v) + 2
+
+
+def series_frames_2d(n_frames=64, side=(96, 224), rate=8.0, max_disp=4.0, seed=0):
+    """Frames of the batched 2D series: per frame a Gaussian-blob phantom of
+    random size (sides uniform in `side`), quantised to uint16 and scaled to
+    float, and the same phantom warped by its own B-spline displacement
+    (coefficients N(0, 1.5) clipped to +-max_disp).  Returns [(moving, fixed)]."""
+    rng = np.random.default_rng(seed)
+    frames = []
+    for i in range(n_frames):
+        ny, nx = (int(v) for v in rng.integers(side[0], side[1] + 1, size=2))
+        fixed = scale_to_unit_range(gaussian_blobs((ny, nx), 8, (6.0, 14.0), (0.3, 1.0), 0.01,
+                                                   seed=int(rng.integers(1 << 30))))
+        cs = (spline_csize(nx, rate, 2), spline_csize(ny, rate, 2))
+        p = random_spline_params(cs, 1.5, max_disp, seed=int(rng.integers(1 << 30)))
+        moving = scale_to_unit_range(spline_warp(fixed, cs, p))
+        frames.append((moving, fixed))
+    return frames
+
+
 def scale_to_unit_range(img, bits=16):
     """uint16 intensities scaled to float32 (config of the batched 2D series)."""
     q = np.round(np.clip(img, 0, None) / max(float(img.max()), 1e-12) * (2 ** bits - 1)).astype(np.uint16)
