@@ -122,6 +122,22 @@ int mia_hip_graph_params(mia_hip_ctx *ctx, double **params);
 int mia_hip_graph_outputs(mia_hip_ctx *ctx, double **grad, double **costs);
 int mia_hip_graph_run(mia_hip_ctx *ctx);
 
+/* Minimise every instance's cost with L-BFGS on the device, all instances in
+ * lockstep: each iteration is the batched evaluation followed by one
+ * optimizer step per instance (one thread block each), `iters_per_graph`
+ * iterations captured in one HIP graph; the host only replays the graph and
+ * checks the instances' done flags.  Same algorithm as the host `lbfgs`
+ * minimizer: memory `memory` (<= 16), `maxeval` evaluations per instance
+ * (NLopt's maxeval, which MIA's `maxiter` maps to, addons/nlopt/nlopt.cc:237),
+ * first step moving no parameter by more than `first_step`.  x0: starting
+ * parameters of all instances (host); outputs (host, any may be NULL): best
+ * parameters, best costs, evaluations per instance, wall time of the
+ * iteration loop in seconds (graph capture excluded).  Replaces the
+ * minimizer loop of TNonrigidRegister::run for the level
+ * (mia/template/nonrigidregister.cxx:261-269). */
+int mia_hip_optimize(mia_hip_ctx *ctx, const double *x0, int maxeval, int memory, double first_step,
+                     int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds);
+
 /* Wait for all work on the context stream. */
 int mia_hip_synchronize(mia_hip_ctx *ctx);
 

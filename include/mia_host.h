@@ -22,6 +22,8 @@ extern "C" {
 #define MIA_HOST_BATCHED_GRAPH 0 /* one context, one captured graph per iteration for all frames */
 #define MIA_HOST_BATCHED 1       /* one context, eager launches per iteration for all frames */
 #define MIA_HOST_PER_FRAME 2     /* one context per frame, frames registered one after the other */
+#define MIA_HOST_DEVICE_BATCHED 3   /* on-device L-BFGS, all frames in one graph per iteration block */
+#define MIA_HOST_DEVICE_PER_FRAME 4 /* on-device L-BFGS, one context (and graph) per frame, one after the other */
 
 typedef struct mia_host_series mia_host_series;
 
@@ -47,7 +49,7 @@ int mia_host_series_n_params(mia_host_series *s, int frame, size_t *n, size_t *o
 /* Register every frame from the identity with the minimizer `minimizer`
  * ("lbfgs:maxiter=100[,memory=10,gtol=0,ftolr=0,step=1]" or MIA's
  * "gdsq:maxiter=100[,step=0.1,scale=2,xtola=0,gtola=0,ftolr=0]") in run mode
- * `mode`.  Outputs: params_out (concatenated best parameters), costs_out
+ * `mode`.  The device modes run lbfgs only (gtol = ftolr = 0).  Outputs: params_out (concatenated best parameters), costs_out
  * (best cost per frame), evals_out (evaluations per frame), seconds (wall time
  * of the optimisation loop, set-up excluded).  Any output may be NULL. */
 int mia_host_series_run(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,

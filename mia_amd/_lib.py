@@ -46,6 +46,8 @@ SIGNATURES = {
     "mia_hip_graph_params": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl_p)]),
     "mia_hip_graph_outputs": (ctypes.c_int, [_vp, ctypes.POINTER(_dbl_p), ctypes.POINTER(_dbl_p)]),
     "mia_hip_graph_run": (ctypes.c_int, [_vp]),
+    "mia_hip_optimize": (ctypes.c_int, [_vp, _dbl_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        _vp, _vp, _vp, _dbl_p]),
     "mia_hip_synchronize": (ctypes.c_int, [_vp]),
     "mia_hip_get_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "mia_hip_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),

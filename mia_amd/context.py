@@ -140,6 +140,21 @@ class HipContext:
         check(self._L.mia_hip_graph_run(self._h))
         return self._gc, self._gg
 
+    # -- on-device optimizer --------------------------------------------------------
+    def optimize(self, x0, maxeval=100, memory=10, first_step=1.0, iters_per_graph=10):
+        """On-device L-BFGS for all instances; returns (x, costs, evals, seconds)."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        if x0.size != self.n_params_total:
+            raise ValueError("expected %d parameters, got %d" % (self.n_params_total, x0.size))
+        x = np.empty_like(x0)
+        costs = np.empty(self.n_inst)
+        evals = np.empty(self.n_inst, dtype=np.int32)
+        sec = ctypes.c_double()
+        check(self._L.mia_hip_optimize(self._h, _f64p(x0), int(maxeval), int(memory), float(first_step),
+                                       int(iters_per_graph), x.ctypes.data, costs.ctypes.data, evals.ctypes.data,
+                                       ctypes.byref(sec)))
+        return x, costs, evals, sec.value
+
     # -- misc ---------------------------------------------------------------------
     def synchronize(self):
         check(self._L.mia_hip_synchronize(self._h))

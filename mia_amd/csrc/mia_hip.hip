@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,7 @@
 
 #include "../../include/mia_hip.h"
 #include "kernels.hh"
+#include "optimizer.hh"
 
 using namespace mia_hip;
 
@@ -134,6 +136,12 @@ struct mia_hip_ctx {
     unsigned *d_cnt = nullptr;
 
     double *h_params = nullptr, *h_grad = nullptr, *h_costs = nullptr;
+    // on-device optimizer state (allocated on first use, sized for `opt_memory`)
+    int opt_memory = 0;
+    double *d_oX = nullptr, *d_oG = nullptr, *d_oD = nullptr, *d_oBX = nullptr, *d_oS = nullptr, *d_oY = nullptr,
+           *d_oRHO = nullptr, *d_obest = nullptr;
+    int *d_odone = nullptr, *h_done = nullptr;
+    OptState *d_ost = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
@@ -154,7 +162,8 @@ void free_buffers(mia_hip_ctx *c)
 {
     void *ptrs[] = {c->d_inst, c->d_warp_prefix, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
-                    c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt};
+                    c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
+                    c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -164,6 +173,12 @@ void free_buffers(mia_hip_ctx *c)
         (void)hipHostFree(c->h_grad);
     if (c->h_costs)
         (void)hipHostFree(c->h_costs);
+    if (c->h_done)
+        (void)hipHostFree(c->h_done);
+    c->d_oX = c->d_oG = c->d_oD = c->d_oBX = c->d_oS = c->d_oY = c->d_oRHO = c->d_obest = nullptr;
+    c->d_odone = c->h_done = nullptr;
+    c->d_ost = nullptr;
+    c->opt_memory = 0;
     if (c->gexec)
         (void)hipGraphExecDestroy(c->gexec);
     if (c->graph)
@@ -768,6 +783,121 @@ int mia_hip_graph_run(mia_hip_ctx *c)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return MIA_HIP_OK;
+}
+
+int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, double first_step,
+                     int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds)
+{
+    int st = check_ready(c);
+    if (st)
+        return st;
+    if (!x0)
+        return fail(MIA_HIP_ERR_ARG, "null starting point");
+    if (maxeval < 1 || memory < 1 || memory > kMaxMemory || !(first_step > 0.0) || iters_per_graph < 1)
+        return fail(MIA_HIP_ERR_ARG, "optimize: need maxeval >= 1, 1 <= memory <= 16, first_step > 0, "
+                                     "iters_per_graph >= 1");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int n = c->n_inst;
+    if (c->opt_memory != memory) {
+        void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone,
+                       c->d_ost};
+        for (void *p : old)
+            if (p)
+                (void)hipFree(p);
+        if (c->h_done)
+            (void)hipHostFree(c->h_done);
+        c->h_done = nullptr;
+        c->opt_memory = 0;
+        if ((st = dev_alloc(&c->d_oX, c->n_par)) || (st = dev_alloc(&c->d_oG, c->n_par)) ||
+            (st = dev_alloc(&c->d_oD, c->n_par)) || (st = dev_alloc(&c->d_oBX, c->n_par)) ||
+            (st = dev_alloc(&c->d_oS, (size_t)memory * c->n_par)) ||
+            (st = dev_alloc(&c->d_oY, (size_t)memory * c->n_par)) || (st = dev_alloc(&c->d_oRHO, memory * n)) ||
+            (st = dev_alloc(&c->d_obest, n)) || (st = dev_alloc(&c->d_odone, n)) || (st = dev_alloc(&c->d_ost, n)))
+            return st;
+        HIP_TRY(hipHostMalloc((void **)&c->h_done, sizeof(int) * n, hipHostMallocDefault));
+        c->opt_memory = memory;
+    }
+    // initial state: parameters, phase 0, not done (plain copies, outside the graph)
+    std::vector<OptState> st0(n);
+    for (auto &o : st0) {
+        o = OptState();
+        o.phase = 0;
+    }
+    std::vector<int> done0(n, 0);
+    HIP_TRY(hipMemcpyAsync(c->d_params, x0, sizeof(double) * c->n_par, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_ost, st0.data(), sizeof(OptState) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_odone, done0.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    OptBuffers ob{c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
+    // capture iters_per_graph iterations + the done flags read-back
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MIA_HIP_OK;
+    for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
+        rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
+        if (rc == MIA_HIP_OK) {
+            hipLaunchKernelGGL(k_lbfgs_step, dim3(n), dim3(kBlock), 0, c->stream, c->d_inst, c->dim, ob, c->d_params,
+                               c->d_grad, c->d_costs, memory, maxeval, first_step);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs_step: ") + hipGetErrorString(e));
+        }
+    }
+    if (rc == MIA_HIP_OK) {
+        hipError_t e = hipMemcpyAsync(c->h_done, c->d_odone, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess)
+            rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+    }
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc != MIA_HIP_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e2 != hipSuccess)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    // every iteration performs one evaluation per unfinished instance
+    const int max_launches = (maxeval + iters_per_graph - 1) / iters_per_graph + 1;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
+        hipError_t e = hipGraphLaunch(ge, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
+            break;
+        }
+        bool all = true;
+        for (int i = 0; i < n; ++i)
+            all = all && c->h_done[i];
+        if (all)
+            break;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (rc != MIA_HIP_OK)
+        return rc;
+    if (seconds)
+        *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (x_out)
+        HIP_TRY(hipMemcpy(x_out, c->d_params, sizeof(double) * c->n_par, hipMemcpyDeviceToHost));
+    if (costs_out)
+        HIP_TRY(hipMemcpy(costs_out, c->d_obest, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (evals_out) {
+        std::vector<OptState> stv(n);
+        HIP_TRY(hipMemcpy(stv.data(), c->d_ost, sizeof(OptState) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)
+            evals_out[i] = stv[i].evals;
+    }
     return MIA_HIP_OK;
 }
 

@@ -1,0 +1,297 @@
+// On-device batched L-BFGS: one thread block per registration instance runs
+// one step of the minimizer after each batched cost + gradient evaluation, so
+// a whole optimisation iteration (warp, stage 1, reduce, step) stays on the
+// GPU and can be captured in a graph.  The algorithm is the one of the host
+// LBFGSMinimizer (mia_amd/host/minimizer.hh): two-loop recursion (Nocedal &
+// Wright, Alg. 7.4/7.5), backtracking Armijo line search with safeguarded
+// quadratic interpolation, first step scaled to move no parameter by more
+// than `first_step`, `maxeval` counting every evaluation.  Dot products are
+// block reductions in a fixed order, so an instance's trajectory does not
+// depend on the batch it runs in.
+#pragma once
+#include "kernels.hh"
+
+namespace mia_hip {
+
+constexpr int kMaxMemory = 16;
+
+struct OptState {
+    double f, gd, alpha, best_f;
+    int phase;  // 0: first evaluation pending, 1: line search, 2: done
+    int evals;
+    int k, head; // memory pairs stored, index of the oldest
+};
+
+// Block sum with the result in every thread (fixed order).
+__device__ __forceinline__ double block_sum_all(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k)
+        s += red[k];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double block_max_all(double v, double *red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmax(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        red[wid] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int k = 1; k < kBlock / 64; ++k)
+        s = fmax(s, red[k]);
+    __syncthreads();
+    return s;
+}
+
+// Buffers (doubles) per instance i with n = dim * ncoef parameters at offset
+// par_off: X, G, D, BX laid out like the parameters; S and Y hold `memory`
+// vectors per instance at offset memory * par_off; RHO memory scalars at
+// offset memory * i.  xt (the evaluation's parameter buffer), gt (its gradient
+// output) and costs are the evaluation's own buffers.
+struct OptBuffers {
+    double *X, *G, *D, *BX, *S, *Y, *RHO, *best_cost;
+    int *done;
+    OptState *st;
+};
+
+__global__ void __launch_bounds__(kBlock)
+    k_lbfgs_step(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
+                 const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
+                 double first_step)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ double alph[kMaxMemory];
+    __shared__ OptState s;
+    const int i = blockIdx.x;
+    const InstDesc d = inst[i];
+    const long long n = (long long)dim * d.cx * d.cy * d.cz;
+    const long long off = d.par_off;
+    double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off, *BX = ob.BX + off;
+    double *S = ob.S + (long long)memory * off, *Y = ob.Y + (long long)memory * off;
+    double *RHO = ob.RHO + (long long)memory * i;
+    double *XT = xt + off;
+    const double *GT = gt + off;
+    if (threadIdx.x == 0)
+        s = ob.st[i];
+    __syncthreads();
+    if (s.phase == 2)
+        return;
+    const double ft = costs[i];
+    const bool finite = isfinite(ft);
+    bool search = false;      // compute a new direction
+    bool first = false;
+    bool finish = false;
+    bool backtrack = false;
+    if (s.phase == 0) {
+        for (long long e = threadIdx.x; e < n; e += kBlock) {
+            X[e] = XT[e];
+            G[e] = GT[e];
+            BX[e] = XT[e];
+        }
+        if (threadIdx.x == 0) {
+            s.f = ft;
+            s.evals = 1;
+            s.best_f = finite ? ft : INFINITY;
+            s.k = 0;
+            s.head = 0;
+        }
+        __syncthreads();
+        double gmax = 0.0;
+        for (long long e = threadIdx.x; e < n; e += kBlock)
+            gmax = fmax(gmax, fabs(GT[e]));
+        gmax = block_max_all(gmax, red);
+        if (!finite || gmax <= 0.0 || s.evals >= maxeval)
+            finish = true;
+        else {
+            search = true;
+            first = true;
+        }
+    } else {
+        const double f0 = s.f, a0 = s.alpha, gd0 = s.gd;
+        const bool better = finite && ft < s.best_f;
+        if (better)
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                BX[e] = XT[e];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s.evals += 1;
+            if (better)
+                s.best_f = ft;
+        }
+        __syncthreads();
+        const bool armijo = finite && ft <= f0 + 1e-4 * a0 * gd0;
+        if (armijo) {
+            double sy = 0.0, yy = 0.0, ss = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock) {
+                const double se = XT[e] - X[e], ye = GT[e] - G[e];
+                sy += se * ye;
+                yy += ye * ye;
+                ss += se * se;
+            }
+            sy = block_sum_all(sy, red);
+            yy = block_sum_all(yy, red);
+            ss = block_sum_all(ss, red);
+            if (sy > 1e-12 * sqrt(ss * yy)) {
+                const int slot = s.k < memory ? (s.head + s.k) % memory : s.head;
+                double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
+                for (long long e = threadIdx.x; e < n; e += kBlock) {
+                    Sj[e] = XT[e] - X[e];
+                    Yj[e] = GT[e] - G[e];
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    RHO[slot] = 1.0 / sy;
+                    if (s.k < memory)
+                        s.k += 1;
+                    else
+                        s.head = (s.head + 1) % memory;
+                }
+            }
+            double gmax = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock) {
+                X[e] = XT[e];
+                G[e] = GT[e];
+                gmax = fmax(gmax, fabs(GT[e]));
+            }
+            gmax = block_max_all(gmax, red);
+            if (threadIdx.x == 0)
+                s.f = ft;
+            __syncthreads();
+            if (s.evals >= maxeval || gmax <= 0.0 || f0 == ft)
+                finish = true;
+            else
+                search = true;
+        } else if (s.evals >= maxeval) {
+            finish = true;
+        } else {
+            backtrack = true;
+        }
+    }
+    if (backtrack) {
+        double a = s.alpha;
+        double an = 0.5 * a;
+        if (finite) {
+            const double den = 2.0 * (ft - s.f - a * s.gd);
+            if (den > 0.0)
+                an = -s.gd * a * a / den;
+        }
+        an = fmin(fmax(an, 0.1 * a), 0.5 * a);
+        if (an < 1e-20) {
+            finish = true;
+        } else {
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                XT[e] = X[e] + an * D[e];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                s.alpha = an;
+        }
+    }
+    if (search) {
+        // two-loop recursion on q (kept in D), newest pair first
+        const int k = s.k, head = s.head;
+        for (long long e = threadIdx.x; e < n; e += kBlock)
+            D[e] = G[e];
+        __syncthreads();
+        for (int j = k - 1; j >= 0; --j) {
+            const int slot = (head + j) % memory;
+            const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
+            double a = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                a += Sj[e] * D[e];
+            a = block_sum_all(a, red) * RHO[slot];
+            if (threadIdx.x == 0)
+                alph[j] = a;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                D[e] -= a * Yj[e];
+            __syncthreads();
+        }
+        if (k > 0) {
+            const int slot = (head + k - 1) % memory;
+            const double *Yj = Y + (long long)slot * n;
+            double yy = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                yy += Yj[e] * Yj[e];
+            yy = block_sum_all(yy, red);
+            const double gamma = 1.0 / (RHO[slot] * yy);
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                D[e] *= gamma;
+            __syncthreads();
+        }
+        for (int j = 0; j < k; ++j) {
+            const int slot = (head + j) % memory;
+            const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
+            double bsum = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                bsum += Yj[e] * D[e];
+            const double b = block_sum_all(bsum, red) * RHO[slot];
+            const double c = alph[j] - b;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                D[e] += c * Sj[e];
+            __syncthreads();
+        }
+        double gd = 0.0;
+        for (long long e = threadIdx.x; e < n; e += kBlock) {
+            D[e] = -D[e];
+            gd += G[e] * D[e];
+        }
+        gd = block_sum_all(gd, red);
+        if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
+            gd = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock) {
+                D[e] = -G[e];
+                gd += G[e] * D[e];
+            }
+            gd = block_sum_all(gd, red);
+            if (threadIdx.x == 0) {
+                s.k = 0;
+                s.head = 0;
+            }
+            first = true;
+        }
+        double alpha = 1.0;
+        if (first) {
+            double dmax = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kBlock)
+                dmax = fmax(dmax, fabs(D[e]));
+            dmax = block_max_all(dmax, red);
+            if (dmax > 0.0)
+                alpha = fmin(1.0, first_step / dmax);
+        }
+        for (long long e = threadIdx.x; e < n; e += kBlock)
+            XT[e] = X[e] + alpha * D[e];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s.alpha = alpha;
+            s.gd = gd;
+            s.phase = 1;
+        }
+    }
+    if (finish) {
+        for (long long e = threadIdx.x; e < n; e += kBlock)
+            XT[e] = BX[e];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s.phase = 2;
+            ob.done[i] = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ob.st[i] = s;
+        ob.best_cost[i] = s.best_f;
+    }
+}
+
+} // namespace mia_hip

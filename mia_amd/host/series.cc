@@ -157,6 +157,67 @@ struct mia_host_series {
     std::vector<char> frame_set;
 };
 
+// On-device L-BFGS through mia_hip_optimize: all frames in the batched context
+// (one graph covers every frame), or each frame in its own context, one frame
+// after the other.  Only `lbfgs` with gtol = ftolr = 0 runs on the device.
+static int run_device(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
+                      int *evals_out, double *seconds)
+{
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> kv;
+    if (!parse_descr(minimizer, name, kv) || name != "lbfgs")
+        return fail(MIA_HIP_ERR_ARG, "series: the device modes run the lbfgs minimizer only");
+    int maxeval = 100, memory = 10;
+    double step = 1.0;
+    try {
+        for (auto &p : kv) {
+            if (p.first == "maxiter")
+                maxeval = std::stoi(p.second);
+            else if (p.first == "memory")
+                memory = std::stoi(p.second);
+            else if (p.first == "step")
+                step = std::stod(p.second);
+            else if ((p.first == "gtol" || p.first == "ftolr") && std::stod(p.second) == 0.0)
+                continue;
+            else
+                return fail(MIA_HIP_ERR_ARG, "series: device lbfgs does not take '" + p.first + "'");
+        }
+    } catch (const std::exception &e) {
+        return fail(MIA_HIP_ERR_ARG, std::string("series: bad minimizer parameter value: ") + e.what());
+    }
+    const int chunk = 10;
+    int st;
+    if (mode == MIA_HOST_DEVICE_BATCHED) {
+        std::vector<double> x0(s->n_total, 0.0);
+        if ((st = mia_hip_set_cost(s->batch, 0, 0.0f, 1.0, MIA_HIP_GRAD_REFERENCE)))
+            return hip_fail(st);
+        if ((st = mia_hip_optimize(s->batch, x0.data(), maxeval, memory, step, chunk, params_out, costs_out, evals_out,
+                                   seconds)))
+            return hip_fail(st);
+        return MIA_HIP_OK;
+    }
+    double total = 0.0;
+    for (int i = 0; i < s->n; ++i) {
+        mia_hip_ctx *c = s->single[i];
+        std::vector<double> x0(s->n_par[i], 0.0);
+        double sec = 0.0, cost = 0.0;
+        int ev = 0;
+        if ((st = mia_hip_set_cost(c, 0, 0.0f, 1.0, MIA_HIP_GRAD_REFERENCE)))
+            return hip_fail(st);
+        if ((st = mia_hip_optimize(c, x0.data(), maxeval, memory, step, chunk,
+                                   params_out ? params_out + s->off[i] : nullptr, &cost, &ev, &sec)))
+            return hip_fail(st);
+        total += sec;
+        if (costs_out)
+            costs_out[i] = cost;
+        if (evals_out)
+            evals_out[i] = ev;
+    }
+    if (seconds)
+        *seconds = total;
+    return MIA_HIP_OK;
+}
+
 extern "C" {
 
 const char *mia_host_last_error(void)
@@ -258,6 +319,8 @@ int mia_host_series_run(mia_host_series *s, const char *minimizer, int mode, dou
     for (int i = 0; i < s->n; ++i)
         if (!s->frame_set[i])
             return fail(MIA_HIP_ERR_STATE, "series: frame " + std::to_string(i) + " has no images");
+    if (mode == MIA_HOST_DEVICE_BATCHED || mode == MIA_HOST_DEVICE_PER_FRAME)
+        return run_device(s, minimizer, mode, params_out, costs_out, evals_out, seconds);
     std::vector<std::unique_ptr<ReverseMinimizer>> opt(s->n);
     for (int i = 0; i < s->n; ++i) {
         std::string err;

@@ -18,6 +18,8 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "mia_host.h")
 BATCHED_GRAPH = 0
 BATCHED = 1
 PER_FRAME = 2
+DEVICE_BATCHED = 3
+DEVICE_PER_FRAME = 4
 
 _vp = ctypes.c_void_p
 SIGNATURES = {

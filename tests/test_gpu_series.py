@@ -107,3 +107,44 @@ def test_bad_minimizer_descriptor():
             s.run("gdsq:scale=1.0")
         with pytest.raises(ValueError):
             s.run("newton:maxiter=3")
+
+
+@pytest.mark.parametrize("dim,n,side", [(2, 6, (48, 140)), (3, 2, (24, 36))])
+def test_device_optimizer_batched_equals_per_frame(dim, n, side):
+    """On-device L-BFGS: one graph for all frames and one context per frame
+    give bitwise identical parameters, costs and evaluation counts."""
+    from mia_amd import series
+    frames = _frames(n, side, 21, dim)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        pb, cb, eb, _ = s.run("lbfgs:maxiter=30", series.DEVICE_BATCHED)
+        pf, cf, ef, _ = s.run("lbfgs:maxiter=30", series.DEVICE_PER_FRAME)
+    assert np.array_equal(pb, pf)
+    assert np.array_equal(cb, cf)
+    assert np.array_equal(eb, ef)
+    assert np.all(eb <= 30)
+
+
+def test_device_optimizer_follows_host_lbfgs():
+    """Same algorithm on the device and on the host (different summation
+    order of the dot products): the first iterations agree tightly and the
+    final costs of a longer run agree closely."""
+    from mia_amd import series
+    frames = _frames(3, (60, 110), 8)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        ph, ch, eh, _ = s.run("lbfgs:maxiter=6", series.BATCHED)
+        pd, cd, ed, _ = s.run("lbfgs:maxiter=6", series.DEVICE_BATCHED)
+        assert np.array_equal(eh, ed)
+        np.testing.assert_allclose(cd, ch, rtol=1e-9)
+        assert np.abs(pd - ph).max() <= 1e-7 * max(1.0, np.abs(ph).max())
+        _, ch2, _, _ = s.run("lbfgs:maxiter=60", series.BATCHED)
+        _, cd2, _, _ = s.run("lbfgs:maxiter=60", series.DEVICE_BATCHED)
+    np.testing.assert_allclose(cd2, ch2, rtol=2e-2)
+    assert np.all(cd2 < cd)
+
+
+def test_device_optimizer_rejects_gdsq():
+    from mia_amd import series
+    frames = _frames(1, (40, 48), 2)
+    with series.SeriesRegistration(frames) as s:
+        with pytest.raises(ValueError):
+            s.run("gdsq:maxiter=5", series.DEVICE_BATCHED)
