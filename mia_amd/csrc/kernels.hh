@@ -38,12 +38,14 @@ constexpr int kBlock = 256;
 constexpr int kMaxRow = 512;       // longest x row the stage-1 kernel keeps in LDS
 // 3D warp tile and LDS budgets
 constexpr int kW3TX = 32, kW3TY = 8, kW3TZ = 8;
-// The coefficient window (phase 1-2) and the box of moving-image taps
-// (phase 3-4) share one LDS buffer; 9984 floats keep a block under 40 KiB so
-// four blocks (16 waves) fit a CU.
-constexpr int kW3Buf = 9984;
-constexpr int kW3CWin = kW3Buf / 4;                 // coefficient window entries (float4)
-constexpr int kW3Region = kW3Buf;                   // tap box (floats)
+// The displacement intermediates (phases 1-2) and the box of moving-image
+// taps (phases 3-4) share one LDS buffer.  The box has compile-time pitches
+// (40 x 16 floats per row / slice, at most 15 slices) so that the 64 taps of
+// a voxel are one base address plus immediate offsets; 9600 floats keep a
+// block under 40 KiB so that four blocks (16 waves) fit a CU.
+constexpr int kW3PX = 40, kW3PY = 16, kW3PZ = 15;
+constexpr int kW3Buf = kW3PX * kW3PY * kW3PZ;       // 9600 floats
+constexpr int kW3Buf4 = kW3Buf / 4;                 // float4 capacity for the intermediates
 // 2D warp tile and LDS budgets
 constexpr int kW2TX = 64, kW2TY = 32;
 constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2
@@ -91,9 +93,14 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
 
 // floor after rounding to a multiple of 2^-16 (mia/core/splinekernel.cc:164-192).
 // v * 65536 and the scaling back are exact in float.
+__device__ __forceinline__ float fastfloor16f(float v)
+{
+    return floorf(rintf(v * 65536.0f) * (1.0f / 65536.0f));
+}
+
 __device__ __forceinline__ int fastfloor16(float v)
 {
-    return (int)floorf(rintf(v * 65536.0f) * (1.0f / 65536.0f));
+    return (int)fastfloor16f(v);
 }
 
 // Sample positions are kept in [-1e6, 1e6]: the float -> int conversion and
@@ -123,8 +130,10 @@ __device__ __forceinline__ int mirror_idx(int i, int n)
 template <int DEG, bool GRAD>
 __device__ __forceinline__ int kweights(float p, float *w, float *dw)
 {
-    const int fl = fastfloor16(p);
-    const float f = p - (float)fl;
+    // positions are clamped to +-1e6, so the floor is an exact integer in float
+    const float flf = fastfloor16f(p);
+    const int fl = (int)flf;
+    const float f = p - flf;
     if constexpr (DEG == 3) {
         const float fm1 = 1.0f - f;
         const float s = 1.0f / 6.0f;
@@ -537,11 +546,18 @@ __global__ void __launch_bounds__(kBlock) k_prefilter_lines(float *__restrict__ 
 // ---------------------------------------------------------------------------
 
 // 3D warp.  Block = 32 x 8 x 8 voxels, thread = one (x, y) column of 8 voxels.
-// Per z slice the tile's coefficient window is contracted along z into LDS
-// (zero beyond the grid), each voxel contracts 4 x 4 window entries with its
-// y / x table weights and keeps its sample position.  The block then reduces
-// the positions to the bounding box of all interpolation taps; if that box
-// fits kW3Region it is loaded once (mirrored) and every tap is an LDS read.
+//   phase 1: the tile's coefficient window (double params -> float, zero
+//            beyond the grid) and the tile's x/y/z spline tables into LDS;
+//   phase 2: displacement, contracted along y (T1[kz][y][kx]), then x
+//            (T2[kz][y][x]) in LDS, then z per voxel; when the intermediates
+//            do not fit, each thread contracts its column along z directly;
+//   phase 3: the bounding box of all interpolation taps of the tile's voxels
+//            (block min/max of the sample positions) is loaded into LDS with
+//            mirror indices applied on the load;
+//   phase 4: every voxel reads its 4^3 taps from LDS at one base address plus
+//            compile-time offsets, clamps and stores W.  A tile whose box
+//            exceeds the 40 x 16 x 15 pitches gathers from global memory with
+//            the same arithmetic.
 template <int DEG>
 __global__ void __launch_bounds__(kBlock)
     k_warp3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int boff, int nblocks,
@@ -549,11 +565,10 @@ __global__ void __launch_bounds__(kBlock)
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
     constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
-    __shared__ float4 buf4[kW3Buf / 4];
+    __shared__ float4 buf4[kW3Buf4];
     __shared__ float red[6][kBlock / 64];
-    __shared__ float4 wz_s[kW3TZ];
-    __shared__ int sz_s[kW3TZ];
-    float4 *cwin = buf4;
+    __shared__ float4 wx_s[kW3TX], wy_s[kW3TY], wz_s[kW3TZ];
+    __shared__ int sx_s[kW3TX], sy_s[kW3TY], sz_s[kW3TZ];
     float *region = (float *)buf4;
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
     const int i = find_inst(prefix, n_inst, b);
@@ -564,29 +579,39 @@ __global__ void __launch_bounds__(kBlock)
     const int tx = lb % tiles_x;
     const int rest = lb / tiles_x;
     const int x0 = tx * kW3TX, y0 = (rest % tiles_y) * kW3TY, z0 = (rest / tiles_y) * kW3TZ;
-    const int x = x0 + (threadIdx.x & (kW3TX - 1)), y = y0 + threadIdx.x / kW3TX;
+    // a wave holds rows yl and yl + 4 of the tile: their LDS box rows are
+    // 4 * kW3PX = 160 floats apart, so the two half-waves use disjoint banks
+    const int xl = threadIdx.x & (kW3TX - 1), yl = ((threadIdx.x >> 5) & 1) * 4 + (threadIdx.x >> 6);
+    const int x = x0 + xl, y = y0 + yl;
     const bool valid = x < d.nx && y < d.ny;
-    const int xc = min(x, d.nx - 1), yc = min(y, d.ny - 1);
     const int nzl = min(kW3TZ, d.nz - z0);
-    const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
+    const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0], kz0 = ts[d.tab_z + z0];
     const int wxn = ts[d.tab_x + min(x0 + kW3TX, d.nx) - 1] + 4 - kx0;
     const int wyn = ts[d.tab_y + min(y0 + kW3TY, d.ny) - 1] + 4 - ky0;
-    const int sx = ts[d.tab_x + xc] - kx0, sy = ts[d.tab_y + yc] - ky0;
-    const float4 wxv = tw[d.tab_x + xc], wyv = tw[d.tab_y + yc];
-    const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
-    const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
-    const double *P = params + d.par_off;
-    // phase 1: the tile's coefficient window (x fastest, then y, then z) and
-    // the z table of its slices
-    if ((int)threadIdx.x < nzl) {
-        wz_s[threadIdx.x] = tw[d.tab_z + z0 + threadIdx.x];
-        sz_s[threadIdx.x] = ts[d.tab_z + z0 + threadIdx.x];
-    }
-    const int kz0 = ts[d.tab_z + z0];
     const int wzn = ts[d.tab_z + z0 + nzl - 1] + 4 - kz0;
-    const bool win_lds = wxn * wyn * wzn <= kW3CWin; // block-uniform
-    if (win_lds) {
-        for (int e = threadIdx.x; e < wxn * wyn * wzn; e += kBlock) {
+    const double *P = params + d.par_off;
+    // phase 1: spline tables of the tile (clamped to the image for the
+    // partial tiles' unused lanes) and the coefficient window
+    if (threadIdx.x < kW3TX) {
+        const int xx = min(x0 + (int)threadIdx.x, d.nx - 1);
+        wx_s[threadIdx.x] = tw[d.tab_x + xx];
+        sx_s[threadIdx.x] = ts[d.tab_x + xx] - kx0;
+    } else if (threadIdx.x < kW3TX + kW3TY) {
+        const int k = threadIdx.x - kW3TX;
+        const int yy = min(y0 + k, d.ny - 1);
+        wy_s[k] = tw[d.tab_y + yy];
+        sy_s[k] = ts[d.tab_y + yy] - ky0;
+    } else if (threadIdx.x < kW3TX + kW3TY + kW3TZ) {
+        const int k = threadIdx.x - kW3TX - kW3TY;
+        const int zz = min(z0 + k, d.nz - 1);
+        wz_s[k] = tw[d.tab_z + zz];
+        sz_s[k] = ts[d.tab_z + zz] - kz0;
+    }
+    const int nwin = wxn * wyn * wzn, nt1 = wzn * kW3TY * wxn, nt2 = wzn * kW3TY * kW3TX;
+    const bool separable = nwin + nt1 + nt2 <= kW3Buf4; // block-uniform
+    float4 *cwin = buf4, *T1 = buf4 + nwin, *T2 = T1 + nt1;
+    if (separable) {
+        for (int e = threadIdx.x; e < nwin; e += kBlock) {
             const int kx = kx0 + e % wxn;
             const int t = e / wxn;
             const int ky = ky0 + t % wyn, kz = kz0 + t / wyn;
@@ -599,32 +624,74 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     __syncthreads();
-    // phase 2: displacement of the thread's 8 voxels, contracted along x, then
-    // y, then z (the order of init_grid, spline.cc:573-650)
     float ux[kW3TZ], uy[kW3TZ], uz[kW3TZ];
 #pragma unroll
     for (int zl = 0; zl < kW3TZ; ++zl)
         ux[zl] = uy[zl] = uz[zl] = 0.0f;
-    for (int kzl = 0; kzl < wzn; ++kzl) {
-        float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-        if (win_lds) {
+    if (separable) {
+        // phase 2a: T1[kz][yl][kx] = sum_b wy_b(yl) * C[kz][sy(yl) + b][kx]
+        for (int e = threadIdx.x; e < nt1; e += kBlock) {
+            const int kx = e % wxn;
+            const int t = e / wxn;
+            const int r = t % kW3TY, kz = t / kW3TY;
+            const float4 w = wy_s[r];
+            const float ws[4] = {w.x, w.y, w.z, w.w};
+            const float4 *src = cwin + (kz * wyn + sy_s[r]) * wxn + kx;
+            float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
             for (int bb = 0; bb < 4; ++bb) {
-                float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-                const float4 *wr = cwin + (kzl * wyn + sy + bb) * wxn + sx;
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const float4 q = wr[a];
-                    hx = fmaf(wxs[a], q.x, hx);
-                    hy = fmaf(wxs[a], q.y, hy);
-                    hz = fmaf(wxs[a], q.z, hz);
-                }
-                vx = fmaf(wys[bb], hx, vx);
-                vy = fmaf(wys[bb], hy, vy);
-                vz = fmaf(wys[bb], hz, vz);
+                const float4 q = src[bb * wxn];
+                ax = fmaf(ws[bb], q.x, ax);
+                ay = fmaf(ws[bb], q.y, ay);
+                az = fmaf(ws[bb], q.z, az);
             }
-        } else {
+            T1[e] = make_float4(ax, ay, az, 0.0f);
+        }
+        __syncthreads();
+        // phase 2b: T2[kz][yl][xl] = sum_a wx_a(xl) * T1[kz][yl][sx(xl) + a]
+        for (int e = threadIdx.x; e < nt2; e += kBlock) {
+            const int c = e & (kW3TX - 1);
+            const int t = e / kW3TX; // kz * kW3TY + yl
+            const float4 w = wx_s[c];
+            const float ws[4] = {w.x, w.y, w.z, w.w};
+            const float4 *src = T1 + t * wxn + sx_s[c];
+            float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float4 q = src[a];
+                ax = fmaf(ws[a], q.x, ax);
+                ay = fmaf(ws[a], q.y, ay);
+                az = fmaf(ws[a], q.z, az);
+            }
+            T2[e] = make_float4(ax, ay, az, 0.0f);
+        }
+        __syncthreads();
+        // phase 2c: z contraction per voxel
+#pragma unroll
+        for (int zl = 0; zl < kW3TZ; ++zl) {
+            if (zl < nzl) {
+                const float4 w = wz_s[zl];
+                const float ws[4] = {w.x, w.y, w.z, w.w};
+                const float4 *src = T2 + (sz_s[zl] * kW3TY + yl) * kW3TX + xl;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float4 q = src[c * kW3TY * kW3TX];
+                    ux[zl] = fmaf(ws[c], q.x, ux[zl]);
+                    uy[zl] = fmaf(ws[c], q.y, uy[zl]);
+                    uz[zl] = fmaf(ws[c], q.z, uz[zl]);
+                }
+            }
+        }
+    } else {
+        // the window does not fit: contract this column's coefficients from
+        // global memory, x then y per kz, then scatter along z
+        const float4 wxv = wx_s[xl], wyv = wy_s[yl];
+        const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
+        const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+        const int sx = sx_s[xl], sy = sy_s[yl];
+        for (int kzl = 0; kzl < wzn; ++kzl) {
             const int kz = kz0 + kzl;
+            float vx = 0.0f, vy = 0.0f, vz = 0.0f;
             if (kz < d.cz) {
 #pragma unroll
                 for (int bb = 0; bb < 4; ++bb) {
@@ -645,15 +712,15 @@ __global__ void __launch_bounds__(kBlock)
                     vz = fmaf(wys[bb], hz, vz);
                 }
             }
-        }
 #pragma unroll
-        for (int zl = 0; zl < kW3TZ; ++zl) {
-            const int c = kz0 + kzl - (zl < nzl ? sz_s[zl] : 0);
-            if (zl < nzl && c >= 0 && c < 4) {
-                const float w = ((const float *)&wz_s[zl])[c];
-                ux[zl] = fmaf(w, vx, ux[zl]);
-                uy[zl] = fmaf(w, vy, uy[zl]);
-                uz[zl] = fmaf(w, vz, uz[zl]);
+            for (int zl = 0; zl < kW3TZ; ++zl) {
+                const int c = kzl - (zl < nzl ? sz_s[zl] : 0);
+                if (zl < nzl && c >= 0 && c < 4) {
+                    const float w = ((const float *)&wz_s[zl])[c];
+                    ux[zl] = fmaf(w, vx, ux[zl]);
+                    uy[zl] = fmaf(w, vy, uy[zl]);
+                    uz[zl] = fmaf(w, vz, uz[zl]);
+                }
             }
         }
     }
@@ -661,23 +728,20 @@ __global__ void __launch_bounds__(kBlock)
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
     for (int zl = 0; zl < kW3TZ; ++zl) {
-        px[zl] = py[zl] = pz[zl] = 0.0f;
-        if (zl < nzl) {
-            const int z = z0 + zl;
-            px[zl] = clamp_pos((float)x - ux[zl]);
-            py[zl] = clamp_pos((float)y - uy[zl]);
-            pz[zl] = clamp_pos((float)z - uz[zl]);
-            if (valid) {
-                mnx = fminf(mnx, px[zl]);
-                mxx = fmaxf(mxx, px[zl]);
-                mny = fminf(mny, py[zl]);
-                mxy = fmaxf(mxy, py[zl]);
-                mnz = fminf(mnz, pz[zl]);
-                mxz = fmaxf(mxz, pz[zl]);
-            }
+        px[zl] = clamp_pos((float)x - ux[zl]);
+        py[zl] = clamp_pos((float)y - uy[zl]);
+        pz[zl] = clamp_pos((float)(z0 + zl) - uz[zl]);
+        if (valid && zl < nzl) {
+            mnx = fminf(mnx, px[zl]);
+            mxx = fmaxf(mxx, px[zl]);
+            mny = fminf(mny, py[zl]);
+            mxy = fmaxf(mxy, py[zl]);
+            mnz = fminf(mnz, pz[zl]);
+            mxz = fmaxf(mxz, pz[zl]);
         }
     }
-    // bounding box of the taps of all valid voxels of the tile
+    // bounding box of the taps of all valid voxels of the tile; the barrier
+    // also ends every read of the phase-2 intermediates, which the box reuses
     mnx = wave_min(mnx);
     mny = wave_min(mny);
     mnz = wave_min(mnz);
@@ -703,16 +767,61 @@ __global__ void __launch_bounds__(kBlock)
         mxy = fmaxf(mxy, red[4][k]);
         mxz = fmaxf(mxz, red[5][k]);
     }
-    const int rx0 = fastfloor16(mnx) - LO, ry0 = fastfloor16(mny) - LO, rz0 = fastfloor16(mnz) - LO;
-    const int RX = fastfloor16(mxx) - LO + NS - rx0;
-    const int RY = fastfloor16(mxy) - LO + NS - ry0;
-    const int RZ = fastfloor16(mxz) - LO + NS - rz0;
-    const bool use_lds = (long long)RX * RY * RZ <= kW3Region;
+    // every lane holds the same box; readfirstlane makes that visible to the
+    // compiler so that the fill below runs its row arithmetic on the scalar unit
+    const int rx0 = __builtin_amdgcn_readfirstlane(fastfloor16(mnx) - LO);
+    const int ry0 = __builtin_amdgcn_readfirstlane(fastfloor16(mny) - LO);
+    const int rz0 = __builtin_amdgcn_readfirstlane(fastfloor16(mnz) - LO);
+    const int RX = __builtin_amdgcn_readfirstlane(fastfloor16(mxx) - LO + NS) - rx0;
+    const int RY = __builtin_amdgcn_readfirstlane(fastfloor16(mxy) - LO + NS) - ry0;
+    const int RZ = __builtin_amdgcn_readfirstlane(fastfloor16(mxz) - LO + NS) - rz0;
+    const bool use_lds = RX <= kW3PX && RY <= kW3PY && RZ <= kW3PZ; // block-uniform
     const float *Mi = M + d.img_off;
     if (use_lds) {
-        // phase 3: flat fill of the box, mirror indices applied here; the box
-        // coordinates advance incrementally (no division in the loop)
-        fill_box3(region, Mi, d.nx, d.ny, d.nz, rx0, ry0, rz0, RX, RY, RZ);
+        // phase 3: one box row (RX <= 40 floats) per wave instruction, the
+        // waves taking rows round robin; kFill3 rows are loaded before any is
+        // stored so that that many loads are in flight per wave
+        constexpr int kFill3 = 8;
+        const int lane = threadIdx.x & 63;
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const bool xact = lane < RX;
+        int gx = rx0 + lane;
+        if (rx0 < 0 || rx0 + RX > d.nx)
+            gx = mirror_idx(xact ? gx : 0, d.nx);
+        const bool yzin = ry0 >= 0 && ry0 + RY <= d.ny && rz0 >= 0 && rz0 + RZ <= d.nz;
+        const int nrows = RY * RZ;
+        int iy = wv, iz = 0;
+        while (iy >= RY) {
+            iy -= RY;
+            ++iz;
+        }
+        float *dst = region + lane;
+        for (int r = wv; r < nrows; r += 4 * kFill3) {
+            float v[kFill3];
+            int lo[kFill3];
+#pragma unroll
+            for (int u = 0; u < kFill3; ++u) {
+                v[u] = 0.0f;
+                lo[u] = -1;
+                if (r + 4 * u < nrows) {
+                    const int gy = yzin ? ry0 + iy : mirror_idx(ry0 + iy, d.ny);
+                    const int gz = yzin ? rz0 + iz : mirror_idx(rz0 + iz, d.nz);
+                    const float *src = Mi + ((long long)gz * d.ny + gy) * d.nx;
+                    if (xact)
+                        v[u] = src[gx];
+                    lo[u] = (iz * kW3PY + iy) * kW3PX;
+                    iy += 4;
+                    while (iy >= RY) {
+                        iy -= RY;
+                        ++iz;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kFill3; ++u)
+                if (lo[u] >= 0 && xact)
+                    dst[lo[u]] = v[u];
+        }
         __syncthreads();
     }
     if (!valid)
@@ -720,9 +829,8 @@ __global__ void __launch_bounds__(kBlock)
     const float2 cl = clamp[i];
     float *Wi = W + d.img_off + ((long long)z0 * d.ny + y) * d.nx + x;
     const long long plane = (long long)d.nx * d.ny;
-    // phase 4 (two separate loops so the LDS one unrolls and overlaps the
-    // latency of consecutive voxels' tap reads)
     if (use_lds) {
+        // phase 4: the taps at one base address plus compile-time offsets
 #pragma unroll
         for (int zl = 0; zl < kW3TZ; ++zl) {
             if (zl < nzl) {
@@ -730,8 +838,29 @@ __global__ void __launch_bounds__(kBlock)
                 const int s0 = kweights<DEG, false>(px[zl], wa, nullptr);
                 const int s1 = kweights<DEG, false>(py[zl], wb, nullptr);
                 const int s2 = kweights<DEG, false>(pz[zl], wc, nullptr);
-                Wi[zl * plane] = clamp_val(
-                    contract_lds3<DEG>(region, RX, RY, s0 - rx0, s1 - ry0, s2 - rz0, wa, wb, wc), cl.x, cl.y);
+                const int b0 = ((s2 - rz0) * kW3PY + (s1 - ry0)) * kW3PX + (s0 - rx0);
+                float v = 0.0f;
+#pragma unroll
+                for (int c = 0; c < NS; ++c) {
+                    // an opaque copy of the slice's index: the 16 taps of the
+                    // slice are then immediate offsets from one register
+                    // (left alone, the compiler forms a new address for most
+                    // pairs of taps beyond the first slice)
+                    int bc = b0 + c * kW3PY * kW3PX;
+                    asm("" : "+v"(bc));
+                    float vy = 0.0f;
+#pragma unroll
+                    for (int bb = 0; bb < NS; ++bb) {
+                        const float *row = region + bc + bb * kW3PX;
+                        float vx = 0.0f;
+#pragma unroll
+                        for (int a = 0; a < NS; ++a)
+                            vx = fmaf(wa[a], row[a], vx);
+                        vy = fmaf(wb[bb], vx, vy);
+                    }
+                    v = fmaf(wc[c], vy, v);
+                }
+                Wi[zl * plane] = clamp_val(v, cl.x, cl.y);
             }
         }
     } else {
