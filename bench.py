@@ -17,6 +17,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -28,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cost+gradient Mvoxel/s (3D 256³) + batched 2D regs/sec at equal SSD"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernels_cost_grad_3d_256cube_rate8_traffic.md")
 
 
 def log(*a):
@@ -48,6 +50,69 @@ def alg_bytes(shape, cs, stage):
         "finalize": 0,
         "adjoint_reduce": part + ncoef * 24,   # read partials, write gradient
     }[stage]
+
+
+class StreamChain:
+    """Serialises work queued on several HIP streams without host
+    synchronisation: each stream waits for an event recorded at the end of the
+    work queued before it.  The host then runs ahead of the GPU, so timing
+    events around a kernel see the kernel and not the host's enqueue latency."""
+
+    def __init__(self):
+        L = None
+        for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                L = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if L is None:
+            raise RuntimeError("HIP runtime library not found")
+        vp = ctypes.c_void_p
+        L.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
+        L.hipEventRecord.argtypes = [vp, vp]
+        L.hipStreamWaitEvent.argtypes = [vp, vp, ctypes.c_uint]
+        L.hipEventDestroy.argtypes = [vp]
+        self.L = L
+        self.events = []
+
+    def link(self, stream):
+        if self.events:
+            if self.L.hipStreamWaitEvent(stream, self.events[-1], 0) != 0:
+                raise RuntimeError("hipStreamWaitEvent failed")
+
+    def mark(self, stream):
+        e = ctypes.c_void_p()
+        if self.L.hipEventCreateWithFlags(ctypes.byref(e), 2) != 0:   # hipEventDisableTiming
+            raise RuntimeError("hipEventCreateWithFlags failed")
+        self.events.append(e)
+        if self.L.hipEventRecord(e, stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def close(self):
+        for e in self.events:
+            self.L.hipEventDestroy(e)
+        self.events = []
+
+
+def profiled_traffic(stage, size, rate, grad_mode):
+    """HBM bytes per launch of `stage` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for a device copy of known size), when that summary covers this
+    workload; None otherwise."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None
+    js = json.loads(m.group(1))
+    w = js["workload"]
+    if w["size"] != size or abs(w["rate"] - rate) > 1e-9 or w["grad_mode"] != grad_mode:
+        return None
+    k = js["kernels"].get(stage)
+    return k["traffic_bytes"] if k else None
 
 
 def cpu_baseline(mov, fix, cs, params, threads):
@@ -215,11 +280,21 @@ def main():
     value = nvox / (ms_per_step * 1e-3) / 1e6
     log("timed %d steps: %.4f ms/step, %.1f Mvoxel/s" % (args.steps, ms_per_step, value))
 
-    # per-kernel durations with HIP events on the context streams
+    # per-kernel durations with HIP events on the context streams: the same
+    # steps, queued back to back (serialised across the contexts' streams by
+    # events) so that each kernel's events bracket the kernel alone
     for c in ctxs:
         c.profile_enable(True)
+    streams = [c.stream() for c in ctxs]
+    chain = StreamChain()
     for s in range(args.steps):
-        step(s)
+        r = s % args.rotate
+        chain.link(streams[r])
+        ctxs[r].cost_grad_device(*bufs[r])
+        chain.mark(streams[r])
+    for c in ctxs:
+        c.synchronize()
+    chain.close()
     prof = {}
     for c in ctxs:
         for k, (ms, cnt) in c.profile_read().items():
@@ -237,8 +312,12 @@ def main():
                       "GBs": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
     kd = kernels[dom]
+    traffic = profiled_traffic(dom, n, args.rate, args.grad_mode)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "traffic_unit": "bytes per launch",
+                "alg_bytes": kd["alg_bytes"],
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None}
     tsum = sum(v["avg_us"] for v in kernels.values())
     log("kernels: %s" % json.dumps(kernels))
 

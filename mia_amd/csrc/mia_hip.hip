@@ -1,6 +1,7 @@
 // C-ABI implementation (include/mia_hip.h): context, device buffers, spline
 // axis tables and the launch sequence of one cost + gradient evaluation.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <chrono>
@@ -204,30 +205,27 @@ hipEvent_t get_event(mia_hip_ctx *c)
         return e;
     }
     hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
+    // timing only: no system-scope fence (no cache writeback between the
+    // timed kernels)
+    (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     return e;
 }
 
-struct StageTimer {
-    mia_hip_ctx *c;
-    int stage;
-    hipEvent_t a = nullptr;
-    StageTimer(mia_hip_ctx *ctx, int st, bool on) : c(ctx), stage(st)
-    {
-        if (on) {
-            a = get_event(c);
-            (void)hipEventRecord(a, c->stream);
-        }
+// Launch a kernel on the context's stream.  With profiling on, a pair of
+// timing events brackets the dispatch itself (hipExtLaunchKernelGGL) rather
+// than separate markers queued around it.
+template <typename K, typename... Args>
+void launch(mia_hip_ctx *c, int stage, bool timed, K kern, dim3 grid, size_t lds, Args... args)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timed) {
+        a = get_event(c);
+        b = get_event(c);
     }
-    ~StageTimer()
-    {
-        if (a) {
-            hipEvent_t b = get_event(c);
-            (void)hipEventRecord(b, c->stream);
-            c->pending.push_back({stage, a, b});
-        }
-    }
-};
+    hipExtLaunchKernelGGL(kern, grid, dim3(kBlock), (uint32_t)lds, c->stream, a, b, 0, args...);
+    if (timed)
+        c->pending.push_back({stage, a, b});
+}
 
 template <typename T>
 int dev_alloc(T **p, size_t count)
@@ -243,16 +241,18 @@ size_t stage1_lds(const mia_hip_ctx *c, bool exact)
     return stage1_layout(c->dim, c->max_nx, c->max_cx, c->max_ncx, exact).bytes;
 }
 
-int launch_warp(mia_hip_ctx *c, const double *d_params, int boff, int nb)
+int launch_warp(mia_hip_ctx *c, const double *d_params, int boff, int nb, bool timed)
 {
     if (c->dim == 3) {
         auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff,
-                           nb, d_params, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, c->n_inst,
+               (const int *)c->d_warp_prefix, boff, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
+               (const float *)c->d_M, (const float2 *)c->d_clamp, c->d_W);
     } else {
         auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, c->n_inst, c->d_warp_prefix, boff,
-                           nb, d_params, c->d_tw, c->d_ts, c->d_M, c->d_clamp, c->d_W);
+        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, c->n_inst,
+               (const int *)c->d_warp_prefix, boff, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
+               (const float *)c->d_M, (const float2 *)c->d_clamp, c->d_W);
     }
     LAUNCH_CHECK("warp");
     return MIA_HIP_OK;
@@ -265,48 +265,44 @@ int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double 
     const int n = c->n_inst;
     const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
     if (!exact) {
-        StageTimer t(c, ST_WARP, timed);
-        int st = launch_warp(c, d_params, 0, c->warp_prefix[n]);
+        int st = launch_warp(c, d_params, 0, c->warp_prefix[n], timed);
         if (st)
             return st;
     }
     {
-        StageTimer t(c, ST_STAGE1, timed);
         const int nb = c->s1_prefix[n];
         const size_t lds = stage1_lds(c, exact);
-        if (c->dim == 3) {
-            auto kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
-                                       : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
-                               d_params, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W, c->d_R, c->thr,
-                               c->d_P, c->d_ssd, c->d_cnt);
-        } else {
-            auto kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
-                                       : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), lds, c->stream, c->d_inst, n, c->d_s1_prefix, nb,
-                               d_params, c->d_tw, c->d_ts, c->d_cells, c->d_M, c->d_clamp, c->d_W, c->d_R, c->thr,
-                               c->d_P, c->d_ssd, c->d_cnt);
-        }
+        decltype(&k_stage1_3d<3, 0>) kern;
+        if (c->dim == 3)
+            kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
+                                  : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
+        else
+            kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
+                                  : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
+        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n, (const int *)c->d_s1_prefix,
+               nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells,
+               (const float *)c->d_M, (const float2 *)c->d_clamp, (const float *)c->d_W, (const float *)c->d_R,
+               c->thr, c->d_P, c->d_ssd, c->d_cnt);
         LAUNCH_CHECK("stage1");
     }
     const bool fused = c->thr == 0.0f;
     if (!fused) {
-        StageTimer t(c, ST_FINALIZE, timed);
-        hipLaunchKernelGGL(k_finalize, dim3(n), dim3(kBlock), 0, c->stream, c->d_inst, c->d_s1_prefix, c->d_ssd,
-                           c->d_cnt, c->norm, c->thr, c->weight, d_costs, c->d_gscale);
+        launch(c, ST_FINALIZE, timed, k_finalize, dim3(n), 0, (const InstDesc *)c->d_inst,
+               (const int *)c->d_s1_prefix, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr,
+               c->weight, d_costs, c->d_gscale);
         LAUNCH_CHECK("finalize");
     }
     {
-        StageTimer t(c, ST_REDUCE, timed);
         const int nb = c->red_prefix[n] + (fused ? n : 0);
         decltype(&k_reduce<3, true>) kern;
         if (c->dim == 3)
             kern = fused ? k_reduce<3, true> : k_reduce<3, false>;
         else
             kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c->stream, c->d_inst, n, c->d_red_prefix, c->d_P, c->d_tw,
-                           c->d_ts, c->d_cells, c->d_s1_prefix, c->d_ssd, c->d_cnt, c->norm, c->thr, c->weight,
-                           d_costs, c->d_gscale, d_grad);
+        launch(c, ST_REDUCE, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, n, (const int *)c->d_red_prefix,
+               (const float *)c->d_P, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells,
+               (const int *)c->d_s1_prefix, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr,
+               c->weight, d_costs, (const double *)c->d_gscale, d_grad);
         LAUNCH_CHECK("reduce");
     }
     return MIA_HIP_OK;
@@ -709,7 +705,7 @@ int mia_hip_warp(mia_hip_ctx *c, int inst, const double *params_inst, float *out
     HIP_TRY(hipMemcpyAsync(c->d_params + d.par_off, c->h_params + d.par_off, sizeof(double) * c->dim * ncoef,
                            hipMemcpyHostToDevice, c->stream));
     const int boff = c->warp_prefix[inst];
-    int st = launch_warp(c, c->d_params, boff, c->warp_prefix[inst + 1] - boff);
+    int st = launch_warp(c, c->d_params, boff, c->warp_prefix[inst + 1] - boff, false);
     if (st)
         return st;
     HIP_TRY(hipMemcpyAsync(out, c->d_W + d.img_off, sizeof(float) * (size_t)d.nx * d.ny * d.nz, hipMemcpyDeviceToHost,
