@@ -152,6 +152,33 @@ int mia_hip_profile_enable(mia_hip_ctx *ctx, int enable);
 int mia_hip_profile_read(mia_hip_ctx *ctx, double *ms, int *launches, int max_stages, int *n_stages);
 const char *mia_hip_stage_name(int stage);
 
+/* ---- per-level work of the multiresolution driver ---------------------- */
+
+/* Resample a float image (dim 2 or 3, sizes x,y[,z]) to out_size with the
+ * `scale` filter's semantics, interp=bspline:d=3: per axis that changes,
+ * downscaling applies a binomial filter of half width round(1/scale), then
+ * the cubic B-spline prefilter with mirror boundaries, then samples at
+ * i * (n_in - 1) / (n_out - 1) with cubic weights and mirrored indices;
+ * double precision between the axes.  Host buffers, synchronous.  Replaces
+ * C3DScale / C2DScale::operator() (mia/3d/filter/scale.cc:38-104,
+ * mia/2d/filter/scale.cc) with C1DScalar (mia/core/scaler1d.cc:26-230). */
+int mia_hip_scale_image(int device, int dim, const uint32_t *in_size, const float *in, const uint32_t *out_size,
+                        float *out);
+
+/* The 3D divcurl spline penalty: value c^T P c and gradient of the banded
+ * quadratic form over the coefficient grid csize (x,y,z) for an image of
+ * extent range (x,y,z) with divergence / curl weights.  Replaces
+ * C3DPPDivcurlMatrix (mia/3d/ppmatrix.cc: reset :178-262, evaluate
+ * :300-520).  coeffs: host, 3 floats per coefficient, x fastest.  value:
+ * the form; gradient (may be NULL): its derivative, 3 doubles per
+ * coefficient, unweighted and with the sign of the derivative (the caller
+ * applies -weight, mia/3d/splinetransformpenalty.cc:60-71). */
+typedef struct mia_hip_divcurl mia_hip_divcurl;
+int mia_hip_divcurl_create(int device, const uint32_t *csize, const float *range, double div_weight,
+                           double curl_weight, mia_hip_divcurl **out);
+int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs, double *value, double *gradient);
+int mia_hip_divcurl_destroy(mia_hip_divcurl *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,11 @@ SIGNATURES = {
     "mia_hip_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_profile_read": (ctypes.c_int, [_vp, _dbl_p, _c_int_p, ctypes.c_int, _c_int_p]),
     "mia_hip_stage_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "mia_hip_scale_image": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32_p, _flt_p, _u32_p, _flt_p]),
+    "mia_hip_divcurl_create": (ctypes.c_int, [ctypes.c_int, _u32_p, _flt_p, ctypes.c_double, ctypes.c_double,
+                                              ctypes.POINTER(_vp)]),
+    "mia_hip_divcurl_evaluate": (ctypes.c_int, [_vp, _flt_p, _dbl_p, _vp]),
+    "mia_hip_divcurl_destroy": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

@@ -12,14 +12,16 @@
 #include <vector>
 
 #include "../../include/mia_hip.h"
+#include "common.hh"
 #include "kernels.hh"
 #include "optimizer.hh"
 
 using namespace mia_hip;
 
+namespace mia_hip_rt {
 namespace {
-
 thread_local std::string g_last_error;
+}
 
 int fail(int code, const std::string &msg)
 {
@@ -27,19 +29,15 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(MIA_HIP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
+const char *last_error()
+{
+    return g_last_error.c_str();
+}
+} // namespace mia_hip_rt
 
-#define LAUNCH_CHECK(name)                                                                              \
-    do {                                                                                                \
-        hipError_t e_ = hipGetLastError();                                                              \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(MIA_HIP_ERR_DEVICE, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
-    } while (0)
+namespace {
+
+using mia_hip_rt::fail;
 
 enum Stage { ST_WARP = 0, ST_STAGE1, ST_FINALIZE, ST_REDUCE, ST_COUNT };
 const char *kStageNames[ST_COUNT] = {"warp", "ssd_force_adjoint1", "finalize", "adjoint_reduce"};
@@ -374,7 +372,7 @@ extern "C" {
 
 const char *mia_hip_last_error(void)
 {
-    return g_last_error.c_str();
+    return mia_hip_rt::last_error();
 }
 
 int mia_hip_spline_csize(uint32_t n, float rate, int dim)

@@ -733,3 +733,290 @@ MO_EXPORT double mo_cost(int dim, const int size[3], const float *moving, const 
     return mo_cost_grad(dim, size, moving, fixed, csize, params, degree, norm, autothresh, 0, 1.0,
                         NULL, NULL, NULL);
 }
+
+/* ------------------------------------------------------------------------ */
+/* f3: image resampling (the `scale` filter)                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Binomial filter of half width F over a line, with the boundary handling of
+ * C1DGaussFilterKernel::do_apply (mia/core/spacialkernel/gauss.cc:76-140):
+ * the taps that fall before the start read data[F-i-1 .. 0] reflected, the
+ * ones after the end read the line backwards from its last sample.  Kernel:
+ * row 2F of Pascal's triangle / 2^(2F) (gauss.cc:25-55). */
+MO_EXPORT void mo_gauss_line(const double *data, int n, int F, double *out)
+{
+    const int K = 2 * F + 1;
+    double *g = (double *)malloc(sizeof(double) * K);
+    for (int k = 0; k < K; ++k)
+        g[k] = 0.0;
+    g[0] = 1.0;
+    for (int i = 1; i < K; ++i)
+        for (int j = i; j > 0; --j)
+            g[j] += g[j - 1];
+    const double norm = pow(2.0, K - 1);
+    for (int k = 0; k < K; ++k)
+        g[k] /= norm;
+    for (int k = 0; k < n; ++k)
+        out[k] = 0.0;
+    int trgt = 0, pos = n;
+    for (int i = 0; i < F && pos >= 0 && trgt < n; ++i, --pos, ++trgt) {
+        double ht = 0.0;
+        int ik = 0;
+        for (int k = F - i; k > 0; --k, ++ik)
+            ht += data[(k - 1) < n ? (k - 1) : (n - 1)] * g[ik];
+        for (int s = 0; ik < K && s < n; ++s, ++ik)
+            ht += data[s] * g[ik];
+        out[trgt] = ht;
+    }
+    int start = 0;
+    while (pos-- > F) {
+        double ht = 0.0;
+        for (int j = 0; j < K; ++j)
+            ht += g[j] * data[start + j];
+        out[trgt++] = ht;
+        ++start;
+    }
+    for (int i = 0; i < F && pos >= 0 && trgt < n && start < n; ++i, --pos, ++trgt, ++start) {
+        double ht = 0.0;
+        int ik = 0;
+        for (int s = start; s < n && ik < K; ++s, ++ik)
+            ht += data[s] * g[ik];
+        for (int r = n - 1; ik < K && r >= 0; --r, ++ik)
+            ht += data[r] * g[ik];
+        out[trgt] = ht;
+    }
+    free(g);
+}
+
+/* C1DScalar with the cubic kernel (mia/core/scaler1d.cc:26-230): scale =
+ * (n_out - 1) / (n_in - 1); upscale prefilters the line and samples at x
+ * advanced by 1/scale per output sample (repeated addition); downscale first
+ * applies the binomial filter of half width floor(1/scale + 0.5), then
+ * prefilters and samples at i / scale; indices mirrored at the line ends. */
+MO_EXPORT void mo_scale_line(const double *in, int n_in, double *out, int n_out)
+{
+    if (n_in == 1) {
+        for (int i = 0; i < n_out; ++i)
+            out[i] = in[0];
+        return;
+    }
+    const double scale = (double)(n_out - 1) / (double)(n_in - 1);
+    if (scale == 1.0) {
+        for (int i = 0; i < n_out; ++i)
+            out[i] = in[i];
+        return;
+    }
+    double *c = (double *)malloc(sizeof(double) * n_in);
+    const double dx = 1.0 / scale;
+    if (scale > 1.0) {
+        for (int k = 0; k < n_in; ++k)
+            c[k] = in[k];
+    } else {
+        mo_gauss_line(in, n_in, (int)floor(1.0 / scale + 0.5), c);
+    }
+    mo_filter_line(c, n_in, 3);
+    double x = 0.0;
+    for (int i = 0; i < n_out; ++i, x += dx) {
+        const double xi = scale > 1.0 ? x : i * dx;
+        double w[4];
+        const int st = mo_fastfloor(xi);
+        mo_bspline3_weights(xi - st, w);
+        double v = 0.0;
+        for (int k = 0; k < 4; ++k)
+            v += w[k] * c[mo_mirror_index(st - 1 + k, n_in)];
+        out[i] = v;
+    }
+    free(c);
+}
+
+/* Separable resampling of a float image, x then y then z, double between the
+ * axes, float result (mia/3d/filter/scale.cc:38-104; the 2D filter is the
+ * same on two axes).  Axes whose size does not change are copied. */
+MO_EXPORT void mo_scale_image(const float *img, const int in_size[3], const int out_size[3], float *out)
+{
+    int cur[3] = {in_size[0], in_size[1], in_size[2]};
+    size_t n = (size_t)cur[0] * cur[1] * cur[2];
+    double *a = (double *)malloc(sizeof(double) * n);
+    for (size_t i = 0; i < n; ++i)
+        a[i] = img[i];
+    for (int ax = 0; ax < 3; ++ax) {
+        if (cur[ax] == out_size[ax])
+            continue;
+        int nxt[3] = {cur[0], cur[1], cur[2]};
+        nxt[ax] = out_size[ax];
+        const size_t m = (size_t)nxt[0] * nxt[1] * nxt[2];
+        double *b = (double *)malloc(sizeof(double) * m);
+        double *li = (double *)malloc(sizeof(double) * cur[ax]);
+        double *lo = (double *)malloc(sizeof(double) * nxt[ax]);
+        const int o1 = ax == 0 ? cur[1] : cur[0];
+        const int o2 = ax == 2 ? cur[1] : cur[2];
+        for (int q = 0; q < o2; ++q)
+            for (int p = 0; p < o1; ++p) {
+                for (int k = 0; k < cur[ax]; ++k) {
+                    const int x = ax == 0 ? k : p, y = ax == 1 ? k : (ax == 0 ? p : q), z = ax == 2 ? k : q;
+                    li[k] = a[((size_t)z * cur[1] + y) * cur[0] + x];
+                }
+                mo_scale_line(li, cur[ax], lo, nxt[ax]);
+                for (int k = 0; k < nxt[ax]; ++k) {
+                    const int x = ax == 0 ? k : p, y = ax == 1 ? k : (ax == 0 ? p : q), z = ax == 2 ? k : q;
+                    b[((size_t)z * nxt[1] + y) * nxt[0] + x] = lo[k];
+                }
+            }
+        free(li);
+        free(lo);
+        free(a);
+        a = b;
+        cur[ax] = nxt[ax];
+    }
+    n = (size_t)cur[0] * cur[1] * cur[2];
+    for (size_t i = 0; i < n; ++i)
+        out[i] = (float)a[i];
+    free(a);
+}
+
+/* ------------------------------------------------------------------------ */
+/* f2: the divcurl penalty                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Cubic B-spline and its derivatives, bspline<3,0..2>::apply
+ * (mia/core/splinekernel/bspline.cc:328-375). */
+MO_EXPORT double mo_bspline3_deg(double x, int deg)
+{
+    if (deg == 0) {
+        x = fabs(x);
+        if (x <= 1.0)
+            return 2.0 / 3.0 - x * x * (1 - 0.5 * x);
+        if (x < 2.0)
+            return (2.0 - x) * (2.0 - x) * (2.0 - x) / 6.0;
+        return 0.0;
+    }
+    if (deg == 1) {
+        const double ax = fabs(x);
+        if (ax <= 1.0)
+            return (1.5 * ax - 2) * x;
+        if (ax <= 2.0)
+            return (x > 0.0 ? -0.5 : 0.5) * (2 - ax) * (2 - ax);
+        return 0.0;
+    }
+    x = fabs(x);
+    if (x > 2.0)
+        return 0.0;
+    if (x > 1.0)
+        return 2.0 - x;
+    return 3.0 * x - 2.0;
+}
+
+/* integrate2 (mia/core/splinekernel.cc:287-315) with n = 1: composite
+ * Simpson (mia/core/simpson.hh:36-52), 8 intervals per unit, of
+ * B^(d1)(x - s1) B^(d2)(x - s2) over the overlap of both supports and [x0, L]. */
+MO_EXPORT double mo_integrate2(double s1, double s2, int d1, int d2, double x0, double L)
+{
+    double a = s1 - 2.0, b = s1 + 2.0;
+    if (a < s2 - 2.0) a = s2 - 2.0;
+    if (a < x0) a = x0;
+    if (b > s2 + 2.0) b = s2 + 2.0;
+    if (b > L) b = L;
+    if (b <= a)
+        return 0.0;
+    const size_t iv = (size_t)(8 * (b - a));
+    const double dx = (b - a) / iv;
+    double sum = 0.5 * (mo_bspline3_deg(a - s1, d1) * mo_bspline3_deg(a - s2, d2) +
+                        mo_bspline3_deg(b - s1, d1) * mo_bspline3_deg(b - s2, d2));
+    double x = a + dx;
+    for (size_t k = 1; k < iv; ++k, x += dx)
+        sum += mo_bspline3_deg(x - s1, d1) * mo_bspline3_deg(x - s2, d2);
+    x = a + 0.5 * dx;
+    for (size_t k = 0; k < iv; ++k, x += dx)
+        sum += 2 * mo_bspline3_deg(x - s1, d1) * mo_bspline3_deg(x - s2, d2);
+    return sum * dx / 3.0;
+}
+
+/* The 9^3 matrix cells of C3DPPDivcurlMatrixImpl::reset
+ * (mia/3d/ppmatrix.cc:178-262), 6 doubles each in the reference's field
+ * order vxx, vxz, vxy, vyy, vyz, vzz; h = (csize - 1) / range in float. */
+MO_EXPORT void mo_pp_cells(const int csize[3], const float range[3], double wdiv, double wrot, double *cells)
+{
+    float h1[3], h2[3], h3[3], h4[3];
+    for (int a = 0; a < 3; ++a) {
+        h1[a] = (float)(csize[a] - 1) / range[a];
+        h2[a] = h1[a] * h1[a];
+        h3[a] = h1[a] * h2[a];
+        h4[a] = h1[a] * h3[a];
+    }
+    const double gs = 1.0 / (h1[0] * h1[1] * h1[2]);
+    const double wsum = wdiv + wrot, wdelta = 2 * (wdiv - wrot);
+    size_t k = 0;
+#define MO_I(d, p, q) mo_integrate2(4.0, (d) + 4.0, p, q, 0.0, 8.0)
+    for (int z = -4; z <= 4; ++z)
+        for (int y = -4; y <= 4; ++y)
+            for (int x = -4; x <= 4; ++x, ++k) {
+                const double r00z = MO_I(z, 0, 0), r01z = h1[2] * MO_I(z, 0, 1), r11z = h2[2] * MO_I(z, 1, 1);
+                const double r12z = h3[2] * MO_I(z, 1, 2), r22z = h4[2] * MO_I(z, 2, 2);
+                const double r00y = MO_I(y, 0, 0), r01y = h1[1] * MO_I(y, 0, 1), r10y = h1[1] * MO_I(y, 1, 0);
+                const double r11y = h2[1] * MO_I(y, 1, 1), r12y = h3[1] * MO_I(y, 1, 2);
+                const double r21y = h3[1] * MO_I(y, 2, 1), r22y = h4[1] * MO_I(y, 2, 2);
+                const double r00x = MO_I(x, 0, 0), r01x = h1[0] * MO_I(x, 0, 1), r10x = h1[0] * MO_I(x, 1, 0);
+                const double r11x = h2[0] * MO_I(x, 1, 1), r21x = h3[0] * MO_I(x, 2, 1);
+                const double r22x = h4[0] * MO_I(x, 2, 2);
+                const double r200200 = r22x * r00y * r00z, r020020 = r00x * r22y * r00z;
+                const double r002002 = r00x * r00y * r22z, r110110 = r11x * r11y * r00z;
+                const double r101101 = r11x * r00y * r11z, r011011 = r00x * r11y * r11z;
+                double *c = cells + 6 * k;
+                c[0] = gs * (r002002 * wrot + r011011 * 2.0 * wrot + r020020 * wrot + r101101 * wsum +
+                             r110110 * wsum + r200200 * wdiv);
+                c[3] = gs * (r002002 * wrot + r011011 * wsum + r020020 * wdiv + r101101 * 2 * wrot +
+                             r110110 * wsum + r200200 * wrot);
+                c[5] = gs * (r002002 * wdiv + r011011 * wsum + r020020 * wrot + r101101 * wsum +
+                             r110110 * wrot * 2 + r200200 * wrot);
+                c[1] = c[2] = c[4] = 0.0;
+                if (wdelta != 0.0) {
+                    c[2] = gs * wdelta * (r21x * r01y * r00z + r10x * r12y * r00z + r01x * r10y * r11z);
+                    c[1] = gs * wdelta * (r21x * r00y * r01z + r10x * r11y * r01z + r10x * r00y * r12z);
+                    c[4] = gs * wdelta * (r11x * r10y * r01z + r00x * r21y * r01z + r00x * r10y * r12z);
+                }
+            }
+#undef MO_I
+}
+
+/* C3DPPDivcurlMatrixImpl::evaluate (mia/3d/ppmatrix.cc:300-505): value
+ * sum_i sum_j c_i (P_(i-j)) c_j over j in [i-4, i+3] per axis and, if grad
+ * is not NULL, its derivative per coefficient; coefficients 3 floats each. */
+MO_EXPORT double mo_divcurl(const int cs[3], const double *cells, const float *coef, double *grad)
+{
+    const int nx = cs[0], ny = cs[1], nz = cs[2];
+    double total = 0.0;
+    for (int zi = 0; zi < nz; ++zi) {
+        double r[6] = {0, 0, 0, 0, 0, 0};
+        for (int yi = 0; yi < ny; ++yi)
+            for (int xi = 0; xi < nx; ++xi) {
+                const size_t i = ((size_t)zi * ny + yi) * nx + xi;
+                const double cix = coef[3 * i], ciy = coef[3 * i + 1], ciz = coef[3 * i + 2];
+                double g[3] = {0, 0, 0};
+                for (int zj = (zi - 4 > 0 ? zi - 4 : 0); zj < (zi + 4 < nz ? zi + 4 : nz); ++zj)
+                    for (int yj = (yi - 4 > 0 ? yi - 4 : 0); yj < (yi + 4 < ny ? yi + 4 : ny); ++yj)
+                        for (int xj = (xi - 4 > 0 ? xi - 4 : 0); xj < (xi + 4 < nx ? xi + 4 : nx); ++xj) {
+                            const size_t j = ((size_t)zj * ny + yj) * nx + xj;
+                            const double *p = cells + 6 * (((size_t)(zi - zj + 4) * 9 + (yi - yj + 4)) * 9 +
+                                                           (xi - xj + 4));
+                            const double cx = coef[3 * j], cy = coef[3 * j + 1], cz = coef[3 * j + 2];
+                            const double vxx = p[0], vxz = p[1], vxy = p[2], vyy = p[3], vyz = p[4], vzz = p[5];
+                            r[0] += cix * cx * vxx;
+                            r[1] += cix * cy * vxy;
+                            r[2] += ciy * cy * vyy;
+                            r[3] += ciz * cx * vxz;
+                            r[4] += ciy * cz * vyz;
+                            r[5] += ciz * cz * vzz;
+                            g[0] += 2 * cx * vxx + cy * vxy + cz * vxz;
+                            g[1] += 2 * cy * vyy + cz * vyz + cx * vxy;
+                            g[2] += 2 * cz * vzz + cx * vxz + cy * vyz;
+                        }
+                if (grad) {
+                    grad[3 * i] = g[0];
+                    grad[3 * i + 1] = g[1];
+                    grad[3 * i + 2] = g[2];
+                }
+            }
+        total += r[0] + r[1] + r[2] + r[3] + r[4] + r[5];
+    }
+    return total;
+}

@@ -71,6 +71,17 @@ def lib():
         L.mo_cost.argtypes = [ctypes.c_int, _ip, _fp, _fp, _ip, _dp, ctypes.c_int, ctypes.c_int,
                               ctypes.c_float]
         L.mo_cost.restype = ctypes.c_double
+        L.mo_gauss_line.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.mo_scale_line.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.mo_scale_image.argtypes = [_fp, _ip, _ip, _fp]
+        L.mo_bspline3_deg.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.mo_bspline3_deg.restype = ctypes.c_double
+        L.mo_integrate2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_double]
+        L.mo_integrate2.restype = ctypes.c_double
+        L.mo_pp_cells.argtypes = [_ip, _fp, ctypes.c_double, ctypes.c_double, _dp]
+        L.mo_divcurl.argtypes = [_ip, _dp, _fp, ctypes.c_void_p]
+        L.mo_divcurl.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -247,3 +258,51 @@ def cost(moving, fixed, csize, params, degree=3, norm=False, autothresh=0.0):
                          np.ascontiguousarray(fixed, dtype=np.float32), cs,
                          np.ascontiguousarray(params, dtype=np.float64), degree, int(norm),
                          float(autothresh))
+
+
+def gauss_line(values, half_width):
+    """Binomial filter of half width F with the reference's boundary taps."""
+    x = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().mo_gauss_line(x, x.size, int(half_width), out)
+    return out
+
+
+def scale_line(values, n_out):
+    x = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros(int(n_out))
+    lib().mo_scale_line(x, x.size, out, int(n_out))
+    return out
+
+
+def scale_image(img, out_shape):
+    """Resample a float image (numpy (z,y,x) or (y,x)) to out_shape."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros(tuple(out_shape), dtype=np.float32)
+    lib().mo_scale_image(img, _size3(img.shape), _size3(out.shape), out)
+    return out
+
+
+def bspline3_deg(x, deg):
+    return lib().mo_bspline3_deg(float(x), int(deg))
+
+
+def integrate2(s1, s2, d1, d2, x0=0.0, L=8.0):
+    return lib().mo_integrate2(float(s1), float(s2), int(d1), int(d2), float(x0), float(L))
+
+
+def pp_cells(csize, rng, wdiv, wrot):
+    """9^3 x 6 divcurl matrix cells (vxx, vxz, vxy, vyy, vyz, vzz); csize/rng (x,y,z)."""
+    cells = np.zeros(9 * 9 * 9 * 6)
+    lib().mo_pp_cells(np.asarray(csize, dtype=np.int32), np.asarray(rng, dtype=np.float32), float(wdiv),
+                      float(wrot), cells)
+    return cells
+
+
+def divcurl(csize, cells, coef, with_grad=True):
+    """Value (and gradient, 3 per coefficient) of the divcurl form; coef float32 (n, 3) x fastest."""
+    c = np.ascontiguousarray(coef, dtype=np.float32).reshape(-1)
+    g = np.zeros(c.size) if with_grad else None
+    v = lib().mo_divcurl(np.asarray(csize, dtype=np.int32), np.ascontiguousarray(cells), c,
+                         g.ctypes.data if with_grad else None)
+    return (v, g) if with_grad else v

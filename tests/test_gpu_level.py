@@ -1,0 +1,91 @@
+"""GPU tests of the per-level work through the C ABI (image resampling with
+the `scale` filter's semantics, the divcurl penalty) against the FP64
+oracle, and the C++ host-layer GPU test program (full cost, cost list,
+minimizer problem with penalty, warp, multiresolution registrations)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _img(shape, seed):
+    from mia_amd import synth
+    return synth.smoothed_noise(shape, 2.0, seed)
+
+
+@pytest.mark.parametrize("a,b", [((80, 100), (40, 50)), ((45, 60), (150, 170)), ((64, 64), (17, 64)),
+                                 ((32, 36, 40), (16, 18, 20)), ((16, 20, 24), (33, 20, 48)),
+                                 ((40, 48, 48), (10, 12, 12))])
+def test_scale_image_matches_oracle(a, b):
+    from mia_amd import level
+    img = _img(a, 3)
+    out = level.scale_image(img, b)
+    ref = O.scale_image(img, b)
+    assert out.shape == tuple(b)
+    assert np.abs(out - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_scale_kat_cosine_rows():
+    """The reference's 1D scaler test (mia/core/test_scaler1d.cc:116-133) on
+    rows of a 2D image: 256 -> 500 and 256 -> 130 within 0.1 %."""
+    from mia_amd import level
+    row = (200 * np.cos(2 * np.pi / 255.0 * np.arange(256))).astype(np.float32)
+    img = np.stack([row, row, row])
+    for tgt in (500, 130):
+        out = level.scale_image(img, (3, tgt))
+        fx = 200 * np.cos(2 * np.pi / (tgt - 1) * np.arange(tgt))
+        for r in out:
+            m = np.abs(fx) > 1e-2
+            assert np.all(np.abs(r[m] - fx[m]) <= 1e-3 * np.abs(fx[m]) + 1e-4)
+
+
+def test_scale_errors():
+    from mia_amd import level
+    with pytest.raises(RuntimeError):
+        level.scale_image(np.ones((8, 8), np.float32), (8, 1))
+    with pytest.raises(ValueError):
+        level.scale_image(np.ones((8,), np.float32), (4,))
+
+
+@pytest.mark.parametrize("cs,rng,w", [((13, 11, 9), (40.0, 36.0, 30.0), (1.0, 1.0)),
+                                      ((10, 10, 10), (32.0, 32.0, 32.0), (1.0, 0.0)),
+                                      ((26, 26, 22), (192.0, 192.0, 160.0), (0.3, 2.0))])
+def test_divcurl_matches_oracle(cs, rng, w):
+    from mia_amd import level
+    r = np.random.default_rng(5)
+    c = r.normal(size=(int(np.prod(cs)), 3)).astype(np.float32)
+    v_ref, g_ref = O.divcurl(cs, O.pp_cells(cs, rng, *w), c)
+    with level.DivcurlPenalty(cs, rng, *w) as p:
+        v, g = p.evaluate(c)
+        v2 = p.evaluate(c, with_grad=False)
+    assert v == pytest.approx(v_ref, rel=1e-10)
+    assert v2 == v
+    assert np.abs(g - g_ref).max() <= 1e-11 * np.abs(g_ref).max()
+
+
+def test_divcurl_reference_kat_on_gpu():
+    """test_nocurl_bspline3_7_4 (mia/3d/test_ppmatrix.cc:132-150) on the GPU."""
+    from mia_amd import level
+    n, dsize, rngv = 19, 9, 4.0
+    ax = (np.arange(n) - dsize) * (rngv / dsize)
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    e = np.exp(-X * X - Y * Y - Z * Z)
+    coef = np.stack([O.prefilter(cc.astype(np.float32), 3)[0] for cc in (X * e, Y * e, Z * e)], axis=-1)
+    test = 105.0 * np.pi ** 1.5 / (np.sqrt(2.0) * 8.0)
+    with level.DivcurlPenalty((n, n, n), (8.0, 8.0, 8.0), 1.0, 0.0) as p:
+        assert p.evaluate(coef, with_grad=False) == pytest.approx(test, rel=0.008)
+
+
+def test_host_layer_gpu_cpp():
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "test_nonrigid_gpu")
+    assert os.path.exists(exe), "tests/cpp/bin/test_nonrigid_gpu is not built: run __graft_entry__.build()"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr[-2000:]
+    assert out.stdout.strip().endswith("OK")

@@ -38,3 +38,23 @@ def test_normalize_pair_moments():
     both = np.concatenate([na.ravel(), nb.ravel()]).astype(np.float64)
     assert abs(both.mean()) < 1e-5
     assert abs(both.std(ddof=1) - 1.0) < 1e-5
+
+
+def _cpp_bin(name):
+    b = os.path.join(ROOT, "tests", "cpp", "bin", name)
+    if not os.path.exists(b):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"), b])
+    return b
+
+
+def test_nonrigid_host_cpp():
+    """C++ mirror of the reference's classes, host-side cases
+    (tests/cpp/test_nonrigid_host.cc), with the plugin module that overrides
+    the library's `ssd` plugin by priority."""
+    exe = _cpp_bin("test_nonrigid_host")
+    mod = _cpp_bin("test_plugin_module.so")
+    env = dict(os.environ, MIA_TEST_PLUGIN_MODULE=mod)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=env)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("OK")
