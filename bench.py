@@ -189,6 +189,61 @@ def series_2d(n_frames, minimizer):
             "seconds": {k: round(v, 5) for k, v in t.items()}}
 
 
+def registration_3d(threads):
+    """Full 3D registration: 192x192x160 smoothed-noise fixed volume, moving
+    volume warped by a seeded B-spline displacement (rate 8, coefficients
+    N(0, 1.5) clipped to 5 voxels); three levels (1/4, 1/2, full), cost
+    `image:cost=[ssd]` plus the divcurl penalty weight 0.01, spline rate 8,
+    L-BFGS with 200 evaluations per minimisation.  Per level: GPU wall time,
+    evaluations, cost; and the CPU port's time for one cost + gradient
+    evaluation at that level's size, times the level's evaluations (an
+    estimate of the CPU path's level time without the penalty)."""
+    import mia_amd
+    from mia_amd import level, series, synth
+    shape = (160, 192, 192)
+    fix = synth.smoothed_noise(shape, 4.0, 21)
+    cs = tuple(mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1])
+    p = synth.random_spline_params(cs, 1.5, 5.0, 22)
+    with mia_amd.HipContext(0) as c:
+        c.setup(3, [shape], [cs], 3)
+        c.set_images(0, fix, fix)
+        mov = c.warp(0, p)
+    transform = "spline:rate=8,penalty=[divcurl:weight=0.01]"
+    t0 = time.perf_counter()
+    params, grid, stats, final = series.nonrigid_register(mov, fix, transform, "image:cost=[ssd]",
+                                                          "lbfgs:maxiter=200", levels=3)
+    wall = time.perf_counter() - t0
+    a, b = series.normalize_pair(mov, fix)
+    c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
+    log("registration 3D: %.2f s, final cost %.6g (identity %.6g)" % (wall, final, c_id))
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    L.mcp_cost_grad_3d.argtypes = [fp, fp] + [ctypes.c_int] * 6 + [dp, ctypes.c_int, dp, dp]
+    L.mcp_cost_grad_3d.restype = ctypes.c_double
+    levels = []
+    for st in stats:
+        sz = st["size"]
+        lshape = (sz[2], sz[1], sz[0])
+        la = np.ascontiguousarray(level.scale_image(a, lshape) if lshape != a.shape else a)
+        lb = np.ascontiguousarray(level.scale_image(b, lshape) if lshape != b.shape else b)
+        gcs = st["coeff_size"]
+        q = np.zeros(3 * gcs[0] * gcs[1] * gcs[2])
+        g = np.zeros_like(q)
+        tm = np.zeros(7)
+        L.mcp_cost_grad_3d(la, lb, sz[0], sz[1], sz[2], gcs[0], gcs[1], gcs[2], q, threads, g, tm)
+        t1 = time.perf_counter()
+        L.mcp_cost_grad_3d(la, lb, sz[0], sz[1], sz[2], gcs[0], gcs[1], gcs[2], q, threads, g, tm)
+        t_eval = time.perf_counter() - t1
+        levels.append({"size": list(sz), "coeff_size": list(gcs), "evaluations": st["evaluations"],
+                       "gpu_seconds": round(st["seconds"], 4), "cost": st["cost"],
+                       "cpu_port_seconds_per_evaluation": round(t_eval, 5),
+                       "cpu_port_level_seconds_estimate": round(t_eval * st["evaluations"], 3)})
+    return {"workload": "3D 192x192x160, 3 levels, spline rate 8 + divcurl 0.01, ssd, lbfgs maxiter=200",
+            "seconds": round(wall, 3), "final_cost": final, "identity_cost": c_id,
+            "coeff_size": list(grid), "cpu_threads": threads, "levels": levels}
+
+
 def copy_calibration(mib, reps):
     """Device-to-device copies of `mib` MiB (the HIP runtime's blit kernel): a
     transfer of known size against which rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -221,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-series", action="store_true", help="skip the batched 2D series measurement")
+    ap.add_argument("--no-registration", action="store_true",
+                    help="skip the 3D multiresolution registration (192x192x160, 3 levels, divcurl)")
     ap.add_argument("--series-frames", type=int, default=64)
     args = ap.parse_args()
 
@@ -340,6 +397,9 @@ def main():
     ser = None
     if not args.no_series:
         ser = series_2d(args.series_frames, "lbfgs:maxiter=100")
+    reg = None
+    if not args.no_registration:
+        reg = registration_3d(args.cpu_threads or min(16, os.cpu_count() or 1))
 
     out = {
         "metric": METRIC,
@@ -364,6 +424,7 @@ def main():
         "kernel_time_us_per_step": round(tsum, 2),
         "parity_vs_cpu_port": parity,
         "series_2d": ser,
+        "registration_3d": reg,
     }
     for c in ctxs:
         c.close()

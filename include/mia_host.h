@@ -57,6 +57,33 @@ int mia_host_series_run(mia_host_series *s, const char *minimizer, int mode, dou
 
 int mia_host_series_destroy(mia_host_series *s);
 
+/* ---- multiresolution nonrigid registration ------------------------------ */
+
+/* Per-level record of mia_host_nonrigid_register. */
+typedef struct {
+    uint32_t size[3];       /* image size of the level */
+    uint32_t coeff_size[3]; /* coefficient grid after the level's refinement */
+    int evaluations;        /* cost + gradient evaluations at the level */
+    double cost;            /* best cost (costs + penalty) at the level */
+    double seconds;         /* wall time of the level */
+} mia_host_level_stats;
+
+/* Register src to ref (dim 2 or 3, size x,y[,z], float images) as the
+ * reference's nonrigid driver does (mia/template/nonrigidregister.cxx:192-296):
+ * images mapped jointly to mean 0 / sigma 1, then for each of `levels`
+ * levels (image size / 2^k): create or upscale the transformation, resample
+ * the images, minimise, refine the coefficient grid, minimise again if it
+ * grew.  transform: "spline:rate=..,penalty=[divcurl:weight=..]"; cost: one
+ * or more full costs "image:cost=[ssd],weight=1" joined by '+'; minimizer:
+ * "lbfgs:maxiter=.." or "gdsq:..".  Outputs (any may be NULL): the final
+ * parameters (params_cap doubles available, *n_params set to the count),
+ * the final coefficient grid, per-level stats (up to max_levels), the final
+ * cost of the registered images at full size.  Runs on `device`. */
+int mia_host_nonrigid_register(int device, int dim, const uint32_t *size, const float *src, const float *ref,
+                               const char *transform, const char *cost, const char *minimizer, int levels,
+                               double *params_out, size_t params_cap, size_t *n_params, uint32_t *coeff_size,
+                               mia_host_level_stats *stats, int max_levels, double *final_cost);
+
 #ifdef __cplusplus
 }
 #endif

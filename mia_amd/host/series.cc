@@ -13,13 +13,22 @@
 
 namespace mia_amd {
 
-namespace {
+namespace detail {
 thread_local std::string g_err;
 
-int fail(int code, const std::string &msg)
+int host_fail(int code, const std::string &msg)
 {
     g_err = msg;
     return code;
+}
+} // namespace detail
+
+namespace {
+using detail::g_err;
+
+int fail(int code, const std::string &msg)
+{
+    return detail::host_fail(code, msg);
 }
 
 int hip_fail(int code)

@@ -6,6 +6,7 @@ lockstep through one batched GPU evaluation per iteration.
 """
 import ctypes
 import os
+import re
 
 import numpy as np
 
@@ -34,7 +35,18 @@ SIGNATURES = {
     "mia_host_series_run": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _vp, _vp, _vp,
                                            ctypes.POINTER(ctypes.c_double)]),
     "mia_host_series_destroy": (ctypes.c_int, [_vp]),
+    "mia_host_nonrigid_register": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                                  ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                                  _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                                  ctypes.POINTER(ctypes.c_uint32), _vp, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_double)]),
 }
+
+
+class LevelStats(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32 * 3), ("coeff_size", ctypes.c_uint32 * 3), ("evaluations", ctypes.c_int),
+                ("cost", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 _lib = None
 
@@ -137,3 +149,36 @@ def normalize_pair(src, ref):
         return src.astype(np.float32), ref.astype(np.float32)
     sa, sb = 1.0 / sigma, -mean / sigma
     return (a * sa + sb).astype(np.float32), (b * sa + sb).astype(np.float32)
+
+
+def nonrigid_register(src, ref, transform="spline:rate=8", cost="image:cost=[ssd]", minimizer="lbfgs:maxiter=100",
+                      levels=3, device=0):
+    """Multiresolution registration of src to ref with the C++ driver
+    (mia/template/nonrigidregister.cxx:192-296).  Returns (params,
+    coefficient grid (x,y,z), per-level stats list of dicts, final cost)."""
+    L = load()
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    if src.shape != ref.shape or src.ndim not in (2, 3):
+        raise ValueError("nonrigid_register: src and ref must be 2D or 3D images of one shape")
+    dim = src.ndim
+    sz = np.array(list(src.shape[::-1]) + [1] * (3 - dim), dtype=np.uint32)
+    # parameter count bound from the smallest rate named in the descriptor
+    # (the spline creator's default rate is 10; anisorate may lower it)
+    rates = [float(v) for v in re.findall(r"rate=\[?([0-9.eE+-]+)", transform)]
+    rates += [float(v) for v in re.findall(r"anisorate=\[([^\]]*)\]", transform) for v in v.split(",") if float(v) > 0]
+    r = max(1.0, min(rates) if rates else 10.0)
+    cap = dim * int(np.prod([int((int(s) - 1 + r) / r) + 3 for s in sz[:dim]]))
+    params = np.zeros(cap)
+    n = ctypes.c_size_t()
+    cs = (ctypes.c_uint32 * 3)()
+    stats = (LevelStats * max(levels, 1))()
+    fc = ctypes.c_double()
+    fp = ctypes.POINTER(ctypes.c_float)
+    _check(L.mia_host_nonrigid_register(int(device), dim, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                        src.ctypes.data_as(fp), ref.ctypes.data_as(fp), transform.encode(),
+                                        cost.encode(), minimizer.encode(), int(levels), params.ctypes.data, cap,
+                                        ctypes.byref(n), cs, ctypes.cast(stats, _vp), int(levels), ctypes.byref(fc)))
+    st = [{"size": tuple(s.size), "coeff_size": tuple(s.coeff_size), "evaluations": s.evaluations,
+           "cost": s.cost, "seconds": s.seconds} for s in stats[:levels]]
+    return params[:n.value].copy(), tuple(cs), st, fc.value

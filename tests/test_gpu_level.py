@@ -89,3 +89,24 @@ def test_host_layer_gpu_cpp():
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr[-2000:]
     assert out.stdout.strip().endswith("OK")
+
+
+def test_nonrigid_register_python_2d():
+    """Three-level 2D registration through the C ABI of the C++ driver:
+    per-level sizes 1/4, 1/2, 1 and a final cost well below the identity's."""
+    from mia_amd import series, synth
+    fix = synth.gaussian_blobs((96, 112), seed=7)
+    import mia_amd
+    cs = tuple(mia_amd.spline_csize(n, 16.0, 2) for n in fix.shape[::-1])
+    p = synth.random_spline_params(cs, 1.5, 3.0, 8)
+    mov = synth.spline_warp(fix, cs, p).astype(np.float32)
+    params, grid, stats, final = series.nonrigid_register(mov, fix, "spline:rate=8", "image:cost=[ssd]",
+                                                          "lbfgs:maxiter=60", levels=3)
+    assert [s["size"][:2] for s in stats] == [(28, 24), (56, 48), (112, 96)]
+    assert grid[:2] == (mia_amd.spline_csize(112, 8.0, 2), mia_amd.spline_csize(96, 8.0, 2))
+    assert params.size == 2 * grid[0] * grid[1]
+    a, b = series.normalize_pair(mov, fix)
+    c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
+    assert final < 0.3 * c_id
+    with pytest.raises(ValueError):
+        series.nonrigid_register(mov, fix, "spline:rate=8", "image:cost=[ncc]", "lbfgs:maxiter=5", levels=1)
