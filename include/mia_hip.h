@@ -152,6 +152,38 @@ int mia_hip_profile_enable(mia_hip_ctx *ctx, int enable);
 int mia_hip_profile_read(mia_hip_ctx *ctx, double *ms, int *launches, int max_stages, int *n_stages);
 const char *mia_hip_stage_name(int stage);
 
+/* ---- 3D warp tile variants and their autotuner ------------------------- */
+
+/* The tile shapes (x, y, z voxels per block) of the 3D warp kernel's
+ * variants; tile_dims receives 3 ints per variant (up to max_variants). */
+int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants);
+
+/* Force tile variant `variant` (0..n-1) for the 3D instances of the next
+ * mia_hip_setup on this context, or -1 (default) for the tuned choice per
+ * size bucket.  Results do not depend on the variant beyond FP32 rounding of
+ * the displacement contraction; every instance always uses the same variant
+ * for a given size, so batched and per-instance runs stay bitwise equal. */
+int mia_hip_set_warp_variant(mia_hip_ctx *ctx, int variant);
+/* The variant instance `inst` runs with after mia_hip_setup. */
+int mia_hip_get_warp_variant(mia_hip_ctx *ctx, int inst, int *variant);
+
+/* Time every variant on synthetic 3D instances of 2^b voxels (cubes), b in
+ * [b_min, b_max], spline rate 6 with displacements N(0, 1.5) clipped to 4
+ * voxels, for interpolation degree 3, 1 or both (0); choose the fastest per
+ * (degree, b), write the table as JSON keyed by the device's gfx
+ * architecture to json_path (may be NULL) and use it from then on.  us_out
+ * (may be NULL): mean microseconds per warp launch, n_degrees x n_buckets x
+ * n_variants. */
+int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const char *json_path, double *us_out);
+
+/* (Re)load the tuning table: json_path, or NULL for $MIA_HIP_WARP_TUNING,
+ * else <library dir>/../tuning/warp3d_<arch>.json.  A table for another
+ * architecture is ignored (variant 0 everywhere). */
+int mia_hip_load_warp_tuning(int device, const char *json_path);
+/* The file the table came from (empty if none) into buf; returns the number
+ * of (degree, bucket) entries. */
+int mia_hip_warp_tuning_source(char *buf, int len);
+
 /* ---- per-level work of the multiresolution driver ---------------------- */
 
 /* Resample a float image (dim 2 or 3, sizes x,y[,z]) to out_size with the

@@ -53,6 +53,13 @@ SIGNATURES = {
     "mia_hip_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_profile_read": (ctypes.c_int, [_vp, _dbl_p, _c_int_p, ctypes.c_int, _c_int_p]),
     "mia_hip_stage_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "mia_hip_warp_variants": (ctypes.c_int, [_c_int_p, ctypes.c_int, _c_int_p]),
+    "mia_hip_set_warp_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mia_hip_get_warp_variant": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p]),
+    "mia_hip_autotune_warp": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_char_p, _vp]),
+    "mia_hip_load_warp_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p]),
+    "mia_hip_warp_tuning_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "mia_hip_scale_image": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32_p, _flt_p, _u32_p, _flt_p]),
     "mia_hip_divcurl_create": (ctypes.c_int, [ctypes.c_int, _u32_p, _flt_p, ctypes.c_double, ctypes.c_double,
                                               ctypes.POINTER(_vp)]),

@@ -46,6 +46,15 @@ class HipContext:
         self.n_params_total = 0
 
     # -- set-up ---------------------------------------------------------------
+    def set_warp_variant(self, variant):
+        """Force a 3D warp tile variant for the next setup (-1: tuned)."""
+        check(self._L.mia_hip_set_warp_variant(self._h, int(variant)))
+
+    def warp_variant(self, inst):
+        v = ctypes.c_int()
+        check(self._L.mia_hip_get_warp_variant(self._h, int(inst), ctypes.byref(v)))
+        return v.value
+
     def setup(self, dim, shapes, csizes, degree=3):
         """shapes: numpy shapes per instance; csizes: coefficient grid sizes per
         instance in (x, y[, z]) order."""

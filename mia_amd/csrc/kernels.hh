@@ -37,15 +37,37 @@ namespace mia_hip {
 constexpr int kBlock = 256;
 constexpr int kMaxRow = 512;       // longest x row the stage-1 kernel keeps in LDS
 // 3D warp tile and LDS budgets
-constexpr int kW3TX = 32, kW3TY = 8, kW3TZ = 8;
-// The displacement intermediates (phases 1-2) and the box of moving-image
-// taps (phases 3-4) share one LDS buffer.  The box has compile-time pitches
-// (40 x 16 floats per row / slice, at most 15 slices) so that the 64 taps of
-// a voxel are one base address plus immediate offsets; 9600 floats keep a
-// block under 40 KiB so that four blocks (16 waves) fit a CU.
-constexpr int kW3PX = 40, kW3PY = 16, kW3PZ = 15;
-constexpr int kW3Buf = kW3PX * kW3PY * kW3PZ;       // 9600 floats
-constexpr int kW3Buf4 = kW3Buf / 4;                 // float4 capacity for the intermediates
+// Tile shapes of the 3D warp: TX x TY threads (TX * TY = kBlock), TZ voxels
+// per thread along z.  The displacement intermediates (phases 1-2) and the box
+// of moving-image taps (phases 3-4) share one LDS buffer; the box has
+// compile-time pitches PX x PY (x PZ slices) so that the 64 taps of a voxel are
+// one base address plus immediate offsets.  A wave covers 64 / TX rows of the
+// tile, 4 rows apart; PX is chosen so that those rows start on different
+// quarters of the LDS banks.
+template <int TX_, int TY_, int TZ_, int PX_, int PY_, int PZ_>
+struct W3Tile {
+    static constexpr int TX = TX_, TY = TY_, TZ = TZ_, PX = PX_, PY = PY_, PZ = PZ_;
+    static_assert(TX * TY == kBlock, "one thread per (x, y) column");
+    static constexpr int LOG_TX = TX == 16 ? 4 : TX == 32 ? 5 : 6;
+    static constexpr int ROWS = 64 / TX; // tile rows per wave
+    static constexpr int Buf = PX * PY * PZ;
+    static constexpr int Buf4 = (Buf + 3) / 4;
+};
+// the variants the autotuner chooses from (index = variant id)
+using W3T0 = W3Tile<32, 8, 8, 40, 16, 15>;  // 38.4 KB LDS: 4 blocks per CU
+using W3T1 = W3Tile<32, 8, 4, 40, 16, 12>;  // 30.7 KB: 5 blocks per CU
+using W3T2 = W3Tile<64, 4, 8, 72, 12, 15>;  // 51.8 KB: 3 blocks per CU, one row per wave
+using W3T3 = W3Tile<16, 16, 8, 36, 24, 15>; // 51.8 KB: 3 blocks per CU
+constexpr int kW3Variants = 4;
+__host__ __device__ inline void w3_tile_dims(int v, int &tx, int &ty, int &tz)
+{
+    switch (v) {
+    case 1: tx = W3T1::TX; ty = W3T1::TY; tz = W3T1::TZ; break;
+    case 2: tx = W3T2::TX; ty = W3T2::TY; tz = W3T2::TZ; break;
+    case 3: tx = W3T3::TX; ty = W3T3::TY; tz = W3T3::TZ; break;
+    default: tx = W3T0::TX; ty = W3T0::TY; tz = W3T0::TZ; break;
+    }
+}
 // 2D warp tile and LDS budgets
 constexpr int kW2TX = 64, kW2TY = 32;
 constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2
@@ -558,57 +580,60 @@ __global__ void __launch_bounds__(kBlock) k_prefilter_lines(float *__restrict__ 
 //            compile-time offsets, clamps and stores W.  A tile whose box
 //            exceeds the 40 x 16 x 15 pitches gathers from global memory with
 //            the same arithmetic.
-template <int DEG>
+template <int DEG, class TL>
 __global__ void __launch_bounds__(kBlock)
-    k_warp3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int boff, int nblocks,
+    k_warp3d(const InstDesc *__restrict__ inst, const int *__restrict__ ids, int n_inst,
+             const int *__restrict__ prefix, int boff, int nblocks,
              const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
     constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
-    __shared__ float4 buf4[kW3Buf4];
+    __shared__ float4 buf4[TL::Buf4];
     __shared__ float red[6][kBlock / 64];
-    __shared__ float4 wx_s[kW3TX], wy_s[kW3TY], wz_s[kW3TZ];
-    __shared__ int sx_s[kW3TX], sy_s[kW3TY], sz_s[kW3TZ];
+    __shared__ float4 wx_s[TL::TX], wy_s[TL::TY], wz_s[TL::TZ];
+    __shared__ int sx_s[TL::TX], sy_s[TL::TY], sz_s[TL::TZ];
     float *region = (float *)buf4;
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
-    const int i = find_inst(prefix, n_inst, b);
+    const int gi = find_inst(prefix, n_inst, b); // position in this launch's instance list
+    const int i = ids[gi];
     const InstDesc d = inst[i];
-    const int lb = b - prefix[i];
-    const int tiles_x = (d.nx + kW3TX - 1) / kW3TX;
-    const int tiles_y = (d.ny + kW3TY - 1) / kW3TY;
+    const int lb = b - prefix[gi];
+    const int tiles_x = (d.nx + TL::TX - 1) / TL::TX;
+    const int tiles_y = (d.ny + TL::TY - 1) / TL::TY;
     const int tx = lb % tiles_x;
     const int rest = lb / tiles_x;
-    const int x0 = tx * kW3TX, y0 = (rest % tiles_y) * kW3TY, z0 = (rest / tiles_y) * kW3TZ;
-    // a wave holds rows yl and yl + 4 of the tile: their LDS box rows are
-    // 4 * kW3PX = 160 floats apart, so the two half-waves use disjoint banks
-    const int xl = threadIdx.x & (kW3TX - 1), yl = ((threadIdx.x >> 5) & 1) * 4 + (threadIdx.x >> 6);
+    const int x0 = tx * TL::TX, y0 = (rest % tiles_y) * TL::TY, z0 = (rest / tiles_y) * TL::TZ;
+    // a wave holds tile rows w, w + 4, ... (64 / TX of them); their LDS box
+    // rows are 4 * PX floats apart, on different quarters of the banks
+    const int xl = threadIdx.x & (TL::TX - 1);
+    const int yl = ((threadIdx.x >> TL::LOG_TX) & (TL::ROWS - 1)) * 4 + (threadIdx.x >> 6);
     const int x = x0 + xl, y = y0 + yl;
     const bool valid = x < d.nx && y < d.ny;
-    const int nzl = min(kW3TZ, d.nz - z0);
+    const int nzl = min(TL::TZ, d.nz - z0);
     const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0], kz0 = ts[d.tab_z + z0];
-    const int wxn = ts[d.tab_x + min(x0 + kW3TX, d.nx) - 1] + 4 - kx0;
-    const int wyn = ts[d.tab_y + min(y0 + kW3TY, d.ny) - 1] + 4 - ky0;
+    const int wxn = ts[d.tab_x + min(x0 + TL::TX, d.nx) - 1] + 4 - kx0;
+    const int wyn = ts[d.tab_y + min(y0 + TL::TY, d.ny) - 1] + 4 - ky0;
     const int wzn = ts[d.tab_z + z0 + nzl - 1] + 4 - kz0;
     const double *P = params + d.par_off;
     // phase 1: spline tables of the tile (clamped to the image for the
     // partial tiles' unused lanes) and the coefficient window
-    if (threadIdx.x < kW3TX) {
+    if (threadIdx.x < TL::TX) {
         const int xx = min(x0 + (int)threadIdx.x, d.nx - 1);
         wx_s[threadIdx.x] = tw[d.tab_x + xx];
         sx_s[threadIdx.x] = ts[d.tab_x + xx] - kx0;
-    } else if (threadIdx.x < kW3TX + kW3TY) {
-        const int k = threadIdx.x - kW3TX;
+    } else if (threadIdx.x < TL::TX + TL::TY) {
+        const int k = threadIdx.x - TL::TX;
         const int yy = min(y0 + k, d.ny - 1);
         wy_s[k] = tw[d.tab_y + yy];
         sy_s[k] = ts[d.tab_y + yy] - ky0;
-    } else if (threadIdx.x < kW3TX + kW3TY + kW3TZ) {
-        const int k = threadIdx.x - kW3TX - kW3TY;
+    } else if (threadIdx.x < TL::TX + TL::TY + TL::TZ) {
+        const int k = threadIdx.x - TL::TX - TL::TY;
         const int zz = min(z0 + k, d.nz - 1);
         wz_s[k] = tw[d.tab_z + zz];
         sz_s[k] = ts[d.tab_z + zz] - kz0;
     }
-    const int nwin = wxn * wyn * wzn, nt1 = wzn * kW3TY * wxn, nt2 = wzn * kW3TY * kW3TX;
-    const bool separable = nwin + nt1 + nt2 <= kW3Buf4; // block-uniform
+    const int nwin = wxn * wyn * wzn, nt1 = wzn * TL::TY * wxn, nt2 = wzn * TL::TY * TL::TX;
+    const bool separable = nwin + nt1 + nt2 <= TL::Buf4; // block-uniform
     float4 *cwin = buf4, *T1 = buf4 + nwin, *T2 = T1 + nt1;
     if (separable) {
         for (int e = threadIdx.x; e < nwin; e += kBlock) {
@@ -624,16 +649,16 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     __syncthreads();
-    float ux[kW3TZ], uy[kW3TZ], uz[kW3TZ];
+    float ux[TL::TZ], uy[TL::TZ], uz[TL::TZ];
 #pragma unroll
-    for (int zl = 0; zl < kW3TZ; ++zl)
+    for (int zl = 0; zl < TL::TZ; ++zl)
         ux[zl] = uy[zl] = uz[zl] = 0.0f;
     if (separable) {
         // phase 2a: T1[kz][yl][kx] = sum_b wy_b(yl) * C[kz][sy(yl) + b][kx]
         for (int e = threadIdx.x; e < nt1; e += kBlock) {
             const int kx = e % wxn;
             const int t = e / wxn;
-            const int r = t % kW3TY, kz = t / kW3TY;
+            const int r = t % TL::TY, kz = t / TL::TY;
             const float4 w = wy_s[r];
             const float ws[4] = {w.x, w.y, w.z, w.w};
             const float4 *src = cwin + (kz * wyn + sy_s[r]) * wxn + kx;
@@ -650,8 +675,8 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
         // phase 2b: T2[kz][yl][xl] = sum_a wx_a(xl) * T1[kz][yl][sx(xl) + a]
         for (int e = threadIdx.x; e < nt2; e += kBlock) {
-            const int c = e & (kW3TX - 1);
-            const int t = e / kW3TX; // kz * kW3TY + yl
+            const int c = e & (TL::TX - 1);
+            const int t = e >> TL::LOG_TX; // kz * TY + yl
             const float4 w = wx_s[c];
             const float ws[4] = {w.x, w.y, w.z, w.w};
             const float4 *src = T1 + t * wxn + sx_s[c];
@@ -668,14 +693,14 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
         // phase 2c: z contraction per voxel
 #pragma unroll
-        for (int zl = 0; zl < kW3TZ; ++zl) {
+        for (int zl = 0; zl < TL::TZ; ++zl) {
             if (zl < nzl) {
                 const float4 w = wz_s[zl];
                 const float ws[4] = {w.x, w.y, w.z, w.w};
-                const float4 *src = T2 + (sz_s[zl] * kW3TY + yl) * kW3TX + xl;
+                const float4 *src = T2 + (sz_s[zl] * TL::TY + yl) * TL::TX + xl;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const float4 q = src[c * kW3TY * kW3TX];
+                    const float4 q = src[c * TL::TY * TL::TX];
                     ux[zl] = fmaf(ws[c], q.x, ux[zl]);
                     uy[zl] = fmaf(ws[c], q.y, uy[zl]);
                     uz[zl] = fmaf(ws[c], q.z, uz[zl]);
@@ -713,7 +738,7 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
 #pragma unroll
-            for (int zl = 0; zl < kW3TZ; ++zl) {
+            for (int zl = 0; zl < TL::TZ; ++zl) {
                 const int c = kzl - (zl < nzl ? sz_s[zl] : 0);
                 if (zl < nzl && c >= 0 && c < 4) {
                     const float w = ((const float *)&wz_s[zl])[c];
@@ -724,10 +749,10 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     }
-    float px[kW3TZ], py[kW3TZ], pz[kW3TZ];
+    float px[TL::TZ], py[TL::TZ], pz[TL::TZ];
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
 #pragma unroll
-    for (int zl = 0; zl < kW3TZ; ++zl) {
+    for (int zl = 0; zl < TL::TZ; ++zl) {
         px[zl] = clamp_pos((float)x - ux[zl]);
         py[zl] = clamp_pos((float)y - uy[zl]);
         pz[zl] = clamp_pos((float)(z0 + zl) - uz[zl]);
@@ -775,7 +800,7 @@ __global__ void __launch_bounds__(kBlock)
     const int RX = __builtin_amdgcn_readfirstlane(fastfloor16(mxx) - LO + NS) - rx0;
     const int RY = __builtin_amdgcn_readfirstlane(fastfloor16(mxy) - LO + NS) - ry0;
     const int RZ = __builtin_amdgcn_readfirstlane(fastfloor16(mxz) - LO + NS) - rz0;
-    const bool use_lds = RX <= kW3PX && RY <= kW3PY && RZ <= kW3PZ; // block-uniform
+    const bool use_lds = RX <= TL::PX && RY <= TL::PY && RZ <= TL::PZ; // block-uniform
     const float *Mi = M + d.img_off;
     if (use_lds) {
         // phase 3: one box row (RX <= 40 floats) per wave instruction, the
@@ -788,6 +813,11 @@ __global__ void __launch_bounds__(kBlock)
         int gx = rx0 + lane;
         if (rx0 < 0 || rx0 + RX > d.nx)
             gx = mirror_idx(xact ? gx : 0, d.nx);
+        // box columns 64.. (pitches wider than a wave)
+        const bool xact2 = TL::PX > 64 && lane + 64 < RX;
+        int gx2 = rx0 + lane + 64;
+        if (TL::PX > 64 && (rx0 < 0 || rx0 + RX > d.nx))
+            gx2 = mirror_idx(xact2 ? gx2 : 0, d.nx);
         const bool yzin = ry0 >= 0 && ry0 + RY <= d.ny && rz0 >= 0 && rz0 + RZ <= d.nz;
         const int nrows = RY * RZ;
         int iy = wv, iz = 0;
@@ -797,11 +827,13 @@ __global__ void __launch_bounds__(kBlock)
         }
         float *dst = region + lane;
         for (int r = wv; r < nrows; r += 4 * kFill3) {
-            float v[kFill3];
+            float v[kFill3], v2[TL::PX > 64 ? kFill3 : 1];
             int lo[kFill3];
 #pragma unroll
             for (int u = 0; u < kFill3; ++u) {
                 v[u] = 0.0f;
+                if constexpr (TL::PX > 64)
+                    v2[u] = 0.0f;
                 lo[u] = -1;
                 if (r + 4 * u < nrows) {
                     const int gy = yzin ? ry0 + iy : mirror_idx(ry0 + iy, d.ny);
@@ -809,7 +841,11 @@ __global__ void __launch_bounds__(kBlock)
                     const float *src = Mi + ((long long)gz * d.ny + gy) * d.nx;
                     if (xact)
                         v[u] = src[gx];
-                    lo[u] = (iz * kW3PY + iy) * kW3PX;
+                    if constexpr (TL::PX > 64) {
+                        if (xact2)
+                            v2[u] = src[gx2];
+                    }
+                    lo[u] = (iz * TL::PY + iy) * TL::PX;
                     iy += 4;
                     while (iy >= RY) {
                         iy -= RY;
@@ -818,9 +854,14 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kFill3; ++u)
+            for (int u = 0; u < kFill3; ++u) {
                 if (lo[u] >= 0 && xact)
                     dst[lo[u]] = v[u];
+                if constexpr (TL::PX > 64) {
+                    if (lo[u] >= 0 && xact2)
+                        dst[lo[u] + 64] = v2[u];
+                }
+            }
         }
         __syncthreads();
     }
@@ -832,13 +873,13 @@ __global__ void __launch_bounds__(kBlock)
     if (use_lds) {
         // phase 4: the taps at one base address plus compile-time offsets
 #pragma unroll
-        for (int zl = 0; zl < kW3TZ; ++zl) {
+        for (int zl = 0; zl < TL::TZ; ++zl) {
             if (zl < nzl) {
                 float wa[NS], wb[NS], wc[NS];
                 const int s0 = kweights<DEG, false>(px[zl], wa, nullptr);
                 const int s1 = kweights<DEG, false>(py[zl], wb, nullptr);
                 const int s2 = kweights<DEG, false>(pz[zl], wc, nullptr);
-                const int b0 = ((s2 - rz0) * kW3PY + (s1 - ry0)) * kW3PX + (s0 - rx0);
+                const int b0 = ((s2 - rz0) * TL::PY + (s1 - ry0)) * TL::PX + (s0 - rx0);
                 float v = 0.0f;
 #pragma unroll
                 for (int c = 0; c < NS; ++c) {
@@ -846,12 +887,12 @@ __global__ void __launch_bounds__(kBlock)
                     // slice are then immediate offsets from one register
                     // (left alone, the compiler forms a new address for most
                     // pairs of taps beyond the first slice)
-                    int bc = b0 + c * kW3PY * kW3PX;
+                    int bc = b0 + c * TL::PY * TL::PX;
                     asm("" : "+v"(bc));
                     float vy = 0.0f;
 #pragma unroll
                     for (int bb = 0; bb < NS; ++bb) {
-                        const float *row = region + bc + bb * kW3PX;
+                        const float *row = region + bc + bb * TL::PX;
                         float vx = 0.0f;
 #pragma unroll
                         for (int a = 0; a < NS; ++a)
@@ -865,7 +906,7 @@ __global__ void __launch_bounds__(kBlock)
         }
     } else {
 #pragma unroll
-        for (int zl = 0; zl < kW3TZ; ++zl)
+        for (int zl = 0; zl < TL::TZ; ++zl)
             if (zl < nzl)
                 Wi[zl * plane] =
                     interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, px[zl], py[zl], pz[zl], cl.x, cl.y, nullptr);
@@ -876,7 +917,8 @@ __global__ void __launch_bounds__(kBlock)
 // (rows y0 + ly + 4k).  One coefficient window per tile.
 template <int DEG>
 __global__ void __launch_bounds__(kBlock)
-    k_warp2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int boff, int nblocks,
+    k_warp2d(const InstDesc *__restrict__ inst, const int *__restrict__ ids, int n_inst,
+             const int *__restrict__ prefix, int boff, int nblocks,
              const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
@@ -886,9 +928,10 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ float region[kW2Region];
     __shared__ float red[4][kBlock / 64];
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
-    const int i = find_inst(prefix, n_inst, b);
+    const int gi = find_inst(prefix, n_inst, b);
+    const int i = ids[gi];
     const InstDesc d = inst[i];
-    const int lb = b - prefix[i];
+    const int lb = b - prefix[gi];
     const int tiles_x = (d.nx + kW2TX - 1) / kW2TX;
     const int x0 = (lb % tiles_x) * kW2TX, y0 = (lb / tiles_x) * kW2TY;
     const int x = x0 + (threadIdx.x & (kW2TX - 1));

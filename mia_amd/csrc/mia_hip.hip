@@ -3,8 +3,13 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -119,13 +124,23 @@ struct mia_hip_ctx {
     double weight = 1.0;
 
     std::vector<InstDesc> inst;
-    std::vector<int> warp_prefix, s1_prefix, red_prefix;
+    std::vector<int> s1_prefix, red_prefix;
+    // warp launches: instances grouped by tile variant (one launch per group)
+    struct WarpGroup {
+        int variant = 0;           // 3D tile variant; unused in 2D
+        std::vector<int> ids;      // instances of the group, ascending
+        std::vector<int> prefix;   // tile prefix over ids
+        int ids_off = 0, prefix_off = 0; // offsets into d_wgroups
+    };
+    std::vector<WarpGroup> wgroups;
+    std::vector<int> inst_variant, inst_group, inst_slot;
+    int force_variant = -1; // -1: tuned per size bucket
     std::vector<char> images_set;
     long long n_img = 0, n_par = 0, n_part = 0, max_vox = 0;
     int max_nx = 0, max_cx = 0, max_ncx = 0;
 
     InstDesc *d_inst = nullptr;
-    int *d_warp_prefix = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr;
+    int *d_wgroups = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr;
     float *d_M = nullptr, *d_R = nullptr, *d_W = nullptr, *d_P = nullptr;
     float2 *d_clamp = nullptr, *d_mm = nullptr;
     float4 *d_tw = nullptr;
@@ -159,7 +174,7 @@ namespace {
 
 void free_buffers(mia_hip_ctx *c)
 {
-    void *ptrs[] = {c->d_inst, c->d_warp_prefix, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
+    void *ptrs[] = {c->d_inst, c->d_wgroups, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
                     c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
@@ -183,7 +198,7 @@ void free_buffers(mia_hip_ctx *c)
     if (c->graph)
         (void)hipGraphDestroy(c->graph);
     c->d_inst = nullptr;
-    c->d_warp_prefix = c->d_s1_prefix = c->d_red_prefix = nullptr;
+    c->d_wgroups = c->d_s1_prefix = c->d_red_prefix = nullptr;
     c->d_M = c->d_R = c->d_W = c->d_P = nullptr;
     c->d_clamp = c->d_mm = nullptr;
     c->d_tw = nullptr;
@@ -193,6 +208,112 @@ void free_buffers(mia_hip_ctx *c)
     c->h_params = c->h_grad = c->h_costs = nullptr;
     c->gexec = nullptr;
     c->graph = nullptr;
+}
+
+// Warp tile variants chosen per size bucket (ceil(log2(voxels))) and
+// interpolation degree, from a JSON table keyed by the GPU architecture,
+// written by mia_hip_autotune_warp.  Source: $MIA_HIP_WARP_TUNING, else
+// <library dir>/../tuning/warp3d_<arch>.json; without a table, variant 0.
+struct WarpTuning {
+    std::mutex m;
+    bool loaded = false;
+    std::string arch, source;
+    std::map<std::pair<int, int>, int> table; // (degree, bucket) -> variant
+};
+
+WarpTuning &warp_tuning()
+{
+    static WarpTuning t;
+    return t;
+}
+
+std::string device_arch(int device)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return "";
+    std::string a(prop.gcnArchName);
+    return a.substr(0, a.find(':'));
+}
+
+std::string default_tuning_path(const std::string &arch)
+{
+    if (const char *e = std::getenv("MIA_HIP_WARP_TUNING"))
+        return e;
+    Dl_info info;
+    if (dladdr((const void *)&default_tuning_path, &info) && info.dli_fname) {
+        std::string lib(info.dli_fname);
+        const size_t k = lib.rfind('/');
+        const std::string dir = k == std::string::npos ? "." : lib.substr(0, k);
+        return dir + "/../tuning/warp3d_" + arch + ".json";
+    }
+    return "";
+}
+
+// parse the table written by write_tuning (entries
+// {"degree": D, "bucket": B, "variant": V, ...}); false if unreadable
+bool parse_tuning(const std::string &path, std::string &arch, std::map<std::pair<int, int>, int> &table)
+{
+    std::FILE *f = std::fopen(path.c_str(), "r");
+    if (!f)
+        return false;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0)
+        text.append(buf, n);
+    std::fclose(f);
+    const size_t a = text.find("\"arch\"");
+    if (a == std::string::npos)
+        return false;
+    const size_t q1 = text.find('"', text.find(':', a) + 1), q2 = text.find('"', q1 + 1);
+    if (q1 == std::string::npos || q2 == std::string::npos)
+        return false;
+    arch = text.substr(q1 + 1, q2 - q1 - 1);
+    table.clear();
+    size_t pos = 0;
+    while ((pos = text.find("{\"degree\":", pos)) != std::string::npos) {
+        int d = 0, b = 0, v = 0;
+        if (std::sscanf(text.c_str() + pos, "{\"degree\": %d, \"bucket\": %d, \"variant\": %d", &d, &b, &v) == 3 &&
+            v >= 0 && v < kW3Variants)
+            table[{d, b}] = v;
+        ++pos;
+    }
+    return true;
+}
+
+void load_tuning_locked(WarpTuning &t, int device, const char *path)
+{
+    t.loaded = true;
+    t.table.clear();
+    t.source.clear();
+    const std::string arch = device_arch(device);
+    const std::string p = path ? std::string(path) : default_tuning_path(arch);
+    std::string file_arch;
+    std::map<std::pair<int, int>, int> table;
+    if (!p.empty() && parse_tuning(p, file_arch, table) && file_arch == arch) {
+        t.table = table;
+        t.source = p;
+    }
+    t.arch = arch;
+}
+
+int tuned_variant(int device, int degree, long long nvox)
+{
+    WarpTuning &t = warp_tuning();
+    std::lock_guard<std::mutex> lock(t.m);
+    if (!t.loaded)
+        load_tuning_locked(t, device, nullptr);
+    int bucket = 0;
+    while ((1LL << bucket) < nvox)
+        ++bucket;
+    int best = 0, dist = 1 << 30;
+    for (const auto &kv : t.table)
+        if (kv.first.first == degree && std::abs(kv.first.second - bucket) < dist) {
+            dist = std::abs(kv.first.second - bucket);
+            best = kv.second;
+        }
+    return best;
 }
 
 hipEvent_t get_event(mia_hip_ctx *c)
@@ -239,18 +360,37 @@ size_t stage1_lds(const mia_hip_ctx *c, bool exact)
     return stage1_layout(c->dim, c->max_nx, c->max_cx, c->max_ncx, exact).bytes;
 }
 
-int launch_warp(mia_hip_ctx *c, const double *d_params, int boff, int nb, bool timed)
+template <int DEG>
+const void *warp3d_kernel(int variant)
 {
+    switch (variant) {
+    case 1: return (const void *)k_warp3d<DEG, W3T1>;
+    case 2: return (const void *)k_warp3d<DEG, W3T2>;
+    case 3: return (const void *)k_warp3d<DEG, W3T3>;
+    default: return (const void *)k_warp3d<DEG, W3T0>;
+    }
+}
+
+// Warp the instances of group g, blocks [boff, boff + nb) of its tile prefix.
+int launch_warp(mia_hip_ctx *c, const double *d_params, int g, int boff, int nb, bool timed)
+{
+    const mia_hip_ctx::WarpGroup &G = c->wgroups[g];
+    const int *ids = c->d_wgroups + G.ids_off, *prefix = c->d_wgroups + G.prefix_off;
+    const int n = (int)G.ids.size();
+    if (nb <= 0)
+        return MIA_HIP_OK;
     if (c->dim == 3) {
-        auto kern = c->degree == 3 ? k_warp3d<3> : k_warp3d<1>;
-        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, c->n_inst,
-               (const int *)c->d_warp_prefix, boff, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
-               (const float *)c->d_M, (const float2 *)c->d_clamp, c->d_W);
+        using K = void (*)(const InstDesc *, const int *, int, const int *, int, int, const double *, const float4 *,
+                           const int *, const float *, const float2 *, float *);
+        auto kern = (K)(c->degree == 3 ? warp3d_kernel<3>(G.variant) : warp3d_kernel<1>(G.variant));
+        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
+               (const float4 *)c->d_tw, (const int *)c->d_ts, (const float *)c->d_M, (const float2 *)c->d_clamp,
+               c->d_W);
     } else {
         auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, c->n_inst,
-               (const int *)c->d_warp_prefix, boff, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
-               (const float *)c->d_M, (const float2 *)c->d_clamp, c->d_W);
+        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
+               (const float4 *)c->d_tw, (const int *)c->d_ts, (const float *)c->d_M, (const float2 *)c->d_clamp,
+               c->d_W);
     }
     LAUNCH_CHECK("warp");
     return MIA_HIP_OK;
@@ -262,11 +402,12 @@ int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double 
 {
     const int n = c->n_inst;
     const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
-    if (!exact) {
-        int st = launch_warp(c, d_params, 0, c->warp_prefix[n], timed);
-        if (st)
-            return st;
-    }
+    if (!exact)
+        for (int g = 0; g < (int)c->wgroups.size(); ++g) {
+            int st = launch_warp(c, d_params, g, 0, c->wgroups[g].prefix.back(), timed);
+            if (st)
+                return st;
+        }
     {
         const int nb = c->s1_prefix[n];
         const size_t lds = stage1_lds(c, exact);
@@ -447,7 +588,6 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->degree = degree;
     c->n_inst = 0;
     c->inst.assign(n_inst, InstDesc());
-    c->warp_prefix.assign(n_inst + 1, 0);
     c->s1_prefix.assign(n_inst + 1, 0);
     c->red_prefix.assign(n_inst + 1, 0);
     c->images_set.assign(n_inst, 0);
@@ -508,15 +648,56 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         max_nx = std::max(max_nx, d.nx);
         max_cx = std::max(max_cx, d.cx);
         max_ncx = std::max(max_ncx, d.ncx);
-        const int tiles = dim == 3 ? ((d.nx + kW3TX - 1) / kW3TX) * ((d.ny + kW3TY - 1) / kW3TY) *
-                                         ((d.nz + kW3TZ - 1) / kW3TZ)
-                                   : ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
-        c->warp_prefix[i + 1] = c->warp_prefix[i] + tiles;
         c->s1_prefix[i + 1] = c->s1_prefix[i] + (dim == 3 ? d.nz * d.ncy : d.ncy);
         // 3D reduce: 64 gradient components per block (4 waves split the z
         // support); 2D: 256 per block
         const int per_block = dim == 3 ? 64 : kBlock;
         c->red_prefix[i + 1] = c->red_prefix[i] + (int)((dim * ncoef + per_block - 1) / per_block);
+    }
+    // warp groups: 3D instances by tile variant (tuned per size bucket
+    // unless forced), 2D in one group
+    c->inst_variant.assign(n_inst, 0);
+    c->inst_group.assign(n_inst, 0);
+    c->inst_slot.assign(n_inst, 0);
+    c->wgroups.clear();
+    {
+        int gidx[kW3Variants];
+        for (int v = 0; v < kW3Variants; ++v)
+            gidx[v] = -1;
+        for (int i = 0; i < n_inst; ++i) {
+            const InstDesc &d = c->inst[i];
+            const int v = dim == 3 ? (c->force_variant >= 0 ? c->force_variant
+                                                           : tuned_variant(c->device, degree,
+                                                                           (long long)d.nx * d.ny * d.nz))
+                                   : 0;
+            c->inst_variant[i] = v;
+            if (gidx[v] < 0) {
+                gidx[v] = (int)c->wgroups.size();
+                c->wgroups.emplace_back();
+                c->wgroups.back().variant = v;
+                c->wgroups.back().prefix.push_back(0);
+            }
+            auto &G = c->wgroups[gidx[v]];
+            int tiles;
+            if (dim == 3) {
+                int tx, ty, tz;
+                w3_tile_dims(v, tx, ty, tz);
+                tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty) * ((d.nz + tz - 1) / tz);
+            } else {
+                tiles = ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
+            }
+            c->inst_group[i] = gidx[v];
+            c->inst_slot[i] = (int)G.ids.size();
+            G.ids.push_back(i);
+            G.prefix.push_back(G.prefix.back() + tiles);
+        }
+    }
+    std::vector<int> wg;
+    for (auto &G : c->wgroups) {
+        G.ids_off = (int)wg.size();
+        wg.insert(wg.end(), G.ids.begin(), G.ids.end());
+        G.prefix_off = (int)wg.size();
+        wg.insert(wg.end(), G.prefix.begin(), G.prefix.end());
     }
     c->n_img = img;
     c->n_par = par;
@@ -541,7 +722,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     int st;
-    if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_warp_prefix, n_inst + 1)) ||
+    if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_wgroups, wg.size())) ||
         (st = dev_alloc(&c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_red_prefix, n_inst + 1)) ||
         (st = dev_alloc(&c->d_M, img)) || (st = dev_alloc(&c->d_R, img)) || (st = dev_alloc(&c->d_W, img)) ||
         (st = dev_alloc(&c->d_P, part)) || (st = dev_alloc(&c->d_clamp, n_inst)) || (st = dev_alloc(&c->d_mm, 1024)) ||
@@ -558,8 +739,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     HIP_TRY(hipHostMalloc((void **)&c->h_grad, sizeof(double) * std::max<long long>(par, 1), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_costs, sizeof(double) * n_inst, hipHostMallocDefault));
     HIP_TRY(hipMemcpyAsync(c->d_inst, c->inst.data(), sizeof(InstDesc) * n_inst, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_warp_prefix, c->warp_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_wgroups, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_s1_prefix, c->s1_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
                            c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_red_prefix, c->red_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
@@ -702,8 +882,9 @@ int mia_hip_warp(mia_hip_ctx *c, int inst, const double *params_inst, float *out
     std::memcpy(c->h_params + d.par_off, params_inst, sizeof(double) * c->dim * ncoef);
     HIP_TRY(hipMemcpyAsync(c->d_params + d.par_off, c->h_params + d.par_off, sizeof(double) * c->dim * ncoef,
                            hipMemcpyHostToDevice, c->stream));
-    const int boff = c->warp_prefix[inst];
-    int st = launch_warp(c, c->d_params, boff, c->warp_prefix[inst + 1] - boff, false);
+    const auto &G = c->wgroups[c->inst_group[inst]];
+    const int boff = G.prefix[c->inst_slot[inst]];
+    int st = launch_warp(c, c->d_params, c->inst_group[inst], boff, G.prefix[c->inst_slot[inst] + 1] - boff, false);
     if (st)
         return st;
     HIP_TRY(hipMemcpyAsync(out, c->d_W + d.img_off, sizeof(float) * (size_t)d.nx * d.ny * d.nz, hipMemcpyDeviceToHost,
@@ -949,3 +1130,167 @@ const char *mia_hip_stage_name(int stage)
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// warp tile variants and their autotuner
+// ---------------------------------------------------------------------------
+
+int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants)
+{
+    if (n_variants)
+        *n_variants = kW3Variants;
+    for (int v = 0; v < kW3Variants && v < max_variants && tile_dims; ++v)
+        w3_tile_dims(v, tile_dims[3 * v], tile_dims[3 * v + 1], tile_dims[3 * v + 2]);
+    return MIA_HIP_OK;
+}
+
+int mia_hip_set_warp_variant(mia_hip_ctx *ctx, int variant)
+{
+    if (!ctx)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (variant < -1 || variant >= kW3Variants)
+        return fail(MIA_HIP_ERR_ARG, "warp variant must be -1 (tuned) or 0.." + std::to_string(kW3Variants - 1));
+    ctx->force_variant = variant;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_get_warp_variant(mia_hip_ctx *ctx, int inst, int *variant)
+{
+    if (!ctx || !variant)
+        return fail(MIA_HIP_ERR_ARG, "null argument");
+    if (ctx->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= ctx->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    *variant = ctx->inst_variant[inst];
+    return MIA_HIP_OK;
+}
+
+int mia_hip_load_warp_tuning(int device, const char *json_path)
+{
+    WarpTuning &t = warp_tuning();
+    std::lock_guard<std::mutex> lock(t.m);
+    load_tuning_locked(t, device, json_path);
+    if (json_path && t.source.empty())
+        return fail(MIA_HIP_ERR_ARG, std::string("no warp tuning for this GPU in ") + json_path);
+    return MIA_HIP_OK;
+}
+
+int mia_hip_warp_tuning_source(char *buf, int len)
+{
+    WarpTuning &t = warp_tuning();
+    std::lock_guard<std::mutex> lock(t.m);
+    if (buf && len > 0) {
+        std::strncpy(buf, t.source.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)t.table.size();
+}
+
+namespace {
+// deterministic N(0, sd) samples clipped to +-clip (Box-Muller on a 64-bit LCG)
+std::vector<double> tuning_params(size_t n, double sd, double clip, unsigned long long seed)
+{
+    std::vector<double> p(n);
+    unsigned long long x = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    auto uni = [&]() {
+        x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        return ((x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    for (size_t k = 0; k < n; ++k) {
+        const double u1 = uni(), u2 = uni();
+        const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        p[k] = std::max(-clip, std::min(clip, sd * z));
+    }
+    return p;
+}
+} // namespace
+
+int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const char *json_path, double *us_out)
+{
+    if (degree != 0 && degree != 1 && degree != 3)
+        return fail(MIA_HIP_ERR_ARG, "autotune: degree must be 1, 3 or 0 (both)");
+    if (b_min < 12 || b_max > 26 || b_min > b_max)
+        return fail(MIA_HIP_ERR_ARG, "autotune: size buckets must satisfy 12 <= b_min <= b_max <= 26");
+    const std::string arch = device_arch(device);
+    if (arch.empty())
+        return fail(MIA_HIP_ERR_DEVICE, "autotune: cannot query device " + std::to_string(device));
+    const int degs[2] = {3, 1};
+    struct Row {
+        int degree, bucket, n, variant;
+        double us[kW3Variants];
+    };
+    std::vector<Row> rows;
+    int k_out = 0;
+    for (int di = 0; di < 2; ++di) {
+        const int deg = degs[di];
+        if (degree != 0 && degree != deg)
+            continue;
+        for (int b = b_min; b <= b_max; ++b) {
+            Row r{deg, b, (int)std::llround(std::pow(2.0, b / 3.0)), 0, {0}};
+            const uint32_t sz[3] = {(uint32_t)r.n, (uint32_t)r.n, (uint32_t)r.n};
+            const int c1 = mia_hip_spline_csize(sz[0], 6.0f, 3);
+            const uint32_t cs[3] = {(uint32_t)c1, (uint32_t)c1, (uint32_t)c1};
+            const std::vector<double> p = tuning_params(3 * (size_t)c1 * c1 * c1, 1.5, 4.0, 1000 + b);
+            std::vector<float> img((size_t)r.n * r.n * r.n, 0.0f);
+            std::vector<double> g(p.size());
+            double best = 1e300;
+            for (int v = 0; v < kW3Variants; ++v) {
+                mia_hip_ctx *ctx = nullptr;
+                int st = mia_hip_ctx_create(device, &ctx);
+                if (st)
+                    return st;
+                double cost = 0.0, ms[ST_COUNT];
+                int nl[ST_COUNT], ns = 0;
+                if ((st = mia_hip_set_warp_variant(ctx, v)) || (st = mia_hip_setup(ctx, 3, 1, sz, cs, deg)) ||
+                    (st = mia_hip_set_images(ctx, 0, img.data(), img.data())) ||
+                    (st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost)) ||
+                    (st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost)) ||
+                    (st = mia_hip_profile_enable(ctx, 1))) {
+                    mia_hip_ctx_destroy(ctx);
+                    return st;
+                }
+                for (int k = 0; k < 5 && !st; ++k)
+                    st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost);
+                if (!st)
+                    st = mia_hip_profile_read(ctx, ms, nl, ST_COUNT, &ns);
+                mia_hip_ctx_destroy(ctx);
+                if (st)
+                    return st;
+                r.us[v] = nl[ST_WARP] ? 1e3 * ms[ST_WARP] / nl[ST_WARP] : 1e300;
+                if (r.us[v] < best) {
+                    best = r.us[v];
+                    r.variant = v;
+                }
+                if (us_out)
+                    us_out[k_out * kW3Variants + v] = r.us[v];
+            }
+            ++k_out;
+            rows.push_back(r);
+        }
+    }
+    if (json_path) {
+        std::FILE *f = std::fopen(json_path, "w");
+        if (!f)
+            return fail(MIA_HIP_ERR_ARG, std::string("autotune: cannot write ") + json_path);
+        std::fprintf(f, "{\"arch\": \"%s\", \"kernel\": \"warp3d\", \"rate\": 6, \"variants\": [", arch.c_str());
+        for (int v = 0; v < kW3Variants; ++v) {
+            int tx, ty, tz;
+            w3_tile_dims(v, tx, ty, tz);
+            std::fprintf(f, "%s[%d, %d, %d]", v ? ", " : "", tx, ty, tz);
+        }
+        std::fprintf(f, "],\n \"table\": [\n");
+        for (size_t k = 0; k < rows.size(); ++k) {
+            const Row &r = rows[k];
+            std::fprintf(f, "  {\"degree\": %d, \"bucket\": %d, \"variant\": %d, \"size\": [%d, %d, %d], \"us\": [",
+                         r.degree, r.bucket, r.variant, r.n, r.n, r.n);
+            for (int v = 0; v < kW3Variants; ++v)
+                std::fprintf(f, "%s%.2f", v ? ", " : "", r.us[v]);
+            std::fprintf(f, "]}%s\n", k + 1 < rows.size() ? "," : "");
+        }
+        std::fprintf(f, " ]}\n");
+        std::fclose(f);
+        return mia_hip_load_warp_tuning(device, json_path);
+    }
+    return MIA_HIP_OK;
+}
