@@ -189,6 +189,61 @@ def series_2d(n_frames, minimizer):
             "seconds": {k: round(v, 5) for k, v in t.items()}}
 
 
+def batch_volumes_3d(n_vol=16, seed=51):
+    """The batched 3D workload: n_vol smoothed-noise volume pairs with sides
+    uniform in [64, 160], spline rate 6, coefficients N(0, 1.5) clipped to
+    4 voxels.  Returns [(moving, fixed, csize, params)]."""
+    import mia_amd
+    from mia_amd import synth
+    rng = np.random.default_rng(seed)
+    shapes = [tuple(int(v) for v in rng.integers(64, 161, size=3)) for _ in range(n_vol)]
+    data = []
+    for k, sh in enumerate(shapes):
+        fix = synth.smoothed_noise(sh, 2.0, 100 + k)
+        mov = synth.smoothed_noise(sh, 2.0, 200 + k)
+        cs = tuple(mia_amd.spline_csize(n, 6.0, 3) for n in sh[::-1])
+        data.append((mov, fix, cs, synth.random_spline_params(cs, 1.5, 4.0, 300 + k)))
+    return data
+
+
+def batched_3d(steps, n_vol=16):
+    """Batched 3D volumes (batch_volumes_3d), forward + adjoint for all
+    volumes per evaluation, cubic interpolation: throughput and the warp tile
+    each volume runs (autotuned per size bucket).  The error against the
+    FP64 oracle for these volumes is checked by tests/test_gpu_batched3d.py."""
+    import mia_amd
+    from mia_amd import autotune
+    data = batch_volumes_3d(n_vol)
+    shapes = [d[0].shape for d in data]
+    nvox = sum(int(np.prod(s)) for s in shapes)
+    c = mia_amd.HipContext(0)
+    c.setup(3, shapes, [d[2] for d in data], 3)
+    for i, d in enumerate(data):
+        c.set_images(i, d[0], d[1])
+    c.cost_grad(np.concatenate([d[3] for d in data]))
+    bufs = c.device_buffers()
+    for _ in range(3):
+        c.cost_grad_device(*bufs)
+    c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.cost_grad_device(*bufs)
+    c.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    variants = [c.warp_variant(i) for i in range(n_vol)]
+    c.close()
+    dims = autotune.variants()
+    src, _ = autotune.tuning_source()
+    buckets = {}
+    for sh, v in zip(shapes, variants):
+        buckets.setdefault(str(int(np.ceil(np.log2(np.prod(sh))))), list(dims[v]))
+    log("batched 3D: %d volumes, %.3f ms per evaluation, %.1f Mvoxel/s" % (n_vol, ms, nvox / ms / 1e3))
+    return {"workload": "%d volumes, sides U[64,160], rate 6, cubic, forward + adjoint" % n_vol,
+            "voxels": nvox, "ms_per_evaluation": round(ms, 4), "mvoxel_per_s": round(nvox / ms / 1e3, 1),
+            "tile_per_bucket": dict(sorted(buckets.items(), key=lambda kv: int(kv[0]))),
+            "tuning_table": os.path.relpath(src, ROOT) if src else None}
+
+
 def registration_3d(threads):
     """Full 3D registration: 192x192x160 smoothed-noise fixed volume, moving
     volume warped by a seeded B-spline displacement (rate 8, coefficients
@@ -276,6 +331,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-series", action="store_true", help="skip the batched 2D series measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 16-volume 3D evaluation")
     ap.add_argument("--no-registration", action="store_true",
                     help="skip the 3D multiresolution registration (192x192x160, 3 levels, divcurl)")
     ap.add_argument("--series-frames", type=int, default=64)
@@ -397,6 +453,9 @@ def main():
     ser = None
     if not args.no_series:
         ser = series_2d(args.series_frames, "lbfgs:maxiter=100")
+    bat = None
+    if not args.no_batched:
+        bat = batched_3d(20)
     reg = None
     if not args.no_registration:
         reg = registration_3d(args.cpu_threads or min(16, os.cpu_count() or 1))
@@ -424,6 +483,7 @@ def main():
         "kernel_time_us_per_step": round(tsum, 2),
         "parity_vs_cpu_port": parity,
         "series_2d": ser,
+        "batched_3d": bat,
         "registration_3d": reg,
     }
     for c in ctxs:
