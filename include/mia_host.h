@@ -84,6 +84,20 @@ int mia_host_nonrigid_register(int device, int dim, const uint32_t *size, const 
                                double *params_out, size_t params_cap, size_t *n_params, uint32_t *coeff_size,
                                mia_host_level_stats *stats, int max_levels, double *final_cost);
 
+/* ---- transformation files ----------------------------------------------- */
+
+/* Save / load a spline transformation in the layout of MIA's "xml"
+ * transformation IO (mia/3d/transio/xml.cc, mia/3d/transio/serialization.hh:36-79):
+ * creator string, size (x, y[, z]), attributes, parameters.  Save creates
+ * the transformation from `creator` and `size` and requires n to match its
+ * degrees of freedom.  Load re-creates it from the file's creator string and
+ * size; outputs (any may be NULL): size, coefficient grid, parameters
+ * (params_cap doubles available), their count, the creator string. */
+int mia_host_transform_save(const char *path, int dim, const uint32_t *size, const char *creator,
+                            const double *params, size_t n);
+int mia_host_transform_load(const char *path, int dim, uint32_t *size, uint32_t *coeff_size, double *params,
+                            size_t params_cap, size_t *n_params, char *creator, int creator_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -330,6 +330,16 @@ private:
     std::vector<LevelStats> m_stats;
 };
 
+// Transformation files in the layout of MIA's "xml" transformation IO
+// (mia/3d/transio/xml.cc, mia/3d/transio/serialization.hh:36-79; 2D
+// without size_z): creator string, size, attributes (none written, ignored
+// on load) and the parameter vector.  load_transformation re-creates the
+// transformation from its creator string and size and sets the parameters;
+// it throws std::runtime_error on a malformed file or a parameter count that
+// does not match the creator (as the reference's loaders do).
+void save_transformation(const std::string &path, const CSplineTransformation &t);
+PSplineTransformation load_transformation(const std::string &path, int dim);
+
 // Joint mean-0 / sigma-1 map of two images (nonrigidregister.cxx:147-184);
 // false if both are of one value (nothing to register).
 bool normalize_images(CImage &src, CImage &ref);

@@ -79,3 +79,58 @@ extern "C" int mia_host_nonrigid_register(int device, int dim, const uint32_t *s
         return detail::host_fail(MIA_HIP_ERR_DEVICE, e.what());
     }
 }
+
+extern "C" int mia_host_transform_save(const char *path, int dim, const uint32_t *size, const char *creator,
+                                       const double *params, size_t n)
+{
+    if (!path || (dim != 2 && dim != 3) || !size || !creator || (!params && n))
+        return detail::host_fail(MIA_HIP_ERR_ARG, "transform_save: bad arguments");
+    try {
+        auto t = produce_transform_creator(dim, creator)->create(Size3{size[0], size[1], dim == 3 ? size[2] : 1u});
+        t->set_parameters(params, n);
+        save_transformation(path, *t);
+        return MIA_HIP_OK;
+    } catch (const std::invalid_argument &e) {
+        return detail::host_fail(MIA_HIP_ERR_ARG, e.what());
+    } catch (const std::exception &e) {
+        return detail::host_fail(MIA_HIP_ERR_SIZE, e.what());
+    }
+}
+
+extern "C" int mia_host_transform_load(const char *path, int dim, uint32_t *size, uint32_t *coeff_size,
+                                       double *params, size_t params_cap, size_t *n_params, char *creator,
+                                       int creator_len)
+{
+    if (!path || (dim != 2 && dim != 3))
+        return detail::host_fail(MIA_HIP_ERR_ARG, "transform_load: bad arguments");
+    try {
+        auto t = load_transformation(path, dim);
+        const auto p = t->get_parameters();
+        if (n_params)
+            *n_params = p.size();
+        if (size) {
+            size[0] = t->get_size().x;
+            size[1] = t->get_size().y;
+            size[2] = t->get_size().z;
+        }
+        if (coeff_size) {
+            coeff_size[0] = t->get_coeff_size().x;
+            coeff_size[1] = t->get_coeff_size().y;
+            coeff_size[2] = t->get_coeff_size().z;
+        }
+        if (params) {
+            if (params_cap < p.size())
+                return detail::host_fail(MIA_HIP_ERR_SIZE, "transform_load: params buffer too small");
+            std::memcpy(params, p.data(), sizeof(double) * p.size());
+        }
+        if (creator && creator_len > 0) {
+            std::strncpy(creator, t->get_creator_string().c_str(), (size_t)creator_len - 1);
+            creator[creator_len - 1] = 0;
+        }
+        return MIA_HIP_OK;
+    } catch (const std::invalid_argument &e) {
+        return detail::host_fail(MIA_HIP_ERR_ARG, e.what());
+    } catch (const std::exception &e) {
+        return detail::host_fail(MIA_HIP_ERR_SIZE, e.what());
+    }
+}

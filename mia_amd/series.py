@@ -41,6 +41,11 @@ SIGNATURES = {
                                                   _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                                   ctypes.POINTER(ctypes.c_uint32), _vp, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_double)]),
+    "mia_host_transform_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.c_char_p, _vp, ctypes.c_size_t]),
+    "mia_host_transform_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(ctypes.c_uint32), _vp, ctypes.c_size_t,
+                                               ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_int]),
 }
 
 
@@ -182,3 +187,30 @@ def nonrigid_register(src, ref, transform="spline:rate=8", cost="image:cost=[ssd
     st = [{"size": tuple(s.size), "coeff_size": tuple(s.coeff_size), "evaluations": s.evaluations,
            "cost": s.cost, "seconds": s.seconds} for s in stats[:levels]]
     return params[:n.value].copy(), tuple(cs), st, fc.value
+
+
+def save_transform(path, creator, shape, params):
+    """Write a spline transformation file (MIA's xml layout).  shape: numpy
+    image shape; creator: e.g. "spline:rate=8"."""
+    L = load()
+    dim = len(shape)
+    sz = np.array(list(shape[::-1]) + [1] * (3 - dim), dtype=np.uint32)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    _check(L.mia_host_transform_save(str(path).encode(), dim, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                     creator.encode(), p.ctypes.data, p.size))
+
+
+def load_transform(path, dim):
+    """Read a spline transformation file; returns (creator, numpy shape,
+    coefficient grid (x,y,z), params)."""
+    L = load()
+    sz = (ctypes.c_uint32 * 3)()
+    cs = (ctypes.c_uint32 * 3)()
+    n = ctypes.c_size_t()
+    cr = ctypes.create_string_buffer(1024)
+    _check(L.mia_host_transform_load(str(path).encode(), int(dim), sz, cs, None, 0, ctypes.byref(n), None, 0))
+    params = np.zeros(n.value)
+    _check(L.mia_host_transform_load(str(path).encode(), int(dim), sz, cs, params.ctypes.data, params.size,
+                                     ctypes.byref(n), cr, 1024))
+    shape = tuple(int(v) for v in list(sz)[:dim][::-1])
+    return cr.value.decode(), shape, tuple(cs), params

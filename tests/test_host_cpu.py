@@ -1,6 +1,8 @@
 """CPU checks of the host layer: the minimizers against problems with known
 answers (compiled C++ test), and the host library's exported symbols."""
 import ctypes
+
+import pytest
 import os
 import re
 import subprocess
@@ -58,3 +60,36 @@ def test_nonrigid_host_cpp():
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().endswith("OK")
+
+
+def test_transform_file_round_trip(tmp_path):
+    """Transformation files in MIA's xml layout: parameters survive exactly
+    (17 significant digits), the grid is re-created from creator and size,
+    and malformed or inconsistent files raise."""
+    import numpy as np
+    from mia_amd import series
+    rng = np.random.default_rng(4)
+    for shape, creator in (((40, 50, 60), "spline:rate=8,penalty=[divcurl:weight=0.01]"),
+                           ((96, 80), "spline:rate=16,imgkernel=[bspline:d=1]")):
+        dim = len(shape)
+        import mia_amd
+        cs = [mia_amd.spline_csize(n, 8.0 if dim == 3 else 16.0, dim) for n in shape[::-1]]
+        p = rng.normal(size=dim * int(np.prod(cs))).astype(np.float32).astype(np.float64)
+        f = tmp_path / ("t%d.x%ddt" % (dim, dim))
+        series.save_transform(f, creator, shape, p)
+        text = f.read_text()
+        assert "<creator>" + creator + "</creator>" in text and "<params>" in text
+        cr, sh, grid, q = series.load_transform(f, dim)
+        assert cr == creator and sh == shape and list(grid[:dim]) == cs
+        assert np.array_equal(q, p)
+    with pytest.raises(ValueError):
+        series.save_transform(tmp_path / "bad.x3dt", "spline:rate=8", (40, 50, 60), np.zeros(5))
+    (tmp_path / "junk.x3dt").write_text("<boost_serialization></boost_serialization>")
+    with pytest.raises(RuntimeError):
+        series.load_transform(tmp_path / "junk.x3dt", 3)
+    with pytest.raises(RuntimeError):
+        series.load_transform(tmp_path / "t2.x2dt", 3)   # a 2D file has no size_z
+    t = re.sub(r"<params>\n\t<count>\d+", "<params>\n\t<count>7", (tmp_path / "t3.x3dt").read_text())
+    (tmp_path / "cut.x3dt").write_text(t)
+    with pytest.raises(RuntimeError):
+        series.load_transform(tmp_path / "cut.x3dt", 3)
