@@ -144,7 +144,26 @@ def cpu_baseline(mov, fix, cs, params, threads):
     return statistics.median(times), t1, cost, g.copy(), tm.copy()
 
 
-def series_2d(n_frames, minimizer):
+def series_cpu_baseline(frames, maxeval, threads):
+    """CPU baseline of the series: the FP32 port of MIA's 2D evaluation with
+    the same L-BFGS, frames in a pfor over `threads` workers
+    (src/2dmany2one-nonrigid.cc:185).  Returns (seconds, costs)."""
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    L.mcp_series_2d.argtypes = [ctypes.c_int, ip, fp, fp, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.c_void_p]
+    L.mcp_series_2d.restype = ctypes.c_double
+    n = len(frames)
+    sizes = np.array([v for m, _ in frames for v in m.shape[::-1]], dtype=np.int32)
+    movs = np.ascontiguousarray(np.concatenate([m.ravel() for m, _ in frames]), dtype=np.float32)
+    fixs = np.ascontiguousarray(np.concatenate([f.ravel() for _, f in frames]), dtype=np.float32)
+    costs = np.zeros(n)
+    sec = L.mcp_series_2d(n, sizes, movs, fixs, 8.0, maxeval, threads, costs.ctypes.data, None)
+    return sec, costs
+
+
+def series_2d(n_frames, minimizer, cpu_threads=0):
     """Batched 2D series: n_frames frames with sides uniform in [96, 224],
     spline rate 8, ssd, registered from the identity with L-BFGS.
 
@@ -170,6 +189,15 @@ def series_2d(n_frames, minimizer):
             res[name] = s.run(minimizer, mode)
             log("series %s: %.4f s" % (name, res[name][3]))
     t = {k: v[3] for k, v in res.items()}
+    cpu = None
+    if cpu_threads:
+        maxeval = int(re.search(r"maxiter=(\d+)", minimizer).group(1))
+        sec, ccost = series_cpu_baseline(frames, maxeval, cpu_threads)
+        log("series cpu_pfor (%d threads): %.3f s" % (cpu_threads, sec))
+        cpu = {"regs_per_sec": round(n_frames / sec, 2), "seconds": round(sec, 4), "threads": cpu_threads,
+               "kind": "port", "final_cost_mean": float(np.mean(ccost)),
+               "sample": "all %d frames, FP32 port of MIA's 2D evaluation with the same L-BFGS, pfor over frames"
+                         % n_frames}
 
     def same(a, b):
         return bool(np.array_equal(res[a][0], res[b][0]) and np.array_equal(res[a][1], res[b][1]))
@@ -186,7 +214,9 @@ def series_2d(n_frames, minimizer):
             "final_cost_mean": float(np.mean(res["device_batched"][1])),
             "final_cost_mean_host": float(np.mean(res["host_batched_graph"][1])),
             "evaluations_mean": float(np.mean(res["device_batched"][2])),
-            "seconds": {k: round(v, 5) for k, v in t.items()}}
+            "seconds": {k: round(v, 5) for k, v in t.items()},
+            "cpu_pfor": cpu,
+            "speedup_vs_cpu_pfor": round(cpu["seconds"] / t["device_batched"], 1) if cpu else None}
 
 
 def batch_volumes_3d(n_vol=16, seed=51):
@@ -452,7 +482,8 @@ def main():
 
     ser = None
     if not args.no_series:
-        ser = series_2d(args.series_frames, "lbfgs:maxiter=100")
+        ser = series_2d(args.series_frames, "lbfgs:maxiter=100",
+                        0 if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
     bat = None
     if not args.no_batched:
         bat = batched_3d(20)

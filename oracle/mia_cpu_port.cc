@@ -15,13 +15,26 @@
 //     mia/3d/transform/spline.cc:731-791, mia/3d/fullcost/image.cc:67-78)
 // pfor uses `nthreads` std::threads (the reference uses hardware_concurrency
 // tasks, mia/core/parallelcxx11.cc:25-31).
+//
+// The 2D evaluation follows the 2D classes: per-pixel 4x4 displacement sums
+// (mia/2d/transform/spline.cc:640-666), the interpolator prefiltered on every
+// warp (mia/2d/interpolator.cxx:66-132), central-difference gradient, force
+// with the float SSD sum that the 2D full cost returns
+// (mia/2d/fullcost/image.cc:81-105) and the 2D translate
+// (mia/2d/transform/spline.cc:509-576).  The series baseline registers
+// frames in a pfor over frames, one frame per task, as
+// mia-2dmany2one-nonrigid does (src/2dmany2one-nonrigid.cc:102-118,185),
+// with the same L-BFGS as the GPU host path (NLopt is not available).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
 #include <thread>
 #include <vector>
+
+#include "../mia_amd/host/minimizer.hh"
 
 namespace {
 
@@ -405,6 +418,181 @@ double mcp_cost_grad_3d(const float *mov, const float *fix, int nx, int ny, int 
         timings[6] = secs(t6, t7);
     }
     return 0.5 * sum;
+}
+
+
+// One 2D evaluation (single thread); returns the float-accumulated cost the
+// 2D full cost reports, writes grad (2 * cx * cy doubles).
+double mcp_cost_grad_2d(const float *mov, const float *fix, int nx, int ny, int cx, int cy, const double *params,
+                        double *grad)
+{
+    const size_t N = (size_t)nx * ny;
+    Axis ax = make_axis(nx, cx), ay = make_axis(ny, cy);
+    // displacement per pixel, 4x4 sums of the float coefficients
+    std::vector<float> px(N), py(N);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            float ux = 0.0f, uy = 0.0f;
+            for (size_t j = 0; j < ay.w[y].size(); ++j) {
+                const int ky = ay.start[y] + (int)j;
+                if (ky >= cy)
+                    break;
+                float hx = 0.0f, hy = 0.0f;
+                for (size_t i = 0; i < ax.w[x].size(); ++i) {
+                    const int kx = ax.start[x] + (int)i;
+                    if (kx >= cx)
+                        break;
+                    const size_t k = (size_t)ky * cx + kx;
+                    hx += (float)ax.w[x][i] * (float)params[2 * k];
+                    hy += (float)ax.w[x][i] * (float)params[2 * k + 1];
+                }
+                ux += (float)ay.w[y][j] * hx;
+                uy += (float)ay.w[y][j] * hy;
+            }
+            px[(size_t)y * nx + x] = (float)x - ux;
+            py[(size_t)y * nx + x] = (float)y - uy;
+        }
+    // interpolator: copy + prefilter along x and y
+    std::vector<float> cf(mov, mov + N);
+    float vmin = mov[0], vmax = mov[0];
+    for (size_t i = 0; i < N; ++i) {
+        vmin = std::min(vmin, mov[i]);
+        vmax = std::max(vmax, mov[i]);
+    }
+    if (0.0f < vmin)
+        vmin = 0.0f;
+    {
+        std::vector<double> buf(nx);
+        for (int y = 0; y < ny; ++y) {
+            for (int x = 0; x < nx; ++x)
+                buf[x] = cf[(size_t)y * nx + x];
+            filter_line(buf);
+            for (int x = 0; x < nx; ++x)
+                cf[(size_t)y * nx + x] = (float)buf[x];
+        }
+        buf.resize(ny);
+        for (int x = 0; x < nx; ++x) {
+            for (int y = 0; y < ny; ++y)
+                buf[y] = cf[(size_t)y * nx + x];
+            filter_line(buf);
+            for (int y = 0; y < ny; ++y)
+                cf[(size_t)y * nx + x] = (float)buf[y];
+        }
+    }
+    std::vector<float> W(N);
+    for (size_t idx = 0; idx < N; ++idx) {
+        const float pc[2] = {px[idx], py[idx]};
+        int s2[2];
+        double wd[2][4];
+        for (int a = 0; a < 2; ++a) {
+            const int f = fastfloor(pc[a]);
+            bspline3((double)pc[a] - f, wd[a]);
+            s2[a] = f - 1;
+        }
+        float r = 0.0f;
+        for (int b = 0; b < 4; ++b) {
+            const float *row = &cf[(size_t)mirror(s2[1] + b, ny) * nx];
+            float rx = 0.0f;
+            for (int a = 0; a < 4; ++a)
+                rx += (float)wd[0][a] * row[mirror(s2[0] + a, nx)];
+            r += (float)wd[1][b] * rx;
+        }
+        W[idx] = r >= vmin ? (r <= vmax ? r : vmax) : vmin;
+    }
+    // gradient, force, float cost
+    std::vector<float> fxv(N), fyv(N);
+    float fcost = 0.0f;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            const size_t i = (size_t)y * nx + x;
+            const float gx = (x > 0 && x < nx - 1) ? 0.5f * (W[i + 1] - W[i - 1]) : 0.0f;
+            const float gy = (y > 0 && y < ny - 1) ? 0.5f * (W[i + nx] - W[i - nx]) : 0.0f;
+            const float d = W[i] - fix[i];
+            fxv[i] = gx * d;
+            fyv[i] = gy * d;
+            fcost += d * d;
+        }
+    // translate: y rows, then x
+    std::vector<float> tx((size_t)cy * nx * 2, 0.0f);
+    for (int oy = 0; oy < cy; ++oy) {
+        int j = ay.row_first[oy];
+        for (double w : ay.row[oy]) {
+            for (int x = 0; x < nx; ++x) {
+                tx[2 * ((size_t)oy * nx + x)] += (float)w * fxv[(size_t)j * nx + x];
+                tx[2 * ((size_t)oy * nx + x) + 1] += (float)w * fyv[(size_t)j * nx + x];
+            }
+            ++j;
+        }
+    }
+    for (int oy = 0; oy < cy; ++oy)
+        for (int ox = 0; ox < cx; ++ox) {
+            float ax2 = 0.0f, ay2 = 0.0f;
+            int j = ax.row_first[ox];
+            for (double w : ax.row[ox]) {
+                ax2 += (float)w * tx[2 * ((size_t)oy * nx + j)];
+                ay2 += (float)w * tx[2 * ((size_t)oy * nx + j) + 1];
+                ++j;
+            }
+            grad[2 * ((size_t)oy * cx + ox)] = -ax2;
+            grad[2 * ((size_t)oy * cx + ox) + 1] = -ay2;
+        }
+    return 0.5 * (double)fcost;
+}
+
+// The series baseline: frames (sizes 2 per frame, images concatenated) each
+// normalised jointly to mean 0 / sigma 1 (nonrigidregister.cxx:147-184) and
+// registered from the identity with L-BFGS (maxeval evaluations, memory 10),
+// frames taken by nthreads workers one at a time.  Returns the wall time in
+// seconds; costs_out / evals_out per frame (may be NULL).
+double mcp_series_2d(int n, const int *sizes, const float *movs, const float *fixs, float rate, int maxeval,
+                     int nthreads, double *costs_out, int *evals_out)
+{
+    std::vector<size_t> off(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        off[i + 1] = off[i] + (size_t)sizes[2 * i] * sizes[2 * i + 1];
+    auto t0 = std::chrono::steady_clock::now();
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n)
+                return;
+            const int nx = sizes[2 * i], ny = sizes[2 * i + 1];
+            const size_t N = (size_t)nx * ny;
+            std::vector<float> a(movs + off[i], movs + off[i] + N), b(fixs + off[i], fixs + off[i] + N);
+            double sum = 0.0, sum2 = 0.0;
+            for (size_t k = 0; k < N; ++k) {
+                sum += (double)a[k] + b[k];
+                sum2 += (double)a[k] * a[k] + (double)b[k] * b[k];
+            }
+            const double m = 2.0 * N, mean = sum / m, sigma = std::sqrt((sum2 - sum * sum / m) / (m - 1));
+            if (sigma > 0.0)
+                for (size_t k = 0; k < N; ++k) {
+                    a[k] = (float)(a[k] / sigma - mean / sigma);
+                    b[k] = (float)(b[k] / sigma - mean / sigma);
+                }
+            const int cx = (int)(((float)nx + rate - 1.0f) / rate) + 2;
+            const int cy = (int)(((float)ny + rate - 1.0f) / rate) + 2;
+            const size_t np = 2 * (size_t)cx * cy;
+            mia_amd::LBFGSMinimizer opt(np, maxeval, 10, 0.0, 0.0, 1.0);
+            std::vector<double> x0(np, 0.0), g(np);
+            opt.start(x0.data());
+            while (!opt.done()) {
+                const double f = mcp_cost_grad_2d(a.data(), b.data(), nx, ny, cx, cy, opt.point(), g.data());
+                opt.tell(f, g.data());
+            }
+            if (costs_out)
+                costs_out[i] = opt.best_f();
+            if (evals_out)
+                evals_out[i] = opt.evaluations();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < std::max(1, nthreads); ++t)
+        th.emplace_back(work);
+    for (auto &t : th)
+        t.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 } // extern "C"
