@@ -1075,6 +1075,13 @@ __host__ __device__ inline Stage1Layout stage1_layout(int dim, int nx, int cx, i
     return L;
 }
 
+// load at a block-uniform row pointer plus a 32-bit byte offset: maps to a
+// scalar base and one vector offset register, no 64-bit address arithmetic
+__device__ __forceinline__ float ld_row(const float *__restrict__ row, unsigned byte_off)
+{
+    return *(const float *)((const char *)row + byte_off);
+}
+
 // rows of a band walked per iteration of the stage-1 loop: unrolled in
 // reference mode (independent loads of consecutive rows in flight), not in
 // exact mode (the interpolation body is large)
@@ -1203,25 +1210,34 @@ __global__ void __launch_bounds__(kBlock)
         const int xm = ix ? x - 1 : x, xp = ix ? x + 1 : x;
         const long long zoff = (long long)z * plane;
         const long long zm = iz ? zoff - plane : zoff, zp = iz ? zoff + plane : zoff;
+        // Row pointers are block-uniform and the column indices are
+        // non-negative 32-bit values, so every load below is a scalar base
+        // plus one constant per-lane offset (no per-load address arithmetic).
+        const unsigned ux = 4u * (unsigned)x, uxm = 4u * (unsigned)xm, uxp = 4u * (unsigned)xp; // byte offsets
+        const float *Wz = Wi + zoff, *Wzm = Wi + zm, *Wzp = Wi + zp, *Rz = Ri + zoff;
         // rows j-1, j, j+1 of column x (reference mode)
         float wm = 0.0f, wc = 0.0f, wp = 0.0f;
         if constexpr (MODE == 0) {
-            wc = Wi[zoff + (long long)j0 * nx + x];
-            wm = Wi[zoff + (long long)max(j0 - 1, 0) * nx + x];
+            wc = ld_row(Wz + (long long)j0 * nx, ux);
+            wm = ld_row(Wz + (long long)max(j0 - 1, 0) * nx, ux);
         }
 #pragma unroll kRowUnroll<MODE>
         for (int j = j0; j < j1; ++j) {
-            const long long idx = zoff + (long long)j * nx + x;
+            const long long ro = (long long)j * nx;
             const float4 wyv = tw[d.tab_y + j];
             const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
             float w, gx, gy, gz;
             if constexpr (MODE == 0) {
                 const bool iy = j > 0 && j < ny - 1;
-                wp = Wi[zoff + (long long)min(j + 1, ny - 1) * nx + x];
+                wp = ld_row(Wz + (long long)min(j + 1, ny - 1) * nx, ux);
                 w = wc;
-                gx = ix ? 0.5f * (Wi[zoff + (long long)j * nx + xp] - Wi[zoff + (long long)j * nx + xm]) : 0.0f;
+                // all four neighbours are loaded (clamped indices keep them in
+                // the image) and the border rule is a select
+                const float wxp = ld_row(Wz + ro, uxp), wxm = ld_row(Wz + ro, uxm);
+                const float wzp = ld_row(Wzp + ro, ux), wzm = ld_row(Wzm + ro, ux);
+                gx = ix ? 0.5f * (wxp - wxm) : 0.0f;
                 gy = iy ? 0.5f * (wp - wm) : 0.0f;
-                gz = iz ? 0.5f * (Wi[zp + (long long)j * nx + x] - Wi[zm + (long long)j * nx + x]) : 0.0f;
+                gz = iz ? 0.5f * (wzp - wzm) : 0.0f;
                 wm = wc;
                 wc = wp;
             } else {
@@ -1248,7 +1264,7 @@ __global__ void __launch_bounds__(kBlock)
                 gy = g[1];
                 gz = g[2];
             }
-            const float r = Ri[idx];
+            const float r = ld_row(Rz + ro, ux);
             const bool on = (thr == 0.0f) || (w > thr);
             const float dl = on ? w - r : 0.0f;
             ssd += (double)dl * (double)dl;
