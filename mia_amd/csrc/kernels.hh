@@ -428,6 +428,39 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+// Minimum / maximum over the wave, returned in every lane: DPP moves within
+// each row of 16 lanes (quad swaps, then the half-row and row mirrors), then
+// the four row results read into scalars.  No LDS traffic, unlike the
+// shuffles above.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float rows16(float v, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+__device__ __forceinline__ float wave_min_dpp(float v)
+{
+    v = fminf(v, dpp_f<0xB1>(v));  // quad_perm [1, 0, 3, 2]
+    v = fminf(v, dpp_f<0x4E>(v));  // quad_perm [2, 3, 0, 1]
+    v = fminf(v, dpp_f<0x141>(v)); // row_half_mirror
+    v = fminf(v, dpp_f<0x140>(v)); // row_mirror
+    return fminf(fminf(rows16(v, 0), rows16(v, 16)), fminf(rows16(v, 32), rows16(v, 48)));
+}
+
+__device__ __forceinline__ float wave_max_dpp(float v)
+{
+    v = fmaxf(v, dpp_f<0xB1>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    v = fmaxf(v, dpp_f<0x140>(v));
+    return fmaxf(fmaxf(rows16(v, 0), rows16(v, 16)), fmaxf(rows16(v, 32), rows16(v, 48)));
+}
+
 // Deterministic block sum (fixed shuffle tree per wave, then waves in order);
 // the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double *red)
@@ -767,12 +800,12 @@ __global__ void __launch_bounds__(kBlock)
     }
     // bounding box of the taps of all valid voxels of the tile; the barrier
     // also ends every read of the phase-2 intermediates, which the box reuses
-    mnx = wave_min(mnx);
-    mny = wave_min(mny);
-    mnz = wave_min(mnz);
-    mxx = wave_max(mxx);
-    mxy = wave_max(mxy);
-    mxz = wave_max(mxz);
+    mnx = wave_min_dpp(mnx);
+    mny = wave_min_dpp(mny);
+    mnz = wave_min_dpp(mnz);
+    mxx = wave_max_dpp(mxx);
+    mxy = wave_max_dpp(mxy);
+    mxz = wave_max_dpp(mxz);
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][wid] = mnx;
@@ -803,10 +836,11 @@ __global__ void __launch_bounds__(kBlock)
     const bool use_lds = RX <= TL::PX && RY <= TL::PY && RZ <= TL::PZ; // block-uniform
     const float *Mi = M + d.img_off;
     if (use_lds) {
-        // phase 3: one box row (RX <= 40 floats) per wave instruction, the
-        // waves taking rows round robin; kFill3 rows are loaded before any is
-        // stored so that that many loads are in flight per wave
-        constexpr int kFill3 = 8;
+        // phase 3: one box row (RX <= PX floats) per wave instruction, the
+        // waves taking rows round robin.  The rows go straight from global
+        // memory into LDS (global_load_lds: lane l's float lands at the row's
+        // base + 4 l bytes), so every row of a wave is in flight at once and
+        // no register holds the data
         const int lane = threadIdx.x & 63;
         const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const bool xact = lane < RX;
@@ -825,44 +859,25 @@ __global__ void __launch_bounds__(kBlock)
             iy -= RY;
             ++iz;
         }
-        float *dst = region + lane;
-        for (int r = wv; r < nrows; r += 4 * kFill3) {
-            float v[kFill3], v2[TL::PX > 64 ? kFill3 : 1];
-            int lo[kFill3];
-#pragma unroll
-            for (int u = 0; u < kFill3; ++u) {
-                v[u] = 0.0f;
-                if constexpr (TL::PX > 64)
-                    v2[u] = 0.0f;
-                lo[u] = -1;
-                if (r + 4 * u < nrows) {
-                    const int gy = yzin ? ry0 + iy : mirror_idx(ry0 + iy, d.ny);
-                    const int gz = yzin ? rz0 + iz : mirror_idx(rz0 + iz, d.nz);
-                    const float *src = Mi + ((long long)gz * d.ny + gy) * d.nx;
-                    if (xact)
-                        v[u] = src[gx];
-                    if constexpr (TL::PX > 64) {
-                        if (xact2)
-                            v2[u] = src[gx2];
-                    }
-                    lo[u] = (iz * TL::PY + iy) * TL::PX;
-                    iy += 4;
-                    while (iy >= RY) {
-                        iy -= RY;
-                        ++iz;
-                    }
-                }
+        for (int r = wv; r < nrows; r += 4) {
+            const int gy = yzin ? ry0 + iy : mirror_idx(ry0 + iy, d.ny);
+            const int gz = yzin ? rz0 + iz : mirror_idx(rz0 + iz, d.nz);
+            const float *src = Mi + ((long long)gz * d.ny + gy) * d.nx;
+            float *row = region + (iz * TL::PY + iy) * TL::PX;
+            if (xact)
+                __builtin_amdgcn_global_load_lds(src + gx, row, 4, 0, 0);
+            if constexpr (TL::PX > 64) {
+                if (xact2)
+                    __builtin_amdgcn_global_load_lds(src + gx2, row + 64, 4, 0, 0);
             }
-#pragma unroll
-            for (int u = 0; u < kFill3; ++u) {
-                if (lo[u] >= 0 && xact)
-                    dst[lo[u]] = v[u];
-                if constexpr (TL::PX > 64) {
-                    if (lo[u] >= 0 && xact2)
-                        dst[lo[u] + 64] = v2[u];
-                }
+            iy += 4;
+            while (iy >= RY) {
+                iy -= RY;
+                ++iz;
             }
         }
+        // the loads count on vmcnt only: wait for them before the barrier
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
     if (!valid)
