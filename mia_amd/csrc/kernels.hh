@@ -53,6 +53,9 @@ struct W3Tile {
     static constexpr int Buf = PX * PY * PZ;
     static constexpr int Buf4 = (Buf + 3) / 4;
 };
+// longest z knot window of a tile that the separable displacement keeps in
+// registers (TZ = 8 at spline rate >= 2)
+constexpr int kW3MaxKz = 8;
 // the variants the autotuner chooses from (index = variant id)
 using W3T0 = W3Tile<32, 8, 8, 40, 16, 15>;  // 38.4 KB LDS: 4 blocks per CU
 using W3T1 = W3Tile<32, 8, 4, 40, 16, 12>;  // 30.7 KB: 5 blocks per CU
@@ -428,10 +431,10 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// Minimum / maximum over the wave, returned in every lane: DPP moves within
-// each row of 16 lanes (quad swaps, then the half-row and row mirrors), then
-// the four row results read into scalars.  No LDS traffic, unlike the
-// shuffles above.
+// Minimum / maximum over the wave of values that are never NaN, returned in
+// every lane: DPP moves within each row of 16 lanes (quad swaps, then the
+// half-row and row mirrors), then the four row results read into scalars.  No
+// LDS traffic, unlike the shuffles above.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
 {
@@ -443,22 +446,77 @@ __device__ __forceinline__ float rows16(float v, int k)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
+// Minimum / maximum of values that are never NaN, as compare and select: two
+// instructions where fminf / fmaxf cost three (IEEE mode quiets both operands
+// before v_min / v_max).  min3 / max3: three operands in one instruction, no
+// quieting (callers rule out NaN inputs or route them elsewhere).
+__device__ __forceinline__ float min_nn(float a, float b)
+{
+    return b < a ? b : a;
+}
+
+__device__ __forceinline__ float max_nn(float a, float b)
+{
+    return b > a ? b : a;
+}
+
+__device__ __forceinline__ float min3_nn(float a, float b, float c)
+{
+    float r;
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+__device__ __forceinline__ float max3_nn(float a, float b, float c)
+{
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// minimum / maximum of v[0 .. N-1] (N >= 1) with three-operand instructions
+template <int N>
+__device__ __forceinline__ float min_n(const float *v)
+{
+    float m = v[0];
+    int k = 1;
+#pragma unroll
+    for (; k + 1 < N; k += 2)
+        m = min3_nn(m, v[k], v[k + 1]);
+    if (k < N)
+        m = min_nn(m, v[k]);
+    return m;
+}
+
+template <int N>
+__device__ __forceinline__ float max_n(const float *v)
+{
+    float m = v[0];
+    int k = 1;
+#pragma unroll
+    for (; k + 1 < N; k += 2)
+        m = max3_nn(m, v[k], v[k + 1]);
+    if (k < N)
+        m = max_nn(m, v[k]);
+    return m;
+}
+
 __device__ __forceinline__ float wave_min_dpp(float v)
 {
-    v = fminf(v, dpp_f<0xB1>(v));  // quad_perm [1, 0, 3, 2]
-    v = fminf(v, dpp_f<0x4E>(v));  // quad_perm [2, 3, 0, 1]
-    v = fminf(v, dpp_f<0x141>(v)); // row_half_mirror
-    v = fminf(v, dpp_f<0x140>(v)); // row_mirror
-    return fminf(fminf(rows16(v, 0), rows16(v, 16)), fminf(rows16(v, 32), rows16(v, 48)));
+    v = min_nn(v, dpp_f<0xB1>(v));  // quad_perm [1, 0, 3, 2]
+    v = min_nn(v, dpp_f<0x4E>(v));  // quad_perm [2, 3, 0, 1]
+    v = min_nn(v, dpp_f<0x141>(v)); // row_half_mirror
+    v = min_nn(v, dpp_f<0x140>(v)); // row_mirror
+    return min_nn(min_nn(rows16(v, 0), rows16(v, 16)), min_nn(rows16(v, 32), rows16(v, 48)));
 }
 
 __device__ __forceinline__ float wave_max_dpp(float v)
 {
-    v = fmaxf(v, dpp_f<0xB1>(v));
-    v = fmaxf(v, dpp_f<0x4E>(v));
-    v = fmaxf(v, dpp_f<0x141>(v));
-    v = fmaxf(v, dpp_f<0x140>(v));
-    return fmaxf(fmaxf(rows16(v, 0), rows16(v, 16)), fmaxf(rows16(v, 32), rows16(v, 48)));
+    v = max_nn(v, dpp_f<0xB1>(v));
+    v = max_nn(v, dpp_f<0x4E>(v));
+    v = max_nn(v, dpp_f<0x141>(v));
+    v = max_nn(v, dpp_f<0x140>(v));
+    return max_nn(max_nn(rows16(v, 0), rows16(v, 16)), max_nn(rows16(v, 32), rows16(v, 48)));
 }
 
 // Deterministic block sum (fixed shuffle tree per wave, then waves in order);
@@ -600,6 +658,135 @@ __global__ void __launch_bounds__(kBlock) k_prefilter_lines(float *__restrict__ 
 // warp
 // ---------------------------------------------------------------------------
 
+// Buffer resource for loads that the kernel keeps in bounds by construction
+// (mirrored image indices, clamped table entries, masked window columns).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, 0x00020000);
+}
+
+// One dword per active lane from base + voff + soff into LDS at lds + 4 * lane
+// (lds must be wave-uniform).
+__device__ __forceinline__ void dma_dword(__amdgpu_buffer_rsrc_t r, void *lds, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 4, voff, soff, 0, 0);
+}
+
+// e / d for 0 <= e < 4096 and 1 <= d <= 512, with inv = 1.0f / d: (e + 0.5) / d
+// is at least 0.5 / d away from an integer, more than its float error
+// (about 2^-23 (e + 0.5) / d <= 4.9e-4).
+__device__ __forceinline__ int small_div(int e, float inv)
+{
+    return (int)(((float)e + 0.5f) * inv);
+}
+
+// The NS^3 taps of one voxel from an LDS box with compile-time pitches
+// (b0: index of the first tap), z -> y -> x contraction as contract_lds3.
+template <int DEG, int PX, int PY>
+__device__ __forceinline__ float box_contract(const float *box, int b0, const float *wa, const float *wb,
+                                              const float *wc)
+{
+    constexpr int NS = DEG == 3 ? 4 : 2;
+    float v = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NS; ++c) {
+        // an opaque copy of the slice's index: the slice's taps are then
+        // immediate offsets from one register (left alone, the compiler forms
+        // a new address for most pairs of taps beyond the first slice)
+        int bc = b0 + c * PY * PX;
+        asm("" : "+v"(bc));
+        float vy = 0.0f;
+#pragma unroll
+        for (int bb = 0; bb < NS; ++bb) {
+            const float *row = box + bc + bb * PX;
+            float vx = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NS; ++a)
+                vx = fmaf(wa[a], row[a], vx);
+            vy = fmaf(wb[bb], vx, vy);
+        }
+        v = fmaf(wc[c], vy, v);
+    }
+    return v;
+}
+
+// Fill an RX x RY x RZ box (pitches PX x PY) with the image values at
+// (rx0 + i, ry0 + j, rz0 + k), mirrored at the image borders, straight from
+// global memory into LDS: one wave instruction per box row and 64 columns,
+// the block's waves taking rows round robin.  A box inside the image in y and
+// z walks its rows with scalar offsets updated per row (no branches, no vector
+// work); otherwise lane j of the wave first works out the mirrored source and
+// LDS offsets of the wave's j-th row and the loop reads them from the lanes.
+// The caller waits (vmcnt) and synchronises before reading the box.  Images
+// are below 2^31 bytes (32-bit byte offsets).
+template <int PX, int PY>
+__device__ __forceinline__ void fill_box_rows(float *box, const float *__restrict__ img, int nx, int ny, int nz,
+                                              int rx0, int ry0, int rz0, int RX, int RY, int RZ)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int nw = kBlock / 64;
+    const __amdgpu_buffer_rsrc_t rm = raw_rsrc(img);
+    const int nrows = RY * RZ;
+    const bool yzin = ry0 >= 0 && ry0 + RY <= ny && rz0 >= 0 && rz0 + RZ <= nz;
+#pragma unroll
+    for (int cb = 0; cb < PX; cb += 64) {
+        // columns cb .. cb + 63 of the box
+        const bool xact = lane + cb < RX;
+        int gx = rx0 + cb + lane;
+        if (rx0 < 0 || rx0 + RX > nx)
+            gx = mirror_idx(xact ? gx : 0, nx);
+        const unsigned vo = (unsigned)gx * 4u;
+        if (yzin) {
+            // row r = wv + nw k at (iy, iz) = (r % RY, r / RY); RY >= 2, so a
+            // step of nw = 4 rows wraps at most twice
+            int iy = wv, iz = 0;
+            while (iy >= RY) {
+                iy -= RY;
+                ++iz;
+            }
+            unsigned lo = (unsigned)((iz * PY + iy) * PX + cb) * 4u;
+            unsigned so = (unsigned)(((rz0 + iz) * ny + ry0 + iy) * nx) * 4u;
+            const unsigned lstep = nw * PX * 4, lwrap = (unsigned)((PY - RY) * PX * 4);
+            const unsigned sstep = (unsigned)(nw * nx * 4), swrap = (unsigned)((ny - RY) * nx * 4);
+            if (xact)
+                for (int r = wv; r < nrows; r += nw) {
+                    dma_dword(rm, (char *)box + lo, vo, so);
+                    iy += nw;
+                    lo += lstep;
+                    so += sstep;
+                    const bool w1 = iy >= RY;
+                    iy -= w1 ? RY : 0;
+                    lo += w1 ? lwrap : 0u;
+                    so += w1 ? swrap : 0u;
+                    const bool w2 = iy >= RY;
+                    iy -= w2 ? RY : 0;
+                    lo += w2 ? lwrap : 0u;
+                    so += w2 ? swrap : 0u;
+                }
+        } else {
+            // 64 rows at a time: lane j holds the offsets of row wv + nw (k0 + j)
+            for (int k0 = 0; wv + nw * k0 < nrows; k0 += 64) {
+                const int nk = min(64, (nrows - wv - nw * k0 + nw - 1) / nw);
+                const int r = min(wv + nw * (k0 + lane), nrows - 1);
+                const int iz = small_div(r, 1.0f / (float)RY), iy = r - iz * RY;
+                const int gy = mirror_idx(ry0 + iy, ny), gz = mirror_idx(rz0 + iz, nz);
+                int so = (gz * ny + gy) * nx * 4;
+                int lo = ((iz * PY + iy) * PX + cb) * 4;
+                // every lane computes its row's offsets before the branch below:
+                // the loop reads them from lanes that the branch switches off,
+                // and left alone the compiler moves this arithmetic into the
+                // branch
+                asm volatile("" : "+v"(so), "+v"(lo));
+                if (xact)
+                    for (int k = 0; k < nk; ++k)
+                        dma_dword(rm, (char *)box + __builtin_amdgcn_readlane(lo, k), vo,
+                                  (unsigned)__builtin_amdgcn_readlane(so, k));
+            }
+        }
+    }
+}
+
 // 3D warp.  Block = 32 x 8 x 8 voxels, thread = one (x, y) column of 8 voxels.
 //   phase 1: the tile's coefficient window (double params -> float, zero
 //            beyond the grid) and the tile's x/y/z spline tables into LDS;
@@ -665,17 +852,22 @@ __global__ void __launch_bounds__(kBlock)
         wz_s[k] = tw[d.tab_z + zz];
         sz_s[k] = ts[d.tab_z + zz] - kz0;
     }
-    const int nwin = wxn * wyn * wzn, nt1 = wzn * TL::TY * wxn, nt2 = wzn * TL::TY * TL::TX;
-    const bool separable = nwin + nt1 + nt2 <= TL::Buf4; // block-uniform
-    float4 *cwin = buf4, *T1 = buf4 + nwin, *T2 = T1 + nt1;
+    const int nwin = wxn * wyn * wzn, nt1 = wzn * TL::TY * wxn;
+    const bool separable = nwin + nt1 <= TL::Buf4 && wzn <= kW3MaxKz; // block-uniform
+    float4 *cwin = buf4, *T1 = buf4 + nwin;
+    // every coefficient this thread reads is finite (then so is the
+    // displacement, a convex combination of them)
+    bool fin = true;
     if (separable) {
+        const float iwx = 1.0f / (float)wxn, iwy = 1.0f / (float)wyn;
         for (int e = threadIdx.x; e < nwin; e += kBlock) {
-            const int kx = kx0 + e % wxn;
-            const int t = e / wxn;
-            const int ky = ky0 + t % wyn, kz = kz0 + t / wyn;
+            const int t = small_div(e, iwx), tz = small_div(t, iwy);
+            const int kx = kx0 + e - t * wxn;
+            const int ky = ky0 + t - tz * wyn, kz = kz0 + tz;
             float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (kx < d.cx && ky < d.cy && kz < d.cz) {
                 const float3 c = coef3(P, ((long long)kz * d.cy + ky) * d.cx + kx);
+                fin = fin && __builtin_isfinite(c.x) && __builtin_isfinite(c.y) && __builtin_isfinite(c.z);
                 q = make_float4(c.x, c.y, c.z, 0.0f);
             }
             cwin[e] = q;
@@ -688,9 +880,10 @@ __global__ void __launch_bounds__(kBlock)
         ux[zl] = uy[zl] = uz[zl] = 0.0f;
     if (separable) {
         // phase 2a: T1[kz][yl][kx] = sum_b wy_b(yl) * C[kz][sy(yl) + b][kx]
+        const float iwx = 1.0f / (float)wxn;
         for (int e = threadIdx.x; e < nt1; e += kBlock) {
-            const int kx = e % wxn;
-            const int t = e / wxn;
+            const int t = small_div(e, iwx);
+            const int kx = e - t * wxn;
             const int r = t % TL::TY, kz = t / TL::TY;
             const float4 w = wy_s[r];
             const float ws[4] = {w.x, w.y, w.z, w.w};
@@ -706,37 +899,49 @@ __global__ void __launch_bounds__(kBlock)
             T1[e] = make_float4(ax, ay, az, 0.0f);
         }
         __syncthreads();
-        // phase 2b: T2[kz][yl][xl] = sum_a wx_a(xl) * T1[kz][yl][sx(xl) + a]
-        for (int e = threadIdx.x; e < nt2; e += kBlock) {
-            const int c = e & (TL::TX - 1);
-            const int t = e >> TL::LOG_TX; // kz * TY + yl
-            const float4 w = wx_s[c];
+        // phase 2b: this thread's own column of the x contraction, in
+        // registers: t[kz] = sum_a wx_a(xl) * T1[kz][yl][sx(xl) + a], kz < wzn
+        float tx[kW3MaxKz], ty[kW3MaxKz], tz[kW3MaxKz];
+        {
+            const float4 w = wx_s[xl];
             const float ws[4] = {w.x, w.y, w.z, w.w};
-            const float4 *src = T1 + t * wxn + sx_s[c];
-            float ax = 0.0f, ay = 0.0f, az = 0.0f;
+            const float4 *src = T1 + yl * wxn + sx_s[xl];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float4 q = src[a];
-                ax = fmaf(ws[a], q.x, ax);
-                ay = fmaf(ws[a], q.y, ay);
-                az = fmaf(ws[a], q.z, az);
+            for (int kz = 0; kz < kW3MaxKz; ++kz) {
+                float ax = 0.0f, ay = 0.0f, az = 0.0f;
+                if (kz < wzn) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const float4 q = src[kz * TL::TY * wxn + a];
+                        ax = fmaf(ws[a], q.x, ax);
+                        ay = fmaf(ws[a], q.y, ay);
+                        az = fmaf(ws[a], q.z, az);
+                    }
+                }
+                tx[kz] = ax;
+                ty[kz] = ay;
+                tz[kz] = az;
             }
-            T2[e] = make_float4(ax, ay, az, 0.0f);
         }
-        __syncthreads();
-        // phase 2c: z contraction per voxel
+        // phase 2c: z contraction per voxel from those registers, also for
+        // the slices beyond a partial tile (their table entries repeat the
+        // last slice's; the results are not stored).  The first knot of a
+        // slice is the same for the whole block, so picking the registers is
+        // a scalar branch.
 #pragma unroll
         for (int zl = 0; zl < TL::TZ; ++zl) {
-            if (zl < nzl) {
-                const float4 w = wz_s[zl];
-                const float ws[4] = {w.x, w.y, w.z, w.w};
-                const float4 *src = T2 + (sz_s[zl] * TL::TY + yl) * TL::TX + xl;
+            const float4 w = wz_s[zl];
+            const float ws[4] = {w.x, w.y, w.z, w.w};
+            const int o = __builtin_amdgcn_readfirstlane(sz_s[zl]);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const float4 q = src[c * TL::TY * TL::TX];
-                    ux[zl] = fmaf(ws[c], q.x, ux[zl]);
-                    uy[zl] = fmaf(ws[c], q.y, uy[zl]);
-                    uz[zl] = fmaf(ws[c], q.z, uz[zl]);
+            for (int oo = 0; oo + 4 <= kW3MaxKz; ++oo) {
+                if (o == oo) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        ux[zl] = fmaf(ws[c], tx[oo + c], ux[zl]);
+                        uy[zl] = fmaf(ws[c], ty[oo + c], uy[zl]);
+                        uz[zl] = fmaf(ws[c], tz[oo + c], uz[zl]);
+                    }
                 }
             }
         }
@@ -760,6 +965,8 @@ __global__ void __launch_bounds__(kBlock)
                         const int kx = kx0 + sx + a;
                         if (kx < d.cx && ky < d.cy) {
                             const float3 q = coef3(P, ((long long)kz * d.cy + ky) * d.cx + kx);
+                            fin = fin && __builtin_isfinite(q.x) && __builtin_isfinite(q.y) &&
+                                  __builtin_isfinite(q.z);
                             hx = fmaf(wxs[a], q.x, hx);
                             hy = fmaf(wxs[a], q.y, hy);
                             hz = fmaf(wxs[a], q.z, hz);
@@ -772,8 +979,8 @@ __global__ void __launch_bounds__(kBlock)
             }
 #pragma unroll
             for (int zl = 0; zl < TL::TZ; ++zl) {
-                const int c = kzl - (zl < nzl ? sz_s[zl] : 0);
-                if (zl < nzl && c >= 0 && c < 4) {
+                const int c = kzl - sz_s[zl];
+                if (c >= 0 && c < 4) {
                     const float w = ((const float *)&wz_s[zl])[c];
                     ux[zl] = fmaf(w, vx, ux[zl]);
                     uy[zl] = fmaf(w, vy, uy[zl]);
@@ -782,22 +989,31 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     }
-    float px[TL::TZ], py[TL::TZ], pz[TL::TZ];
-    float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
+    // Sample positions x - u(x), kept clamped to +-1e6 for the interpolation.
+    // The tap box comes from the extremes of the unclamped positions (results
+    // of arithmetic, so v_min / v_max need not quiet them first; clamping the
+    // extremes afterwards gives the extremes of the clamped positions).  A
+    // non-finite coefficient makes the box test fail below, and the tile then
+    // takes the global-memory path, which is safe for any position.
+    // Lanes beyond the image in x or y and slices beyond it in z take the
+    // coordinates of the last column, row or slice (their table entries are
+    // those of the last ones already), so they repeat a real voxel's position
+    // and leave the box as it is; they are not stored.
+    float px[TL::TZ], py[TL::TZ], pz[TL::TZ], qx[TL::TZ], qy[TL::TZ], qz[TL::TZ];
+    const float xc = (float)min(x, d.nx - 1), yc = (float)min(y, d.ny - 1);
 #pragma unroll
     for (int zl = 0; zl < TL::TZ; ++zl) {
-        px[zl] = clamp_pos((float)x - ux[zl]);
-        py[zl] = clamp_pos((float)y - uy[zl]);
-        pz[zl] = clamp_pos((float)(z0 + zl) - uz[zl]);
-        if (valid && zl < nzl) {
-            mnx = fminf(mnx, px[zl]);
-            mxx = fmaxf(mxx, px[zl]);
-            mny = fminf(mny, py[zl]);
-            mxy = fmaxf(mxy, py[zl]);
-            mnz = fminf(mnz, pz[zl]);
-            mxz = fmaxf(mxz, pz[zl]);
-        }
+        qx[zl] = xc - ux[zl];
+        qy[zl] = yc - uy[zl];
+        qz[zl] = (float)min(z0 + zl, d.nz - 1) - uz[zl];
+        px[zl] = clamp_pos(qx[zl]);
+        py[zl] = clamp_pos(qy[zl]);
+        pz[zl] = clamp_pos(qz[zl]);
     }
+    float mnx = min_n<TL::TZ>(qx), mny = min_n<TL::TZ>(qy), mnz = min_n<TL::TZ>(qz);
+    float mxx = max_n<TL::TZ>(qx), mxy = max_n<TL::TZ>(qy), mxz = max_n<TL::TZ>(qz);
+    if (!fin)
+        mnx = -FLT_MAX;
     // bounding box of the taps of all valid voxels of the tile; the barrier
     // also ends every read of the phase-2 intermediates, which the box reuses
     mnx = wave_min_dpp(mnx);
@@ -818,64 +1034,26 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kBlock / 64; ++k) {
-        mnx = fminf(mnx, red[0][k]);
-        mny = fminf(mny, red[1][k]);
-        mnz = fminf(mnz, red[2][k]);
-        mxx = fmaxf(mxx, red[3][k]);
-        mxy = fmaxf(mxy, red[4][k]);
-        mxz = fmaxf(mxz, red[5][k]);
+        mnx = min_nn(mnx, red[0][k]);
+        mny = min_nn(mny, red[1][k]);
+        mnz = min_nn(mnz, red[2][k]);
+        mxx = max_nn(mxx, red[3][k]);
+        mxy = max_nn(mxy, red[4][k]);
+        mxz = max_nn(mxz, red[5][k]);
     }
     // every lane holds the same box; readfirstlane makes that visible to the
     // compiler so that the fill below runs its row arithmetic on the scalar unit
-    const int rx0 = __builtin_amdgcn_readfirstlane(fastfloor16(mnx) - LO);
-    const int ry0 = __builtin_amdgcn_readfirstlane(fastfloor16(mny) - LO);
-    const int rz0 = __builtin_amdgcn_readfirstlane(fastfloor16(mnz) - LO);
-    const int RX = __builtin_amdgcn_readfirstlane(fastfloor16(mxx) - LO + NS) - rx0;
-    const int RY = __builtin_amdgcn_readfirstlane(fastfloor16(mxy) - LO + NS) - ry0;
-    const int RZ = __builtin_amdgcn_readfirstlane(fastfloor16(mxz) - LO + NS) - rz0;
+    const int rx0 = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mnx)) - LO);
+    const int ry0 = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mny)) - LO);
+    const int rz0 = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mnz)) - LO);
+    const int RX = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mxx)) - LO + NS) - rx0;
+    const int RY = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mxy)) - LO + NS) - ry0;
+    const int RZ = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mxz)) - LO + NS) - rz0;
     const bool use_lds = RX <= TL::PX && RY <= TL::PY && RZ <= TL::PZ; // block-uniform
     const float *Mi = M + d.img_off;
     if (use_lds) {
-        // phase 3: one box row (RX <= PX floats) per wave instruction, the
-        // waves taking rows round robin.  The rows go straight from global
-        // memory into LDS (global_load_lds: lane l's float lands at the row's
-        // base + 4 l bytes), so every row of a wave is in flight at once and
-        // no register holds the data
-        const int lane = threadIdx.x & 63;
-        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        const bool xact = lane < RX;
-        int gx = rx0 + lane;
-        if (rx0 < 0 || rx0 + RX > d.nx)
-            gx = mirror_idx(xact ? gx : 0, d.nx);
-        // box columns 64.. (pitches wider than a wave)
-        const bool xact2 = TL::PX > 64 && lane + 64 < RX;
-        int gx2 = rx0 + lane + 64;
-        if (TL::PX > 64 && (rx0 < 0 || rx0 + RX > d.nx))
-            gx2 = mirror_idx(xact2 ? gx2 : 0, d.nx);
-        const bool yzin = ry0 >= 0 && ry0 + RY <= d.ny && rz0 >= 0 && rz0 + RZ <= d.nz;
-        const int nrows = RY * RZ;
-        int iy = wv, iz = 0;
-        while (iy >= RY) {
-            iy -= RY;
-            ++iz;
-        }
-        for (int r = wv; r < nrows; r += 4) {
-            const int gy = yzin ? ry0 + iy : mirror_idx(ry0 + iy, d.ny);
-            const int gz = yzin ? rz0 + iz : mirror_idx(rz0 + iz, d.nz);
-            const float *src = Mi + ((long long)gz * d.ny + gy) * d.nx;
-            float *row = region + (iz * TL::PY + iy) * TL::PX;
-            if (xact)
-                __builtin_amdgcn_global_load_lds(src + gx, row, 4, 0, 0);
-            if constexpr (TL::PX > 64) {
-                if (xact2)
-                    __builtin_amdgcn_global_load_lds(src + gx2, row + 64, 4, 0, 0);
-            }
-            iy += 4;
-            while (iy >= RY) {
-                iy -= RY;
-                ++iz;
-            }
-        }
+        // phase 3: the box rows straight into LDS
+        fill_box_rows<TL::PX, TL::PY>(region, Mi, d.nx, d.ny, d.nz, rx0, ry0, rz0, RX, RY, RZ);
         // the loads count on vmcnt only: wait for them before the barrier
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -895,27 +1073,7 @@ __global__ void __launch_bounds__(kBlock)
                 const int s1 = kweights<DEG, false>(py[zl], wb, nullptr);
                 const int s2 = kweights<DEG, false>(pz[zl], wc, nullptr);
                 const int b0 = ((s2 - rz0) * TL::PY + (s1 - ry0)) * TL::PX + (s0 - rx0);
-                float v = 0.0f;
-#pragma unroll
-                for (int c = 0; c < NS; ++c) {
-                    // an opaque copy of the slice's index: the 16 taps of the
-                    // slice are then immediate offsets from one register
-                    // (left alone, the compiler forms a new address for most
-                    // pairs of taps beyond the first slice)
-                    int bc = b0 + c * TL::PY * TL::PX;
-                    asm("" : "+v"(bc));
-                    float vy = 0.0f;
-#pragma unroll
-                    for (int bb = 0; bb < NS; ++bb) {
-                        const float *row = region + bc + bb * TL::PX;
-                        float vx = 0.0f;
-#pragma unroll
-                        for (int a = 0; a < NS; ++a)
-                            vx = fmaf(wa[a], row[a], vx);
-                        vy = fmaf(wb[bb], vx, vy);
-                    }
-                    v = fmaf(wc[c], vy, v);
-                }
+                const float v = box_contract<DEG, TL::PX, TL::PY>(region, b0, wa, wb, wc);
                 Wi[zl * plane] = clamp_val(v, cl.x, cl.y);
             }
         }

@@ -666,9 +666,9 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
             gidx[v] = -1;
         for (int i = 0; i < n_inst; ++i) {
             const InstDesc &d = c->inst[i];
+            const long long nvox = (long long)d.nx * d.ny * d.nz;
             const int v = dim == 3 ? (c->force_variant >= 0 ? c->force_variant
-                                                           : tuned_variant(c->device, degree,
-                                                                           (long long)d.nx * d.ny * d.nz))
+                                                           : tuned_variant(c->device, degree, nvox))
                                    : 0;
             c->inst_variant[i] = v;
             if (gidx[v] < 0) {
