@@ -86,6 +86,16 @@ struct InstDesc {
     long long part_off;       // offset of its stage-1 partials (floats)
     int tab_x, tab_y, tab_z;  // offsets into the axis tables (one entry per voxel)
     int cell_x, cell_y, cell_z; // offsets into the band tables (csize + 1 entries)
+    int zcell;                // 3D reference gradient: stage 1 per (z knot cell, y band)
+};
+
+// Where the stage-1 SSD partials of an instance are: per (slice, y band) block
+// at s1[i] .. s1[i + 1], or, for an instance with zcell set while zmode is on
+// (3D reference gradient), per (z knot cell, y band) block at zoff + s1z[i] ..
+// zoff + s1z[i + 1].
+struct SsdIndex {
+    const int *s1, *s1z;
+    int zoff, zmode;
 };
 
 // ---------------------------------------------------------------------------
@@ -1319,6 +1329,8 @@ __global__ void __launch_bounds__(kBlock)
     const int b = xcd_remap(blockIdx.x, nblocks);
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
+    if (MODE == 0 && d.zcell) // k_stage1z_3d's instance
+        return;
     const int lb = b - prefix[i];
     const int band = lb % d.ncy;
     const int z = lb / d.ncy;
@@ -1465,6 +1477,200 @@ __global__ void __launch_bounds__(kBlock)
                   Pout_all + d.part_off + ((long long)z * d.ncy + band) * 12 * cx);
 }
 
+// register window of k_stage1z_3d (band rows + 2): y bands of up to 9 rows,
+// i.e. spline rates up to about 9
+constexpr int kS1zRows = 11;
+
+// One float per lane from a buffer at a per-lane byte offset plus a scalar
+// byte offset (the builtin returns the raw 32 bits)
+__device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// x neighbours within a wave: lane l receives lane l + 1 (wave_shl:1) or
+// lane l - 1 (wave_shr:1); the lane at the wave's edge receives `edge`
+__device__ __forceinline__ float lane_next(float v, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_prev(float v, float edge)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// Stage 1, 3D, reference gradient, one block per (z knot cell, y knot band):
+// the slices z of the cell (those whose first z knot is the cell, so that
+// their four z knots are cell .. cell + 3) and the rows of the band; one
+// thread per x column (nx <= kBlock).  Each thread keeps its column of W for
+// the band's rows and the rows either side in registers for three slices
+// (z - 1, z, z + 1) and walks z, loading one new slice of W and of R per
+// slice, a slice ahead of its use (buffer loads at scalar row offsets).  The
+// x neighbours come from the neighbouring lanes (DPP); the columns at a
+// wave's edges are exchanged through LDS.  The gradient rule, residual and
+// force are those of k_stage1_3d.  f * wy_t(j) accumulates over the band's
+// rows; at the end of the slice the 12 sums (t, component) are folded with
+// the slice's four z weights into 48 per column (k = kz - cell, t,
+// component), kept in registers across the cell.  The x contraction then runs
+// through LDS once per k with contract_x's two fixed-order steps.  Output of
+// the block: P[cell][band][k][t][kx][comp] (48 * cx floats), so the reduce
+// reads every partial sum once.  MAXR >= band rows + 2.
+template <int DEG, int MAXR>
+__global__ void __launch_bounds__(kBlock, 4)
+    k_stage1z_3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int nblocks,
+                 const float4 *__restrict__ tw, const int *__restrict__ ts, const int *__restrict__ cells,
+                 const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ Pout_all,
+                 double *__restrict__ ssd_part, unsigned *__restrict__ cnt_part)
+{
+    constexpr int NW = kBlock / 64;
+    extern __shared__ float smem[];
+    __shared__ double red[NW];
+    __shared__ unsigned redu[NW];
+    __shared__ float edge_s[2][NW][2][MAXR]; // [slice parity][wave][lane 0 / lane 63][window row]
+    const int b = xcd_remap(blockIdx.x, nblocks);
+    const int i = find_inst(prefix, n_inst, b);
+    const InstDesc d = inst[i];
+    if (!d.zcell) // k_stage1_3d's instance
+        return;
+    const int lb = b - prefix[i];
+    const int band = lb % d.ncy;
+    const int cz = lb / d.ncy;
+    const int j0 = cells[d.cell_y + band], j1 = cells[d.cell_y + band + 1];
+    const int zb = cells[d.cell_z + cz], ze = cells[d.cell_z + cz + 1];
+    const int nx = d.nx, ny = d.ny, nz = d.nz, cx = d.cx;
+    const Stage1Layout L = stage1_layout(3, nx, cx, d.ncx, false);
+    const int AS = L.acc_stride;
+    float *acc_s = smem + L.acc;
+    float4 *xw_s = (float4 *)(smem + L.xw);
+    int *xs_s = (int *)(smem + L.xs);
+    int *cl_s = (int *)(smem + L.cl);
+    float *S_s = smem + L.S;
+    for (int x = threadIdx.x; x < nx; x += kBlock) {
+        xw_s[x] = tw[d.tab_x + x];
+        xs_s[x] = ts[d.tab_x + x];
+    }
+    for (int k = threadIdx.x; k <= cx; k += kBlock)
+        cl_s[k] = cells[d.cell_x + k];
+    const int x = threadIdx.x;
+    const int lane = x & 63, wv = x >> 6;
+    const bool ix = x > 0 && x < nx - 1;
+    const int nrw = j1 - j0 + 2; // window rows j0 - 1 .. j1 (clamped to the image)
+    const __amdgpu_buffer_rsrc_t rwi = raw_rsrc(W + d.img_off), rri = raw_rsrc(R + d.img_off);
+    const unsigned vo = (unsigned)min(x, nx - 1) * 4u;
+    // byte offset of window row r of slice z (slices and rows clamped)
+    auto soff = [&](int z, int r) {
+        const int zc = min(max(z, 0), nz - 1), jr = min(max(j0 - 1 + r, 0), ny - 1);
+        return (unsigned)((zc * ny + jr) * nx) * 4u;
+    };
+    float wm[MAXR], wc[MAXR], wp[MAXR], wn[MAXR], rc[MAXR], rn[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+        wm[r] = wc[r] = wp[r] = wn[r] = rc[r] = rn[r] = 0.0f;
+        if (r < nrw) {
+            wm[r] = ld_buf(rwi, vo, soff(zb - 1, r));
+            wc[r] = ld_buf(rwi, vo, soff(zb, r));
+            wp[r] = ld_buf(rwi, vo, soff(zb + 1, r));
+            if (r >= 1 && r < nrw - 1)
+                rc[r] = ld_buf(rri, vo, soff(zb, r));
+        }
+    }
+    double ssd = 0.0;
+    unsigned cnt = 0;
+    float a48[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k)
+        a48[k] = 0.0f;
+    for (int z = zb; z < ze; ++z) {
+        // next slice's loads: W of z + 2, R of z + 1
+        if (z + 1 < ze) {
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                if (r < nrw) {
+                    wn[r] = ld_buf(rwi, vo, soff(z + 2, r));
+                    if (r >= 1 && r < nrw - 1)
+                        rn[r] = ld_buf(rri, vo, soff(z + 1, r));
+                }
+            }
+        }
+        // the wave's edge columns of slice z for the neighbouring waves (two
+        // buffers by slice parity: a buffer is rewritten only after the
+        // barrier of the following slice, when every read of it is done)
+        const int par = (z - zb) & 1;
+        if (lane == 0 || lane == 63) {
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r)
+                edge_s[par][wv][lane == 63][r] = wc[r];
+        }
+        __syncthreads();
+        const float4 wzv = tw[d.tab_z + z];
+        const float wzs[4] = {wzv.x, wzv.y, wzv.z, wzv.w};
+        const bool iz = z > 0 && z < nz - 1;
+        float acc[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            acc[k] = 0.0f;
+#pragma unroll
+        for (int r = 1; r + 1 < MAXR; ++r) {
+            if (r + 1 < nrw) {
+                const int j = j0 + r - 1;
+                const float w = wc[r];
+                const float em = (lane == 0 && wv > 0) ? edge_s[par][wv - 1][1][r] : w;
+                const float ep = (lane == 63 && wv + 1 < NW) ? edge_s[par][wv + 1][0][r] : w;
+                const float wxm = lane_prev(w, em), wxp = lane_next(w, ep);
+                const float4 wyv = tw[d.tab_y + j];
+                const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+                const bool iy = j > 0 && j < ny - 1;
+                const float gx = ix ? 0.5f * (wxp - wxm) : 0.0f;
+                const float gy = iy ? 0.5f * (wc[r + 1] - wc[r - 1]) : 0.0f;
+                const float gz = iz ? 0.5f * (wp[r] - wm[r]) : 0.0f;
+                const bool on = (thr == 0.0f) || (w > thr);
+                const float dl = on ? w - rc[r] : 0.0f;
+                if (x < nx) {
+                    ssd += (double)dl * (double)dl;
+                    cnt += on ? 1u : 0u;
+                }
+                const float fx = gx * dl, fy = gy * dl, fz = gz * dl;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[3 * t + 0] = fmaf(wys[t], fx, acc[3 * t + 0]);
+                    acc[3 * t + 1] = fmaf(wys[t], fy, acc[3 * t + 1]);
+                    acc[3 * t + 2] = fmaf(wys[t], fz, acc[3 * t + 2]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int m = 0; m < 12; ++m)
+                a48[12 * k + m] = fmaf(wzs[k], acc[m], a48[12 * k + m]);
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            wm[r] = wc[r];
+            wc[r] = wp[r];
+            wp[r] = wn[r];
+            rc[r] = rn[r];
+        }
+    }
+    const double s = block_sum(ssd, red);
+    const unsigned c = block_sum_u(cnt, redu);
+    if (threadIdx.x == 0) {
+        ssd_part[b] = s;
+        cnt_part[b] = c;
+    }
+    float *Pb = Pout_all + d.part_off + ((long long)cz * d.ncy + band) * 48 * cx;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        __syncthreads(); // the previous pass is done with acc_s and S_s
+        if (x < nx)
+#pragma unroll
+            for (int m = 0; m < 12; ++m)
+                acc_s[AS * x + m] = a48[12 * k + m];
+        __syncthreads();
+        contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
+    }
+}
+
 // Stage 1, 2D: block = y knot band.
 template <int DEG, int MODE>
 __global__ void __launch_bounds__(kBlock)
@@ -1595,7 +1801,7 @@ __global__ void __launch_bounds__(kBlock)
 // Per instance: SSD sum and mask count in a fixed order, cost and gradient
 // scale (mia/template/ssd.hh:54-99,125-181; weight:
 // mia/template/fullcost.cxx:36-45).  Run by thread block `i` of either kernel.
-__device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, const int *__restrict__ prefix,
+__device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdIndex si,
                                                   const double *__restrict__ ssd_part,
                                                   const unsigned *__restrict__ cnt_part, int norm, float thr,
                                                   double weight, double *__restrict__ costs,
@@ -1603,7 +1809,8 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, cons
 {
     __shared__ double red[kBlock / 64];
     __shared__ unsigned redu[kBlock / 64];
-    const int beg = prefix[i], end = prefix[i + 1];
+    const bool z = si.zmode && d.zcell;
+    const int beg = z ? si.zoff + si.s1z[i] : si.s1[i], end = z ? si.zoff + si.s1z[i + 1] : si.s1[i + 1];
     double s = 0.0;
     unsigned c = 0;
     for (int k = beg + threadIdx.x; k < end; k += kBlock) {
@@ -1632,38 +1839,38 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, cons
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_finalize(const InstDesc *__restrict__ inst, const int *__restrict__ prefix, const double *__restrict__ ssd_part,
+    k_finalize(const InstDesc *__restrict__ inst, SsdIndex si, const double *__restrict__ ssd_part,
                const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
                double *__restrict__ gscale)
 {
     const int i = blockIdx.x;
-    finalize_instance(inst[i], i, prefix, ssd_part, cnt_part, norm, thr, weight, costs, gscale);
+    finalize_instance(inst[i], i, si, ssd_part, cnt_part, norm, thr, weight, costs, gscale);
 }
 
 // Per coefficient component: gradient = scale * sum_z wz(z) sum_t
 // P[z][ky - t][t][kx][comp] in FP64, fixed order (the adjoint of
-// spline.cc:731-791 completed along y and z).  FUSED: the first n_inst blocks
-// do the finalize work and the scale is -weight * (norm ? 1/N : 1), known
-// without the mask count (no autothresh); otherwise the scale comes from
-// k_finalize.
+// spline.cc:731-791 completed along y and z).  FUSED: the scale is
+// -weight * (norm ? 1/N : 1), known without the mask count (no autothresh);
+// otherwise it comes from k_finalize.  The first nfin blocks (0 or n_inst) do
+// the finalize work.  Instances that k_reducez_3d handles are skipped.
 template <int DIM, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
     k_reduce(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
              const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
-             const int *__restrict__ cells, const int *__restrict__ s1_prefix, const double *__restrict__ ssd_part,
+             const int *__restrict__ cells, SsdIndex si, int nfin, const double *__restrict__ ssd_part,
              const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
              const double *__restrict__ gscale, double *__restrict__ grad)
 {
     int b = blockIdx.x;
-    if constexpr (FUSED) {
-        if (b < n_inst) {
-            finalize_instance(inst[b], b, s1_prefix, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
-            return;
-        }
-        b -= n_inst;
+    if (b < nfin) {
+        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
+        return;
     }
+    b -= nfin;
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
+    if (si.zmode && d.zcell)
+        return;
     const long long ncoef = (long long)d.cx * d.cy * d.cz;
     double sc;
     if constexpr (FUSED)
@@ -1722,6 +1929,61 @@ __global__ void __launch_bounds__(kBlock)
         }
         grad[d.par_off + e3] = sc * g;
     }
+}
+
+// Reduce of k_stage1z_3d's partial sums, one gradient component per thread:
+// gradient(kz, ky, kx, comp) = scale * sum_{k < 4} sum_{t < 4}
+// P[kz - k][ky - t][k][t][kx][comp], in FP64 and this fixed order (the
+// adjoint of spline.cc:731-791 completed along y and z).  Every partial sum
+// is read once.  FUSED and nfin as in k_reduce; instances without zcell are
+// skipped.
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+    k_reducez_3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
+                 const float *__restrict__ P, SsdIndex si, int nfin, const double *__restrict__ ssd_part,
+                 const unsigned *__restrict__ cnt_part, int norm, float thr, double weight,
+                 double *__restrict__ costs, const double *__restrict__ gscale, double *__restrict__ grad)
+{
+    int b = blockIdx.x;
+    if (b < nfin) {
+        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
+        return;
+    }
+    b -= nfin;
+    const int i = find_inst(prefix, n_inst, b);
+    const InstDesc d = inst[i];
+    if (!d.zcell)
+        return;
+    const long long ncoef = (long long)d.cx * d.cy * d.cz;
+    const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
+    if (e3 >= 3 * ncoef)
+        return;
+    double sc;
+    if constexpr (FUSED)
+        sc = -weight * (norm ? 1.0 / ((double)d.nx * d.ny * d.nz) : 1.0);
+    else
+        sc = gscale[i];
+    const int comp = (int)(e3 % 3);
+    const long long e = e3 / 3;
+    const int kx = (int)(e % d.cx);
+    const int ky = (int)((e / d.cx) % d.cy);
+    const int kz = (int)(e / ((long long)d.cx * d.cy));
+    const float *Pi = P + d.part_off + 3 * kx + comp;
+    const long long cell_stride = 48LL * d.cx;
+    double g = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int cz = kz - k;
+        if (cz < 0 || cz >= d.ncz)
+            continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int band = ky - t;
+            if (band >= 0 && band < d.ncy)
+                g += (double)Pi[((long long)cz * d.ncy + band) * cell_stride + (4 * k + t) * 3LL * d.cx];
+        }
+    }
+    grad[d.par_off + e3] = sc * g;
 }
 
 } // namespace mia_hip

@@ -125,6 +125,9 @@ struct mia_hip_ctx {
 
     std::vector<InstDesc> inst;
     std::vector<int> s1_prefix, red_prefix;
+    // 3D reference gradient: stage-1 blocks per (z knot cell, y band), reduce
+    // blocks of kBlock gradient components
+    std::vector<int> s1z_prefix, redz_prefix;
     // warp launches: instances grouped by tile variant (one launch per group)
     struct WarpGroup {
         int variant = 0;           // 3D tile variant; unused in 2D
@@ -137,10 +140,11 @@ struct mia_hip_ctx {
     int force_variant = -1; // -1: tuned per size bucket
     std::vector<char> images_set;
     long long n_img = 0, n_par = 0, n_part = 0, max_vox = 0;
-    int max_nx = 0, max_cx = 0, max_ncx = 0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0;
 
     InstDesc *d_inst = nullptr;
-    int *d_wgroups = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr;
+    int *d_wgroups = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr, *d_s1z_prefix = nullptr,
+        *d_redz_prefix = nullptr;
     float *d_M = nullptr, *d_R = nullptr, *d_W = nullptr, *d_P = nullptr;
     float2 *d_clamp = nullptr, *d_mm = nullptr;
     float4 *d_tw = nullptr;
@@ -174,7 +178,8 @@ namespace {
 
 void free_buffers(mia_hip_ctx *c)
 {
-    void *ptrs[] = {c->d_inst, c->d_wgroups, c->d_s1_prefix, c->d_red_prefix, c->d_M, c->d_R, c->d_W,
+    void *ptrs[] = {c->d_inst, c->d_wgroups, c->d_s1_prefix, c->d_red_prefix, c->d_s1z_prefix, c->d_redz_prefix,
+                    c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
                     c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
@@ -198,7 +203,7 @@ void free_buffers(mia_hip_ctx *c)
     if (c->graph)
         (void)hipGraphDestroy(c->graph);
     c->d_inst = nullptr;
-    c->d_wgroups = c->d_s1_prefix = c->d_red_prefix = nullptr;
+    c->d_wgroups = c->d_s1_prefix = c->d_red_prefix = c->d_s1z_prefix = c->d_redz_prefix = nullptr;
     c->d_M = c->d_R = c->d_W = c->d_P = nullptr;
     c->d_clamp = c->d_mm = nullptr;
     c->d_tw = nullptr;
@@ -408,9 +413,19 @@ int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double 
             if (st)
                 return st;
         }
-    {
+    // 3D reference gradient: instances with zcell run stage 1 per (z knot
+    // cell, y band) and the single-read reduce, the others per (slice, y
+    // band); their SSD partials are kept apart (SsdIndex)
+    const int n_z = c->dim == 3 && !exact ? c->n_zcell : 0;
+    const bool any_slice = n_z < n;
+    SsdIndex si;
+    si.s1 = c->d_s1_prefix;
+    si.s1z = c->d_s1z_prefix;
+    si.zoff = c->s1_prefix[n];
+    si.zmode = c->dim == 3 && !exact ? 1 : 0;
+    const size_t lds = stage1_lds(c, exact);
+    if (any_slice) {
         const int nb = c->s1_prefix[n];
-        const size_t lds = stage1_lds(c, exact);
         decltype(&k_stage1_3d<3, 0>) kern;
         if (c->dim == 3)
             kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
@@ -418,30 +433,50 @@ int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double 
         else
             kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
                                   : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
-        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n, (const int *)c->d_s1_prefix,
-               nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells,
-               (const float *)c->d_M, (const float2 *)c->d_clamp, (const float *)c->d_W, (const float *)c->d_R,
-               c->thr, c->d_P, c->d_ssd, c->d_cnt);
+        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n,
+               (const int *)c->d_s1_prefix, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
+               (const int *)c->d_cells, (const float *)c->d_M, (const float2 *)c->d_clamp, (const float *)c->d_W,
+               (const float *)c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+        LAUNCH_CHECK("stage1");
+    }
+    if (n_z > 0) {
+        const int nb = c->s1z_prefix[n];
+        auto kern = c->degree == 3 ? k_stage1z_3d<3, kS1zRows> : k_stage1z_3d<1, kS1zRows>;
+        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n,
+               (const int *)c->d_s1z_prefix, nb, (const float4 *)c->d_tw, (const int *)c->d_ts,
+               (const int *)c->d_cells, (const float *)c->d_W, (const float *)c->d_R, c->thr, c->d_P,
+               c->d_ssd + si.zoff, c->d_cnt + si.zoff);
         LAUNCH_CHECK("stage1");
     }
     const bool fused = c->thr == 0.0f;
     if (!fused) {
-        launch(c, ST_FINALIZE, timed, k_finalize, dim3(n), 0, (const InstDesc *)c->d_inst,
-               (const int *)c->d_s1_prefix, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr,
-               c->weight, d_costs, c->d_gscale);
+        launch(c, ST_FINALIZE, timed, k_finalize, dim3(n), 0, (const InstDesc *)c->d_inst, si,
+               (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs,
+               c->d_gscale);
         LAUNCH_CHECK("finalize");
     }
-    {
-        const int nb = c->red_prefix[n] + (fused ? n : 0);
+    // with a fused finalize, the first reduce launch does it for every instance
+    int nfin = fused ? n : 0;
+    if (any_slice) {
+        const int nb = c->red_prefix[n] + nfin;
         decltype(&k_reduce<3, true>) kern;
         if (c->dim == 3)
             kern = fused ? k_reduce<3, true> : k_reduce<3, false>;
         else
             kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
         launch(c, ST_REDUCE, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, n, (const int *)c->d_red_prefix,
-               (const float *)c->d_P, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells,
-               (const int *)c->d_s1_prefix, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr,
-               c->weight, d_costs, (const double *)c->d_gscale, d_grad);
+               (const float *)c->d_P, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells, si,
+               nfin, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs,
+               (const double *)c->d_gscale, d_grad);
+        LAUNCH_CHECK("reduce");
+        nfin = 0;
+    }
+    if (n_z > 0) {
+        const int nb = c->redz_prefix[n] + nfin;
+        auto kern = fused ? k_reducez_3d<true> : k_reducez_3d<false>;
+        launch(c, ST_REDUCE, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, n,
+               (const int *)c->d_redz_prefix, (const float *)c->d_P, si, nfin, (const double *)c->d_ssd,
+               (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs, (const double *)c->d_gscale, d_grad);
         LAUNCH_CHECK("reduce");
     }
     return MIA_HIP_OK;
@@ -590,11 +625,13 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->inst.assign(n_inst, InstDesc());
     c->s1_prefix.assign(n_inst + 1, 0);
     c->red_prefix.assign(n_inst + 1, 0);
+    c->s1z_prefix.assign(n_inst + 1, 0);
+    c->redz_prefix.assign(n_inst + 1, 0);
     c->images_set.assign(n_inst, 0);
     std::vector<float> tw;
     std::vector<int> ts, cells;
     long long img = 0, par = 0, part = 0, maxv = 0;
-    int max_nx = 0, max_cx = 0, max_ncx = 0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0;
     for (int i = 0; i < n_inst; ++i) {
         InstDesc &d = c->inst[i];
         d.nx = (int)sizes[3 * i];
@@ -626,6 +663,14 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         d.ncx = ax.nbands;
         d.ncy = ay.nbands;
         d.ncz = az.nbands;
+        int band_rows = 0;
+        for (int k = 0; k < d.ncy; ++k)
+            band_rows = std::max(band_rows, ay.cells[k + 1] - ay.cells[k]);
+        // reference-gradient stage 1 per (z knot cell, y band): one x column
+        // per thread and the band's rows in registers (decided per instance,
+        // so that an instance computes the same in any batch)
+        d.zcell = dim == 3 && d.nx <= kBlock && band_rows + 2 <= kS1zRows ? 1 : 0;
+        n_zcell += d.zcell;
         AxisTable *axes[3] = {&ax, &ay, &az};
         int *toff[3] = {&d.tab_x, &d.tab_y, &d.tab_z};
         int *coff[3] = {&d.cell_x, &d.cell_y, &d.cell_z};
@@ -643,7 +688,10 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         d.par_off = par;
         par += dim * ncoef;
         d.part_off = part;
-        part += dim == 3 ? (long long)d.nz * d.ncy * 12 * d.cx : (long long)d.ncy * 8 * d.cx;
+        // stage-1 partial sums: per (slice, y band) row of 12 cx (exact
+        // gradient) or per (z knot cell, y band) row of 48 cx (reference)
+        part += dim == 3 ? std::max((long long)d.nz * d.ncy * 12 * d.cx, (long long)d.ncz * d.ncy * 48 * d.cx)
+                         : (long long)d.ncy * 8 * d.cx;
         maxv = std::max(maxv, nvox);
         max_nx = std::max(max_nx, d.nx);
         max_cx = std::max(max_cx, d.cx);
@@ -653,6 +701,8 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         // support); 2D: 256 per block
         const int per_block = dim == 3 ? 64 : kBlock;
         c->red_prefix[i + 1] = c->red_prefix[i] + (int)((dim * ncoef + per_block - 1) / per_block);
+        c->s1z_prefix[i + 1] = c->s1z_prefix[i] + d.ncz * d.ncy;
+        c->redz_prefix[i + 1] = c->redz_prefix[i] + (int)((dim * ncoef + kBlock - 1) / kBlock);
     }
     // warp groups: 3D instances by tile variant (tuned per size bucket
     // unless forced), 2D in one group
@@ -706,6 +756,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->max_nx = max_nx;
     c->max_cx = max_cx;
     c->max_ncx = max_ncx;
+    c->n_zcell = n_zcell;
     // stage 1 keeps one x row (plus tables and, in exact mode, a coefficient
     // window) in dynamic LDS; allow more than the 64 KiB default where needed
     for (int exact = 0; exact < 2; ++exact) {
@@ -716,19 +767,26 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                                 {(const void *)k_stage1_3d<3, 1>, (const void *)k_stage1_3d<1, 1>}};
         const void *fn2[2][2] = {{(const void *)k_stage1_2d<3, 0>, (const void *)k_stage1_2d<1, 0>},
                                  {(const void *)k_stage1_2d<3, 1>, (const void *)k_stage1_2d<1, 1>}};
-        if (lds > 64 * 1024)
+        const void *fnz[2] = {(const void *)k_stage1z_3d<3, kS1zRows>, (const void *)k_stage1z_3d<1, kS1zRows>};
+        if (lds > 64 * 1024) {
             for (int k = 0; k < 2; ++k)
                 HIP_TRY(hipFuncSetAttribute(dim == 3 ? fn[exact][k] : fn2[exact][k],
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            if (dim == 3 && !exact)
+                for (int k = 0; k < 2; ++k)
+                    HIP_TRY(hipFuncSetAttribute(fnz[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
     }
     int st;
     if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_wgroups, wg.size())) ||
         (st = dev_alloc(&c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_red_prefix, n_inst + 1)) ||
+        (st = dev_alloc(&c->d_s1z_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_redz_prefix, n_inst + 1)) ||
         (st = dev_alloc(&c->d_M, img)) || (st = dev_alloc(&c->d_R, img)) || (st = dev_alloc(&c->d_W, img)) ||
         (st = dev_alloc(&c->d_P, part)) || (st = dev_alloc(&c->d_clamp, n_inst)) || (st = dev_alloc(&c->d_mm, 1024)) ||
         (st = dev_alloc(&c->d_tw, tw.size() / 4)) ||
         (st = dev_alloc(&c->d_ts, ts.size())) || (st = dev_alloc(&c->d_cells, cells.size())) ||
-        (st = dev_alloc(&c->d_ssd, c->s1_prefix[n_inst])) || (st = dev_alloc(&c->d_cnt, c->s1_prefix[n_inst])) ||
+        (st = dev_alloc(&c->d_ssd, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
+        (st = dev_alloc(&c->d_cnt, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
         (st = dev_alloc(&c->d_gscale, n_inst)) || (st = dev_alloc(&c->d_params, par)) ||
         (st = dev_alloc(&c->d_grad, par)) || (st = dev_alloc(&c->d_costs, n_inst)) ||
         (st = dev_alloc(&c->d_scratch, maxv))) {
@@ -744,6 +802,10 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                            c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_red_prefix, c->red_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
                            c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_s1z_prefix, c->s1z_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_redz_prefix, c->redz_prefix.data(), sizeof(int) * (n_inst + 1),
+                           hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_tw, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_ts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_cells, cells.data(), sizeof(int) * cells.size(), hipMemcpyHostToDevice, c->stream));

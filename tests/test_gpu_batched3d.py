@@ -45,3 +45,35 @@ def test_batched_volumes_vs_oracle():
         assert r["cost_rel"] < 1e-5, r
         assert r["grad_rel_inf"] < 2e-5, r
         assert r["warped_max_abs"] < 2e-5, r
+
+
+def test_mixed_stage1_paths_bitwise():
+    """A batch whose instances take different stage-1 paths (x rows longer
+    than a block, y bands taller than the register window, and instances that
+    fit both) matches one context per instance bit for bit, and the oracle."""
+    import mia_amd
+    from mia_amd import synth
+    shapes = [(12, 20, 300), (25, 38, 27), (37, 25, 27), (30, 41, 64)]
+    data = []
+    for k, sh in enumerate(shapes):
+        fix = synth.smoothed_noise(sh, 2.0, 500 + k)
+        mov = synth.smoothed_noise(sh, 2.0, 600 + k)
+        cs = tuple(O.spline_csize(n, 8.0, 3) for n in sh[::-1])
+        data.append((mov, fix, cs, synth.random_spline_params(cs, 1.5, 3.0, 700 + k)))
+    with mia_amd.HipContext(0) as c:
+        c.setup(3, shapes, [d[2] for d in data], 3)
+        for i, d in enumerate(data):
+            c.set_images(i, d[0], d[1])
+        cb, gb = c.cost_grad(np.concatenate([d[3] for d in data]))
+    off = 0
+    for i, (mov, fix, cs, p) in enumerate(data):
+        with mia_amd.HipContext(0) as c1:
+            c1.setup(3, [shapes[i]], [cs], 3)
+            c1.set_images(0, mov, fix)
+            c1c, g1 = c1.cost_grad(p)
+        assert c1c[0] == cb[i], i
+        assert np.array_equal(g1, gb[off:off + p.size]), i
+        off += p.size
+        c_ref, g_ref = O.cost_grad(mov, fix, cs, p)
+        assert c1c[0] == pytest.approx(c_ref, rel=1e-5)
+        assert np.abs(g1 - g_ref).max() <= 2e-5 * np.abs(g_ref).max()
