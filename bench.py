@@ -37,11 +37,14 @@ def log(*a):
 
 
 def alg_bytes(shape, cs, stage):
-    """Algorithmic HBM bytes per launch of each kernel (float volumes)."""
+    """Algorithmic HBM bytes per launch of each kernel (float volumes).  Stage 1
+    (reference gradient, one block per z knot cell and y band) writes 48 * cx
+    partial sums per block; the reduce reads each once and writes 3 doubles per
+    coefficient."""
     nz, ny, nx = shape
     nvox = nx * ny * nz
-    ncy = cs[1] - 2
-    part = nz * ncy * 12 * cs[0] * 4           # stage-1 partial sums (float)
+    ncy, ncz = cs[1] - 2, cs[2] - 2             # y bands, z knot cells
+    part = ncz * ncy * 48 * cs[0] * 4           # stage-1 partial sums (float)
     ncoef = cs[0] * cs[1] * cs[2]
     return {
         "coef": ncoef * (24 + 16),

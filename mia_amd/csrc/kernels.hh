@@ -682,9 +682,9 @@ __device__ __forceinline__ void dma_dword(__amdgpu_buffer_rsrc_t r, void *lds, u
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 4, voff, soff, 0, 0);
 }
 
-// e / d for 0 <= e < 4096 and 1 <= d <= 512, with inv = 1.0f / d: (e + 0.5) / d
-// is at least 0.5 / d away from an integer, more than its float error
-// (about 2^-23 (e + 0.5) / d <= 4.9e-4).
+// e / d for 0 <= e < 2^21 and d >= 1, with inv = 1.0f / d: (e + 0.5) / d is at
+// least 0.5 / d away from an integer, more than its float error (two roundings,
+// at most about 2^-23 (e + 0.5) / d < 0.25 / d). Callers pass e < 4096.
 __device__ __forceinline__ int small_div(int e, float inv)
 {
     return (int)(((float)e + 0.5f) * inv);

@@ -643,6 +643,10 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         if (d.nx > kMaxRow)
             return fail(MIA_HIP_ERR_LIMIT, "image x size " + std::to_string(d.nx) + " exceeds the limit " +
                                                std::to_string(kMaxRow));
+        // the 3D box fill and z-cell stage 1 address an image with 32-bit byte offsets
+        if (dim == 3 && (long long)d.nx * d.ny * d.nz * (long long)sizeof(float) >= (1LL << 31))
+            return fail(MIA_HIP_ERR_LIMIT, "3D image of instance " + std::to_string(i) +
+                                               " has 2^29 voxels or more (limit of the 32-bit image offsets)");
         AxisTable ax, ay, az;
         int st = make_axis_table(d.nx, d.cx, ax, "x");
         if (st)

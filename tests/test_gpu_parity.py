@@ -288,3 +288,10 @@ def test_errors(mia):
     with pytest.raises(ValueError):
         mia.spline_csize(10, 10.0, 2)
     c.close()
+    # a 3D instance of 2^29 voxels is beyond the 32-bit image offsets; set-up
+    # rejects it before allocating anything
+    c = mia.HipContext(0)
+    with pytest.raises(mia.MiaHipError) as e:
+        c.setup(3, [(512, 1024, 1024)], [(131, 131, 67)])
+    assert e.value.code == mia.ERR_LIMIT
+    c.close()
