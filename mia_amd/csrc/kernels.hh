@@ -208,9 +208,13 @@ __device__ __forceinline__ void axis_index(int s, int n, int *idx)
     }
 }
 
+// v clamped to [vmin, vmax] (vmin <= vmax) as one median of three (values
+// interpolated from finite coefficients are never NaN).
 __device__ __forceinline__ float clamp_val(float v, float vmin, float vmax)
 {
-    return v >= vmin ? (v <= vmax ? v : vmax) : vmin;
+    float r;
+    asm("v_med3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(vmin), "v"(vmax));
+    return r;
 }
 
 // Interpolation of the prefiltered image at p from global memory with mirror
@@ -441,33 +445,26 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// Minimum / maximum over the wave of values that are never NaN, returned in
-// every lane: DPP moves within each row of 16 lanes (quad swaps, then the
-// half-row and row mirrors), then the four row results read into scalars.  No
-// LDS traffic, unlike the shuffles above.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-
-__device__ __forceinline__ float rows16(float v, int k)
+__device__ __forceinline__ float lane_f(float v, int k)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Minimum / maximum of values that are never NaN, as compare and select: two
-// instructions where fminf / fmaxf cost three (IEEE mode quiets both operands
-// before v_min / v_max).  min3 / max3: three operands in one instruction, no
-// quieting (callers rule out NaN inputs or route them elsewhere).
-__device__ __forceinline__ float min_nn(float a, float b)
+// Minimum / maximum of values that are never NaN in one instruction each
+// (fminf / fmaxf cost three: IEEE mode quiets both operands before v_min /
+// v_max); callers rule out NaN inputs or route them elsewhere.
+__device__ __forceinline__ float min2_nn(float a, float b)
 {
-    return b < a ? b : a;
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
-__device__ __forceinline__ float max_nn(float a, float b)
+__device__ __forceinline__ float max2_nn(float a, float b)
 {
-    return b > a ? b : a;
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
 __device__ __forceinline__ float min3_nn(float a, float b, float c)
@@ -494,7 +491,7 @@ __device__ __forceinline__ float min_n(const float *v)
     for (; k + 1 < N; k += 2)
         m = min3_nn(m, v[k], v[k + 1]);
     if (k < N)
-        m = min_nn(m, v[k]);
+        m = min2_nn(m, v[k]);
     return m;
 }
 
@@ -507,26 +504,66 @@ __device__ __forceinline__ float max_n(const float *v)
     for (; k + 1 < N; k += 2)
         m = max3_nn(m, v[k], v[k + 1]);
     if (k < N)
-        m = max_nn(m, v[k]);
+        m = max2_nn(m, v[k]);
     return m;
 }
 
-__device__ __forceinline__ float wave_min_dpp(float v)
+// Minimum of (a, b, c) and maximum of (d, e, f) over the wave, as uniform
+// values.  One v_min / v_max per step with the DPP operand folded in: within
+// each row of 16 lanes (quad swaps, half-row and row mirrors), then row 0 into
+// row 1 and row 2 into row 3 (row_bcast:15), then lane 31 into rows 2 and 3
+// (row_bcast:31); lane 63 ends with the result, read into a scalar.  Rows
+// masked off keep their value (the destination is also the second source).
+// The six reductions interleave, so every DPP read comes six instructions
+// after the write of its source (two wait states are needed; "s_nop 1" covers
+// the inputs).  No NaN quieting: callers pass values that are never NaN.
+__device__ __forceinline__ void wave_min3_max3(float &a, float &b, float &c, float &d, float &e, float &f)
 {
-    v = min_nn(v, dpp_f<0xB1>(v));  // quad_perm [1, 0, 3, 2]
-    v = min_nn(v, dpp_f<0x4E>(v));  // quad_perm [2, 3, 0, 1]
-    v = min_nn(v, dpp_f<0x141>(v)); // row_half_mirror
-    v = min_nn(v, dpp_f<0x140>(v)); // row_mirror
-    return min_nn(min_nn(rows16(v, 0), rows16(v, 16)), min_nn(rows16(v, 32), rows16(v, 48)));
-}
-
-__device__ __forceinline__ float wave_max_dpp(float v)
-{
-    v = max_nn(v, dpp_f<0xB1>(v));
-    v = max_nn(v, dpp_f<0x4E>(v));
-    v = max_nn(v, dpp_f<0x141>(v));
-    v = max_nn(v, dpp_f<0x140>(v));
-    return max_nn(max_nn(rows16(v, 0), rows16(v, 16)), max_nn(rows16(v, 32), rows16(v, 48)));
+    asm volatile("s_nop 1\n\t"
+                 "v_min_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %1, %1, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "v_min_f32_dpp %2, %2, %2 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %3, %3, %3 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %4, %4, %4 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "v_max_f32_dpp %5, %5, %5 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+                 "s_nop 1"
+                 : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "+v"(e), "+v"(f));
+    a = lane_f(a, 63);
+    b = lane_f(b, 63);
+    c = lane_f(c, 63);
+    d = lane_f(d, 63);
+    e = lane_f(e, 63);
+    f = lane_f(f, 63);
 }
 
 // Deterministic block sum (fixed shuffle tree per wave, then waves in order);
@@ -819,7 +856,7 @@ __global__ void __launch_bounds__(kBlock)
 {
     constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
     __shared__ float4 buf4[TL::Buf4];
-    __shared__ float red[6][kBlock / 64];
+    __shared__ __align__(16) float red[6][kBlock / 64];
     __shared__ float4 wx_s[TL::TX], wy_s[TL::TY], wz_s[TL::TZ];
     __shared__ int sx_s[TL::TX], sy_s[TL::TY], sz_s[TL::TZ];
     float *region = (float *)buf4;
@@ -1026,12 +1063,7 @@ __global__ void __launch_bounds__(kBlock)
         mnx = -FLT_MAX;
     // bounding box of the taps of all valid voxels of the tile; the barrier
     // also ends every read of the phase-2 intermediates, which the box reuses
-    mnx = wave_min_dpp(mnx);
-    mny = wave_min_dpp(mny);
-    mnz = wave_min_dpp(mnz);
-    mxx = wave_max_dpp(mxx);
-    mxy = wave_max_dpp(mxy);
-    mxz = wave_max_dpp(mxz);
+    wave_min3_max3(mnx, mny, mnz, mxx, mxy, mxz);
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][wid] = mnx;
@@ -1042,15 +1074,13 @@ __global__ void __launch_bounds__(kBlock)
         red[5][wid] = mxz;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kBlock / 64; ++k) {
-        mnx = min_nn(mnx, red[0][k]);
-        mny = min_nn(mny, red[1][k]);
-        mnz = min_nn(mnz, red[2][k]);
-        mxx = max_nn(mxx, red[3][k]);
-        mxy = max_nn(mxy, red[4][k]);
-        mxz = max_nn(mxz, red[5][k]);
-    }
+    static_assert(kBlock / 64 == 4, "one float4 of wave results per quantity");
+    mnx = min_n<4>(red[0]);
+    mny = min_n<4>(red[1]);
+    mnz = min_n<4>(red[2]);
+    mxx = max_n<4>(red[3]);
+    mxy = max_n<4>(red[4]);
+    mxz = max_n<4>(red[5]);
     // every lane holds the same box; readfirstlane makes that visible to the
     // compiler so that the fill below runs its row arithmetic on the scalar unit
     const int rx0 = __builtin_amdgcn_readfirstlane(fastfloor16(clamp_pos(mnx)) - LO);
