@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace mia_hip {
 
@@ -1606,11 +1607,19 @@ __global__ void __launch_bounds__(kBlock, 4)
         }
     }
     double ssd = 0.0;
-    unsigned cnt = 0;
+    unsigned cnt_w = 0; // the wave's count, a scalar
     float a48[48];
 #pragma unroll
     for (int k = 0; k < 48; ++k)
         a48[k] = 0.0f;
+    // lanes beyond the image count nothing and their force is never stored
+    const bool xin = x < nx;
+    // edge column read by this lane: lane 0 the previous wave's lane 63,
+    // lane 63 the next wave's lane 0 (the DPP shifts below use it only there;
+    // at the block's edges the lane is on the image border, where the x
+    // difference is zeroed, so any value will do)
+    const int ew = lane == 0 ? max(wv - 1, 0) : (lane == 63 ? min(wv + 1, NW - 1) : wv);
+    const int es = lane == 0 ? 1 : 0;
     for (int z = zb; z < ze; ++z) {
         // next slice's loads: W of z + 2, R of z + 1
         if (z + 1 < ze) {
@@ -1636,6 +1645,7 @@ __global__ void __launch_bounds__(kBlock, 4)
         const float4 wzv = tw[d.tab_z + z];
         const float wzs[4] = {wzv.x, wzv.y, wzv.z, wzv.w};
         const bool iz = z > 0 && z < nz - 1;
+        const float *er = edge_s[par][ew][es];
         float acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k)
@@ -1645,22 +1655,23 @@ __global__ void __launch_bounds__(kBlock, 4)
             if (r + 1 < nrw) {
                 const int j = j0 + r - 1;
                 const float w = wc[r];
-                const float em = (lane == 0 && wv > 0) ? edge_s[par][wv - 1][1][r] : w;
-                const float ep = (lane == 63 && wv + 1 < NW) ? edge_s[par][wv + 1][0][r] : w;
-                const float wxm = lane_prev(w, em), wxp = lane_next(w, ep);
+                const float e = er[r];
+                const float wxm = lane_prev(w, e), wxp = lane_next(w, e);
                 const float4 wyv = tw[d.tab_y + j];
                 const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
                 const bool iy = j > 0 && j < ny - 1;
-                const float gx = ix ? 0.5f * (wxp - wxm) : 0.0f;
-                const float gy = iy ? 0.5f * (wc[r + 1] - wc[r - 1]) : 0.0f;
-                const float gz = iz ? 0.5f * (wp[r] - wm[r]) : 0.0f;
-                const bool on = (thr == 0.0f) || (w > thr);
+                const bool on = xin && (thr == 0.0f || w > thr);
                 const float dl = on ? w - rc[r] : 0.0f;
-                if (x < nx) {
-                    ssd += (double)dl * (double)dl;
-                    cnt += on ? 1u : 0u;
-                }
-                const float fx = gx * dl, fy = gy * dl, fz = gz * dl;
+                ssd += (double)dl * (double)dl;
+                cnt_w += (unsigned)__popcll(__ballot(on));
+                // force = gradient * difference with the reference's 0.5 on
+                // the difference: the same products as (0.5 * (w+ - w-)) * dl,
+                // 0.5 being exact; the y and z components are zero on the
+                // image border, their sums are skipped there (uniform branches)
+                const float hdl = 0.5f * dl;
+                const float fx = (wxp - wxm) * (ix ? hdl : 0.0f);
+                const float fy = (wc[r + 1] - wc[r - 1]) * (iy ? hdl : 0.0f);
+                const float fz = (wp[r] - wm[r]) * (iz ? hdl : 0.0f);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     acc[3 * t + 0] = fmaf(wys[t], fx, acc[3 * t + 0]);
@@ -1683,7 +1694,7 @@ __global__ void __launch_bounds__(kBlock, 4)
         }
     }
     const double s = block_sum(ssd, red);
-    const unsigned c = block_sum_u(cnt, redu);
+    const unsigned c = block_sum_u(lane == 0 ? cnt_w : 0u, redu);
     if (threadIdx.x == 0) {
         ssd_part[b] = s;
         cnt_part[b] = c;
