@@ -42,15 +42,21 @@ constexpr int kMaxRow = 512;       // longest x row the stage-1 kernel keeps in 
 // per thread along z.  The displacement intermediates (phases 1-2) and the box
 // of moving-image taps (phases 3-4) share one LDS buffer; the box has
 // compile-time pitches PX x PY (x PZ slices) so that the 64 taps of a voxel are
-// one base address plus immediate offsets.  A wave covers 64 / TX rows of the
-// tile, 4 rows apart; PX is chosen so that those rows start on different
-// quarters of the LDS banks.
+// one base address plus immediate offsets.  LDS serves a wave's two halves
+// (32 lanes each) one after the other, 32 banks of 4 bytes; the voxels of a
+// half-wave are 8 x columns by 4 rows (G8X4, when PX = 8 mod 32: the four
+// rows' taps start a quarter of the banks apart, and a deformation that moves
+// a row's taps by a row stays within its quarter), else TX columns of 32 / TX
+// rows.  A tap gather then costs about 1.5 bank cycles per half-wave at
+// spline-scale deformations, where one 32-column row costs about 2 (a row
+// switch inside it shifts its banks by PX).
 template <int TX_, int TY_, int TZ_, int PX_, int PY_, int PZ_>
 struct W3Tile {
     static constexpr int TX = TX_, TY = TY_, TZ = TZ_, PX = PX_, PY = PY_, PZ = PZ_;
     static_assert(TX * TY == kBlock, "one thread per (x, y) column");
     static constexpr int LOG_TX = TX == 16 ? 4 : TX == 32 ? 5 : 6;
     static constexpr int ROWS = 64 / TX; // tile rows per wave
+    static constexpr bool G8X4 = PX % 32 == 8 && TX % 8 == 0 && TY % 4 == 0;
     static constexpr int Buf = PX * PY * PZ;
     static constexpr int Buf4 = (Buf + 3) / 4;
 };
@@ -871,10 +877,17 @@ __global__ void __launch_bounds__(kBlock)
     const int tx = lb % tiles_x;
     const int rest = lb / tiles_x;
     const int x0 = tx * TL::TX, y0 = (rest % tiles_y) * TL::TY, z0 = (rest / tiles_y) * TL::TZ;
-    // a wave holds tile rows w, w + 4, ... (64 / TX of them); their LDS box
-    // rows are 4 * PX floats apart, on different quarters of the banks
-    const int xl = threadIdx.x & (TL::TX - 1);
-    const int yl = ((threadIdx.x >> TL::LOG_TX) & (TL::ROWS - 1)) * 4 + (threadIdx.x >> 6);
+    // the tile's (x, y) column of this thread: half-wave h holds 8 columns
+    // by 4 rows (G8X4), else a wave holds tile rows w, w + 4, ...
+    int xl, yl;
+    if constexpr (TL::G8X4) {
+        const int h = threadIdx.x >> 5, hl = threadIdx.x & 31;
+        xl = (h % (TL::TX / 8)) * 8 + (hl & 7);
+        yl = (h / (TL::TX / 8)) * 4 + (hl >> 3);
+    } else {
+        xl = threadIdx.x & (TL::TX - 1);
+        yl = ((threadIdx.x >> TL::LOG_TX) & (TL::ROWS - 1)) * 4 + (threadIdx.x >> 6);
+    }
     const int x = x0 + xl, y = y0 + yl;
     const bool valid = x < d.nx && y < d.ny;
     const int nzl = min(TL::TZ, d.nz - z0);
