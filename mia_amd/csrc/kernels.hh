@@ -168,15 +168,26 @@ __device__ __forceinline__ int mirror_idx(int i, int n)
 }
 
 // Image-kernel start index and weights (and derivative weights) for degree
-// 3 (mia/core/splinekernel/bspline.cc:258-285) or 1 (:90-107).
-template <int DEG, bool GRAD>
+// 3 (mia/core/splinekernel/bspline.cc:258-285) or 1 (:90-107).  SCALED
+// (degree 3): the weights times 6, without the two multiplications by 1/6;
+// a product of three axes' scaled weights then carries the factor 216
+// (kWScale3).
+template <int DEG, bool GRAD, bool SCALED = false>
 __device__ __forceinline__ int kweights(float p, float *w, float *dw)
 {
     // positions are clamped to +-1e6, so the floor is an exact integer in float
     const float flf = fastfloor16f(p);
     const int fl = (int)flf;
     const float f = p - flf;
-    if constexpr (DEG == 3) {
+    if constexpr (DEG == 3 && SCALED) {
+        static_assert(!GRAD, "scaled weights carry no derivative");
+        const float fm1 = 1.0f - f;
+        w[3] = f * f * f;
+        w[0] = fm1 * fm1 * fm1;
+        w[2] = fmaf(6.0f, f, fmaf(-2.0f, w[3], w[0]));
+        w[1] = fmaf(6.0f, fm1, fmaf(-2.0f, w[0], w[3]));
+        return fl - 1;
+    } else if constexpr (DEG == 3) {
         const float fm1 = 1.0f - f;
         const float s = 1.0f / 6.0f;
         w[3] = s * f * f * f;
@@ -224,19 +235,24 @@ __device__ __forceinline__ float clamp_val(float v, float vmin, float vmax)
     return r;
 }
 
+// 1 / 216: the factor of a trilinear product of scaled cubic weights
+template <int DEG>
+constexpr float kWScale3 = DEG == 3 ? 1.0f / 216.0f : 1.0f;
+
 // Interpolation of the prefiltered image at p from global memory with mirror
 // indices and clamp to [vmin, vmax] (mia/3d/interpolator.cxx:250-287,
 // z->y->x contraction as add_3d :187-210); GRAD adds the analytic gradient,
-// zero where the value is clamped.
+// zero where the value is clamped.  Without GRAD the weights are the scaled
+// ones and the sum is multiplied by kWScale3 once, as in the warp's LDS path.
 template <int DEG, bool GRAD>
 __device__ __forceinline__ float interp3(const float *__restrict__ M, int nx, int ny, int nz, float px,
                                          float py, float pz, float vmin, float vmax, float *g)
 {
     constexpr int NS = DEG == 3 ? 4 : 2;
     float wx[NS], wy[NS], wz[NS], dx[NS], dy[NS], dz[NS];
-    const int sx = kweights<DEG, GRAD>(px, wx, dx);
-    const int sy = kweights<DEG, GRAD>(py, wy, dy);
-    const int sz = kweights<DEG, GRAD>(pz, wz, dz);
+    const int sx = kweights<DEG, GRAD, !GRAD>(px, wx, dx);
+    const int sy = kweights<DEG, GRAD, !GRAD>(py, wy, dy);
+    const int sz = kweights<DEG, GRAD, !GRAD>(pz, wz, dz);
     int ix[NS], iy[NS], iz[NS];
     axis_index<NS>(sx, nx, ix);
     axis_index<NS>(sy, ny, iy);
@@ -275,6 +291,8 @@ __device__ __forceinline__ float interp3(const float *__restrict__ M, int nx, in
         g[0] = clamped ? 0.0f : gx;
         g[1] = clamped ? 0.0f : gy;
         g[2] = clamped ? 0.0f : gz;
+    } else if constexpr (DEG == 3) {
+        v *= kWScale3<DEG>;
     }
     return clamp_val(v, vmin, vmax);
 }
@@ -735,24 +753,25 @@ __device__ __forceinline__ int small_div(int e, float inv)
 }
 
 // The NS^3 taps of one voxel from an LDS box with compile-time pitches
-// (b0: index of the first tap), z -> y -> x contraction as contract_lds3.
+// (o0: byte offset of the first tap), z -> y -> x contraction as
+// contract_lds3.
 template <int DEG, int PX, int PY>
-__device__ __forceinline__ float box_contract(const float *box, int b0, const float *wa, const float *wb,
+__device__ __forceinline__ float box_contract(const float *box, unsigned o0, const float *wa, const float *wb,
                                               const float *wc)
 {
     constexpr int NS = DEG == 3 ? 4 : 2;
     float v = 0.0f;
 #pragma unroll
     for (int c = 0; c < NS; ++c) {
-        // an opaque copy of the slice's index: the slice's taps are then
-        // immediate offsets from one register (left alone, the compiler forms
-        // a new address for most pairs of taps beyond the first slice)
-        int bc = b0 + c * PY * PX;
-        asm("" : "+v"(bc));
+        // an opaque copy of the slice's byte offset: the slice's taps are
+        // then immediate offsets from one register (left alone, the compiler
+        // forms a new address for most pairs of taps beyond the first slice)
+        unsigned oc = o0 + (unsigned)(c * PY * PX * 4);
+        asm("" : "+v"(oc));
         float vy = 0.0f;
 #pragma unroll
         for (int bb = 0; bb < NS; ++bb) {
-            const float *row = box + bc + bb * PX;
+            const float *row = (const float *)((const char *)box + oc) + bb * PX;
             float vx = 0.0f;
 #pragma unroll
             for (int a = 0; a < NS; ++a)
@@ -1115,28 +1134,41 @@ __global__ void __launch_bounds__(kBlock)
     if (!valid)
         return;
     const float2 cl = clamp[i];
-    float *Wi = W + d.img_off + ((long long)z0 * d.ny + y) * d.nx + x;
-    const long long plane = (long long)d.nx * d.ny;
+    // W stores as buffer stores: the lane's column offset plus the slice's
+    // scalar offset (images are below 2^31 bytes)
+    const __amdgpu_buffer_rsrc_t rwo = raw_rsrc(W + d.img_off);
+    const unsigned wvo = (unsigned)(y * d.nx + x) * 4u;
+    const unsigned planeb = (unsigned)(d.nx * d.ny) * 4u;
+    auto store = [&](int zl, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rwo, wvo, (unsigned)(z0 + zl) * planeb, 0);
+    };
     if (use_lds) {
-        // phase 4: the taps at one base address plus compile-time offsets
+        // phase 4: the taps at one base address plus compile-time offsets;
+        // the first tap's byte offset in the box is
+        // 4 (s2 PX PY + s1 PX + s0) - kbox, kbox from the box origin
+        const int kbox = 4 * ((rz0 * TL::PY + ry0) * TL::PX + rx0);
+        // the clamp bounds in vector registers once (not copied per voxel)
+        float vlo = cl.x, vhi = cl.y;
+        asm("" : "+v"(vlo), "+v"(vhi));
 #pragma unroll
         for (int zl = 0; zl < TL::TZ; ++zl) {
             if (zl < nzl) {
                 float wa[NS], wb[NS], wc[NS];
-                const int s0 = kweights<DEG, false>(px[zl], wa, nullptr);
-                const int s1 = kweights<DEG, false>(py[zl], wb, nullptr);
-                const int s2 = kweights<DEG, false>(pz[zl], wc, nullptr);
-                const int b0 = ((s2 - rz0) * TL::PY + (s1 - ry0)) * TL::PX + (s0 - rx0);
-                const float v = box_contract<DEG, TL::PX, TL::PY>(region, b0, wa, wb, wc);
-                Wi[zl * plane] = clamp_val(v, cl.x, cl.y);
+                const int s0 = kweights<DEG, false, true>(px[zl], wa, nullptr);
+                const int s1 = kweights<DEG, false, true>(py[zl], wb, nullptr);
+                const int s2 = kweights<DEG, false, true>(pz[zl], wc, nullptr);
+                // (24-bit products: the indices are within the image's size)
+                const unsigned o0 =
+                    (unsigned)(__mul24(s2, 4 * TL::PY * TL::PX) + __mul24(s1, 4 * TL::PX) + s0 * 4 - kbox);
+                const float v = box_contract<DEG, TL::PX, TL::PY>(region, o0, wa, wb, wc) * kWScale3<DEG>;
+                store(zl, clamp_val(v, vlo, vhi));
             }
         }
     } else {
 #pragma unroll
         for (int zl = 0; zl < TL::TZ; ++zl)
             if (zl < nzl)
-                Wi[zl * plane] =
-                    interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, px[zl], py[zl], pz[zl], cl.x, cl.y, nullptr);
+                store(zl, interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, px[zl], py[zl], pz[zl], cl.x, cl.y, nullptr));
     }
 }
 
