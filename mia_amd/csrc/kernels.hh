@@ -153,6 +153,16 @@ __device__ __forceinline__ float clamp_pos(float p)
     return fminf(fmaxf(p, -1.0e6f), 1.0e6f);
 }
 
+// The same clamp as one median of three, equal to clamp_pos for any p that
+// is not NaN (a NaN p gives an unspecified value; callers that may see one
+// apply clamp_pos to the result).
+__device__ __forceinline__ float clamp_pos_nn(float p)
+{
+    float r;
+    asm("v_med3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(-1.0e6f), "v"(1.0e6f));
+    return r;
+}
+
 // mirror boundary index map (mia/core/splinebc/bc.cc:52-64)
 __device__ __forceinline__ int mirror_idx(int i, int n)
 {
@@ -1086,9 +1096,11 @@ __global__ void __launch_bounds__(kBlock)
         qx[zl] = xc - ux[zl];
         qy[zl] = yc - uy[zl];
         qz[zl] = (float)min(z0 + zl, d.nz - 1) - uz[zl];
-        px[zl] = clamp_pos(qx[zl]);
-        py[zl] = clamp_pos(qy[zl]);
-        pz[zl] = clamp_pos(qz[zl]);
+        // NaN positions (non-finite coefficients) only reach the global
+        // path below, which clamps them again
+        px[zl] = clamp_pos_nn(qx[zl]);
+        py[zl] = clamp_pos_nn(qy[zl]);
+        pz[zl] = clamp_pos_nn(qz[zl]);
     }
     float mnx = min_n<TL::TZ>(qx), mny = min_n<TL::TZ>(qy), mnz = min_n<TL::TZ>(qz);
     float mxx = max_n<TL::TZ>(qx), mxy = max_n<TL::TZ>(qy), mxz = max_n<TL::TZ>(qz);
@@ -1168,7 +1180,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int zl = 0; zl < TL::TZ; ++zl)
             if (zl < nzl)
-                store(zl, interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, px[zl], py[zl], pz[zl], cl.x, cl.y, nullptr));
+                store(zl, interp3<DEG, false>(Mi, d.nx, d.ny, d.nz, clamp_pos(px[zl]), clamp_pos(py[zl]),
+                                              clamp_pos(pz[zl]), cl.x, cl.y, nullptr));
     }
 }
 
