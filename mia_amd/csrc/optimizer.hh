@@ -66,6 +66,109 @@ struct OptBuffers {
     OptState *st;
 };
 
+// Instances of up to kLbRegN parameters run the direction search with the
+// block's vectors in registers (element e = thread + k * kBlock, k < kLbK):
+// the same operations in the same per-thread order as the global-memory
+// loop below (bitwise-equal results), with each pass's S and Y loaded one
+// pass ahead, so that only the history is read from memory.
+constexpr int kLbK = 8;
+constexpr long long kLbRegN = (long long)kLbK * kBlock;
+
+__device__ __forceinline__ void lb_load(double *v, const double *__restrict__ src, long long n)
+{
+#pragma unroll
+    for (int k = 0; k < kLbK; ++k) {
+        const long long e = threadIdx.x + (long long)k * kBlock;
+        v[k] = e < n ? src[e] : 0.0;
+    }
+}
+
+// q <- H g (two-loop recursion), D <- -q, returns g.D; pairs `kk` from `head`
+__device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int kk, int head, const double *S,
+                                                    const double *Y, const double *RHO, const double *G,
+                                                    double *D, double *red, double *alph)
+{
+    double q[kLbK], g[kLbK], sv[kLbK], yv[kLbK], sn[kLbK], yn[kLbK];
+    lb_load(g, G, n);
+#pragma unroll
+    for (int k = 0; k < kLbK; ++k)
+        q[k] = g[k];
+    if (kk > 0) {
+        const int slot = (head + kk - 1) % memory;
+        lb_load(sv, S + (long long)slot * n, n);
+        lb_load(yv, Y + (long long)slot * n, n);
+    }
+    // newest pair first
+    for (int j = kk - 1; j >= 0; --j) {
+        const int slot = (head + j) % memory;
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k)
+            a += sv[k] * q[k];
+        if (j > 0) {
+            const int nslot = (head + j - 1) % memory;
+            lb_load(sn, S + (long long)nslot * n, n);
+            lb_load(yn, Y + (long long)nslot * n, n);
+        }
+        a = block_sum_all(a, red) * RHO[slot];
+        if (threadIdx.x == 0)
+            alph[j] = a;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k) {
+            q[k] -= a * yv[k];
+            sv[k] = sn[k];
+            yv[k] = yn[k];
+        }
+    }
+    if (kk > 0) {
+        const int slot = (head + kk - 1) % memory;
+        lb_load(yv, Y + (long long)slot * n, n);
+        double yy = 0.0;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k)
+            yy += yv[k] * yv[k];
+        yy = block_sum_all(yy, red);
+        const double gamma = 1.0 / (RHO[slot] * yy);
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k)
+            q[k] *= gamma;
+        lb_load(sv, S + (long long)(head % memory) * n, n);
+        lb_load(yv, Y + (long long)(head % memory) * n, n);
+    }
+    __syncthreads(); // alph[] complete
+    for (int j = 0; j < kk; ++j) {
+        const int slot = (head + j) % memory;
+        double bsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k)
+            bsum += yv[k] * q[k];
+        if (j + 1 < kk) {
+            const int nslot = (head + j + 1) % memory;
+            lb_load(sn, S + (long long)nslot * n, n);
+            lb_load(yn, Y + (long long)nslot * n, n);
+        }
+        const double b = block_sum_all(bsum, red) * RHO[slot];
+        const double c = alph[j] - b;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k) {
+            q[k] += c * sv[k];
+            sv[k] = sn[k];
+            yv[k] = yn[k];
+        }
+    }
+    double gd = 0.0;
+#pragma unroll
+    for (int k = 0; k < kLbK; ++k) {
+        const long long e = threadIdx.x + (long long)k * kBlock;
+        if (e < n) {
+            const double dk = -q[k];
+            D[e] = dk;
+            gd += g[k] * dk;
+        }
+    }
+    return block_sum_all(gd, red);
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_lbfgs_step(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
                  const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
@@ -199,8 +302,12 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     if (search) {
-        // two-loop recursion on q (kept in D), newest pair first
         const int k = s.k, head = s.head;
+        double gd;
+        if (n <= kLbRegN) {
+            gd = lbfgs_search_regs(n, memory, k, head, S, Y, RHO, G, D, red, alph);
+        } else {
+        // two-loop recursion on q (kept in D), newest pair first
         for (long long e = threadIdx.x; e < n; e += kBlock)
             D[e] = G[e];
         __syncthreads();
@@ -241,12 +348,13 @@ __global__ void __launch_bounds__(kBlock)
                 D[e] += c * Sj[e];
             __syncthreads();
         }
-        double gd = 0.0;
+        gd = 0.0;
         for (long long e = threadIdx.x; e < n; e += kBlock) {
             D[e] = -D[e];
             gd += G[e] * D[e];
         }
         gd = block_sum_all(gd, red);
+        }
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
             gd = 0.0;
             for (long long e = threadIdx.x; e < n; e += kBlock) {
