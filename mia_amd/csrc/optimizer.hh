@@ -22,37 +22,61 @@ struct OptState {
     int k, head; // memory pairs stored, index of the oldest
 };
 
-// Block sum with the result in every thread (fixed order).
-__device__ __forceinline__ double block_sum_all(double v, double *red)
+// A double moved across lanes by DPP (both halves; lanes that the row mask
+// leaves out, or whose source is out of range, receive `fill`)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp64(double v, double fill)
 {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-        red[wid] = v;
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave reduction in a fixed order, result in lane 63: within each row of 16
+// lanes (quad swaps, half-row and row mirrors), then row 0 into row 1 and row
+// 2 into row 3 (row_bcast:15), then lane 31 into rows 2 and 3 (row_bcast:31).
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce63(double v)
+{
+    const double id = MAX ? -INFINITY : 0.0;
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dpp64<0xB1, 0xf>(v, id));
+    v = op(v, dpp64<0x4E, 0xf>(v, id));
+    v = op(v, dpp64<0x141, 0xf>(v, id));
+    v = op(v, dpp64<0x140, 0xf>(v, id));
+    v = op(v, dpp64<0x142, 0xa>(v, id));
+    v = op(v, dpp64<0x143, 0xc>(v, id));
+    return v;
+}
+
+// Block sum / max with the result in every thread (fixed order).  red holds
+// two sets of per-wave results used alternately (par flips per call, the same
+// in every thread), so one barrier per reduction suffices: a set is written
+// again only after every thread has passed the next reduction's barrier.
+template <bool MAX>
+__device__ __forceinline__ double block_reduce_all(double v, double *red, int &par)
+{
+    v = wave_reduce63<MAX>(v);
+    double *r = red + par * (kBlock / 64);
+    par ^= 1;
+    if ((threadIdx.x & 63) == 63)
+        r[threadIdx.x >> 6] = v;
     __syncthreads();
-    double s = 0.0;
+    double s = r[0];
 #pragma unroll
-    for (int k = 0; k < kBlock / 64; ++k)
-        s += red[k];
-    __syncthreads();
+    for (int k = 1; k < kBlock / 64; ++k)
+        s = MAX ? fmax(s, r[k]) : s + r[k];
     return s;
 }
 
-__device__ __forceinline__ double block_max_all(double v, double *red)
+__device__ __forceinline__ double block_sum_all(double v, double *red, int &par)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v = fmax(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-        red[wid] = v;
-    __syncthreads();
-    double s = red[0];
-#pragma unroll
-    for (int k = 1; k < kBlock / 64; ++k)
-        s = fmax(s, red[k]);
-    __syncthreads();
-    return s;
+    return block_reduce_all<false>(v, red, par);
+}
+
+__device__ __forceinline__ double block_max_all(double v, double *red, int &par)
+{
+    return block_reduce_all<true>(v, red, par);
 }
 
 // Buffers (doubles) per instance i with n = dim * ncoef parameters at offset
@@ -86,7 +110,7 @@ __device__ __forceinline__ void lb_load(double *v, const double *__restrict__ sr
 // q <- H g (two-loop recursion), D <- -q, returns g.D; pairs `kk` from `head`
 __device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int kk, int head, const double *S,
                                                     const double *Y, const double *RHO, const double *G,
-                                                    double *D, double *red, double *alph)
+                                                    double *D, double *red, int &rpar, double *alph)
 {
     double q[kLbK], g[kLbK], sv[kLbK], yv[kLbK], sn[kLbK], yn[kLbK];
     lb_load(g, G, n);
@@ -110,7 +134,7 @@ __device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int
             lb_load(sn, S + (long long)nslot * n, n);
             lb_load(yn, Y + (long long)nslot * n, n);
         }
-        a = block_sum_all(a, red) * RHO[slot];
+        a = block_sum_all(a, red, rpar) * RHO[slot];
         if (threadIdx.x == 0)
             alph[j] = a;
 #pragma unroll
@@ -127,7 +151,7 @@ __device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int
 #pragma unroll
         for (int k = 0; k < kLbK; ++k)
             yy += yv[k] * yv[k];
-        yy = block_sum_all(yy, red);
+        yy = block_sum_all(yy, red, rpar);
         const double gamma = 1.0 / (RHO[slot] * yy);
 #pragma unroll
         for (int k = 0; k < kLbK; ++k)
@@ -147,7 +171,7 @@ __device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int
             lb_load(sn, S + (long long)nslot * n, n);
             lb_load(yn, Y + (long long)nslot * n, n);
         }
-        const double b = block_sum_all(bsum, red) * RHO[slot];
+        const double b = block_sum_all(bsum, red, rpar) * RHO[slot];
         const double c = alph[j] - b;
 #pragma unroll
         for (int k = 0; k < kLbK; ++k) {
@@ -166,7 +190,7 @@ __device__ __forceinline__ double lbfgs_search_regs(long long n, int memory, int
             gd += g[k] * dk;
         }
     }
-    return block_sum_all(gd, red);
+    return block_sum_all(gd, red, rpar);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -174,8 +198,9 @@ __global__ void __launch_bounds__(kBlock)
                  const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
                  double first_step)
 {
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[2 * (kBlock / 64)];
     __shared__ double alph[kMaxMemory];
+    int rpar = 0; // which half of red the next block reduction uses
     __shared__ OptState s;
     const int i = blockIdx.x;
     const InstDesc d = inst[i];
@@ -214,7 +239,7 @@ __global__ void __launch_bounds__(kBlock)
         double gmax = 0.0;
         for (long long e = threadIdx.x; e < n; e += kBlock)
             gmax = fmax(gmax, fabs(GT[e]));
-        gmax = block_max_all(gmax, red);
+        gmax = block_max_all(gmax, red, rpar);
         if (!finite || gmax <= 0.0 || s.evals >= maxeval)
             finish = true;
         else {
@@ -243,9 +268,9 @@ __global__ void __launch_bounds__(kBlock)
                 yy += ye * ye;
                 ss += se * se;
             }
-            sy = block_sum_all(sy, red);
-            yy = block_sum_all(yy, red);
-            ss = block_sum_all(ss, red);
+            sy = block_sum_all(sy, red, rpar);
+            yy = block_sum_all(yy, red, rpar);
+            ss = block_sum_all(ss, red, rpar);
             if (sy > 1e-12 * sqrt(ss * yy)) {
                 const int slot = s.k < memory ? (s.head + s.k) % memory : s.head;
                 double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
@@ -268,7 +293,7 @@ __global__ void __launch_bounds__(kBlock)
                 G[e] = GT[e];
                 gmax = fmax(gmax, fabs(GT[e]));
             }
-            gmax = block_max_all(gmax, red);
+            gmax = block_max_all(gmax, red, rpar);
             if (threadIdx.x == 0)
                 s.f = ft;
             __syncthreads();
@@ -305,7 +330,7 @@ __global__ void __launch_bounds__(kBlock)
         const int k = s.k, head = s.head;
         double gd;
         if (n <= kLbRegN) {
-            gd = lbfgs_search_regs(n, memory, k, head, S, Y, RHO, G, D, red, alph);
+            gd = lbfgs_search_regs(n, memory, k, head, S, Y, RHO, G, D, red, rpar, alph);
         } else {
         // two-loop recursion on q (kept in D), newest pair first
         for (long long e = threadIdx.x; e < n; e += kBlock)
@@ -317,7 +342,7 @@ __global__ void __launch_bounds__(kBlock)
             double a = 0.0;
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 a += Sj[e] * D[e];
-            a = block_sum_all(a, red) * RHO[slot];
+            a = block_sum_all(a, red, rpar) * RHO[slot];
             if (threadIdx.x == 0)
                 alph[j] = a;
             for (long long e = threadIdx.x; e < n; e += kBlock)
@@ -330,7 +355,7 @@ __global__ void __launch_bounds__(kBlock)
             double yy = 0.0;
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 yy += Yj[e] * Yj[e];
-            yy = block_sum_all(yy, red);
+            yy = block_sum_all(yy, red, rpar);
             const double gamma = 1.0 / (RHO[slot] * yy);
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 D[e] *= gamma;
@@ -342,7 +367,7 @@ __global__ void __launch_bounds__(kBlock)
             double bsum = 0.0;
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 bsum += Yj[e] * D[e];
-            const double b = block_sum_all(bsum, red) * RHO[slot];
+            const double b = block_sum_all(bsum, red, rpar) * RHO[slot];
             const double c = alph[j] - b;
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 D[e] += c * Sj[e];
@@ -353,7 +378,7 @@ __global__ void __launch_bounds__(kBlock)
             D[e] = -D[e];
             gd += G[e] * D[e];
         }
-        gd = block_sum_all(gd, red);
+        gd = block_sum_all(gd, red, rpar);
         }
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
             gd = 0.0;
@@ -361,7 +386,7 @@ __global__ void __launch_bounds__(kBlock)
                 D[e] = -G[e];
                 gd += G[e] * D[e];
             }
-            gd = block_sum_all(gd, red);
+            gd = block_sum_all(gd, red, rpar);
             if (threadIdx.x == 0) {
                 s.k = 0;
                 s.head = 0;
@@ -373,7 +398,7 @@ __global__ void __launch_bounds__(kBlock)
             double dmax = 0.0;
             for (long long e = threadIdx.x; e < n; e += kBlock)
                 dmax = fmax(dmax, fabs(D[e]));
-            dmax = block_max_all(dmax, red);
+            dmax = block_max_all(dmax, red, rpar);
             if (dmax > 0.0)
                 alpha = fmin(1.0, first_step / dmax);
         }
