@@ -1397,6 +1397,53 @@ __device__ __forceinline__ void contract_x(const float *acc_s, int AS, const flo
     }
 }
 
+// contract_x with the first step's work split as (x band, weight index, half
+// of the components) per thread: one band lookup and one weight read per
+// column for T / 2 sums (contract_x: index arithmetic, a band lookup and a
+// weight read per sum).  Every sum has contract_x's operands and order, so
+// the results are the same.
+template <int DIM>
+__device__ __forceinline__ void contract_x_halves(const float *acc_s, int AS, const float4 *xw_s, const int *cl_s,
+                                                  float *S_s, int ncx, int cx, float *__restrict__ Pout)
+{
+    constexpr int T = 4 * DIM, TH = T / 2;
+    static_assert(T % 2 == 0, "two halves");
+    for (int it = threadIdx.x; it < ncx * 8; it += kBlock) {
+        const int cell = it >> 3, a = (it >> 1) & 3, h = it & 1;
+        const int xb = cl_s[cell], xe = cl_s[cell + 1];
+        float sum[TH];
+#pragma unroll
+        for (int m = 0; m < TH; ++m)
+            sum[m] = 0.0f;
+        for (int xx = xb; xx < xe; ++xx) {
+            const float w = ((const float *)&xw_s[xx])[a];
+            const float *row = acc_s + AS * xx + h * TH;
+#pragma unroll
+            for (int m = 0; m < TH; ++m)
+                sum[m] = fmaf(w, row[m], sum[m]);
+        }
+        float *dst = S_s + (cell * 4 + a) * T + h * TH;
+#pragma unroll
+        for (int m = 0; m < TH; ++m)
+            dst[m] = sum[m];
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < T * cx; o += kBlock) {
+        const int comp = o % DIM;
+        const int q = o / DIM;
+        const int kx = q % cx;
+        const int tc = (q / cx) * DIM + comp;
+        float sum = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int cell = kx - a;
+            if (cell >= 0 && cell < ncx)
+                sum += S_s[(cell * 4 + a) * T + tc];
+        }
+        Pout[o] = sum;
+    }
+}
+
 // Stage 1, 3D: block = (z slice, y knot band).  Each thread owns x columns;
 // rows of the band are walked with the y neighbours rotated through registers
 // (reference mode).  Force f = grad * d with the reference's central
@@ -1766,7 +1813,7 @@ __global__ void __launch_bounds__(kBlock, 4)
             for (int m = 0; m < 12; ++m)
                 acc_s[AS * x + m] = a48[12 * k + m];
         __syncthreads();
-        contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
+        contract_x_halves<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
     }
 }
 
@@ -2053,20 +2100,23 @@ __global__ void __launch_bounds__(kBlock)
     const InstDesc d = inst[i];
     if (!d.zcell)
         return;
-    const long long ncoef = (long long)d.cx * d.cy * d.cz;
-    const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
-    if (e3 >= 3 * ncoef)
+    // 32-bit index arithmetic: the set-up keeps images below 2^29 voxels, and
+    // the coefficient grid is smaller than the image
+    const int ncoef3 = 3 * d.cx * d.cy * d.cz;
+    const int e3 = (b - prefix[i]) * kBlock + threadIdx.x;
+    if (e3 >= ncoef3)
         return;
     double sc;
     if constexpr (FUSED)
         sc = -weight * (norm ? 1.0 / ((double)d.nx * d.ny * d.nz) : 1.0);
     else
         sc = gscale[i];
-    const int comp = (int)(e3 % 3);
-    const long long e = e3 / 3;
-    const int kx = (int)(e % d.cx);
-    const int ky = (int)((e / d.cx) % d.cy);
-    const int kz = (int)(e / ((long long)d.cx * d.cy));
+    const unsigned ue = (unsigned)e3 / 3u;
+    const int comp = e3 - 3 * (int)ue;
+    const unsigned row = ue / (unsigned)d.cx;
+    const int kx = (int)(ue - row * (unsigned)d.cx);
+    const int kz = (int)(row / (unsigned)d.cy);
+    const int ky = (int)(row - (unsigned)kz * (unsigned)d.cy);
     const float *Pi = P + d.part_off + 3 * kx + comp;
     const long long cell_stride = 48LL * d.cx;
     double g = 0.0;

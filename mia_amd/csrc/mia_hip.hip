@@ -673,7 +673,11 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         // reference-gradient stage 1 per (z knot cell, y band): one x column
         // per thread and the band's rows in registers (decided per instance,
         // so that an instance computes the same in any batch)
-        d.zcell = dim == 3 && d.nx <= kBlock && band_rows + 2 <= kS1zRows ? 1 : 0;
+        // (its reduce indexes the gradient with 32-bit arithmetic)
+        d.zcell = dim == 3 && d.nx <= kBlock && band_rows + 2 <= kS1zRows &&
+                          3LL * d.cx * d.cy * d.cz < (1LL << 31)
+                      ? 1
+                      : 0;
         n_zcell += d.zcell;
         AxisTable *axes[3] = {&ax, &ay, &az};
         int *toff[3] = {&d.tab_x, &d.tab_y, &d.tab_z};
