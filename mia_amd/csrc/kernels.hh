@@ -1363,47 +1363,11 @@ constexpr int kRowUnroll = MODE == 0 ? 4 : 1;
 // x contraction of the per-column band sums into the block's partial-sum row:
 // P[(t * cx + kx) * DIM + comp] = sum_x wx_{kx - start(x)}(x) * acc[x][t*DIM + comp].
 // Two fixed-order steps: for each x band c and weight index a,
-// S[c][a][tc] = sum over the band's columns in ascending x; then
-// P = sum_{a = 0..3} S[kx - a][a][tc].
+// S[c][a][tc] = sum over the band's columns in ascending x (one thread per
+// band, weight index and half of the tc: one band lookup and one weight read
+// per column for T / 2 sums); then P = sum_{a = 0..3} S[kx - a][a][tc].
 template <int DIM>
 __device__ __forceinline__ void contract_x(const float *acc_s, int AS, const float4 *xw_s, const int *cl_s,
-                                           float *S_s, int ncx, int cx, float *__restrict__ Pout)
-{
-    constexpr int T = 4 * DIM;
-    for (int o2 = threadIdx.x; o2 < ncx * 4 * T; o2 += kBlock) {
-        const int cell = o2 / (4 * T);
-        const int r = o2 - cell * 4 * T;
-        const int a = r / T, tc = r - a * T;
-        const int xb = cl_s[cell], xe = cl_s[cell + 1];
-        float sum = 0.0f;
-        for (int xx = xb; xx < xe; ++xx)
-            sum = fmaf(((const float *)&xw_s[xx])[a], acc_s[AS * xx + tc], sum);
-        S_s[o2] = sum;
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < T * cx; o += kBlock) {
-        const int comp = o % DIM;
-        const int q = o / DIM;
-        const int kx = q % cx;
-        const int tc = (q / cx) * DIM + comp;
-        float sum = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int cell = kx - a;
-            if (cell >= 0 && cell < ncx)
-                sum += S_s[(cell * 4 + a) * T + tc];
-        }
-        Pout[o] = sum;
-    }
-}
-
-// contract_x with the first step's work split as (x band, weight index, half
-// of the components) per thread: one band lookup and one weight read per
-// column for T / 2 sums (contract_x: index arithmetic, a band lookup and a
-// weight read per sum).  Every sum has contract_x's operands and order, so
-// the results are the same.
-template <int DIM>
-__device__ __forceinline__ void contract_x_halves(const float *acc_s, int AS, const float4 *xw_s, const int *cl_s,
                                                   float *S_s, int ncx, int cx, float *__restrict__ Pout)
 {
     constexpr int T = 4 * DIM, TH = T / 2;
@@ -1813,7 +1777,7 @@ __global__ void __launch_bounds__(kBlock, 4)
             for (int m = 0; m < 12; ++m)
                 acc_s[AS * x + m] = a48[12 * k + m];
         __syncthreads();
-        contract_x_halves<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
+        contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
     }
 }
 
