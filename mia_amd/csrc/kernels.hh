@@ -163,6 +163,14 @@ __device__ __forceinline__ float clamp_pos_nn(float p)
     return r;
 }
 
+// e / d for 0 <= e < 2^21 and d >= 1, with inv = 1.0f / d: (e + 0.5) / d is at
+// least 0.5 / d away from an integer, more than its float error (two roundings,
+// at most about 2^-23 (e + 0.5) / d < 0.25 / d). Callers pass e < 4096.
+__device__ __forceinline__ int small_div(int e, float inv)
+{
+    return (int)(((float)e + 0.5f) * inv);
+}
+
 // mirror boundary index map (mia/core/splinebc/bc.cc:52-64)
 __device__ __forceinline__ int mirror_idx(int i, int n)
 {
@@ -429,22 +437,31 @@ __device__ __forceinline__ void fill_box3(float *box, const float *__restrict__ 
 __device__ __forceinline__ void fill_box2(float *box, const float *__restrict__ img, int nx, int ny, int rx0,
                                           int ry0, int RX, int RY)
 {
+    // eight loads in flight per thread before their LDS stores (a load and
+    // its store per iteration would wait out the memory latency each time);
+    // vol <= kW2Region, so small_div is exact
+    constexpr int U = 8;
     const int vol = RX * RY;
-    const int stx = kBlock % RX, sty = kBlock / RX;
-    int ix = threadIdx.x % RX;
-    int iy = threadIdx.x / RX;
+    const float irx = 1.0f / (float)RX;
     const bool xin = rx0 >= 0 && rx0 + RX <= nx;
     const bool yin = ry0 >= 0 && ry0 + RY <= ny;
-    for (int e = threadIdx.x; e < vol; e += kBlock) {
-        const int gx = xin ? rx0 + ix : mirror_idx(rx0 + ix, nx);
-        const int gy = yin ? ry0 + iy : mirror_idx(ry0 + iy, ny);
-        box[e] = img[(long long)gy * nx + gx];
-        ix += stx;
-        iy += sty;
-        if (ix >= RX) {
-            ix -= RX;
-            ++iy;
+    for (int e0 = threadIdx.x; e0 < vol; e0 += U * kBlock) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * kBlock;
+            v[u] = 0.0f;
+            if (e < vol) {
+                const int iy = small_div(e, irx), ix = e - iy * RX;
+                const int gx = xin ? rx0 + ix : mirror_idx(rx0 + ix, nx);
+                const int gy = yin ? ry0 + iy : mirror_idx(ry0 + iy, ny);
+                v[u] = img[(long long)gy * nx + gx];
+            }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (e0 + u * kBlock < vol)
+                box[e0 + u * kBlock] = v[u];
     }
 }
 
@@ -754,13 +771,6 @@ __device__ __forceinline__ void dma_dword(__amdgpu_buffer_rsrc_t r, void *lds, u
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 4, voff, soff, 0, 0);
 }
 
-// e / d for 0 <= e < 2^21 and d >= 1, with inv = 1.0f / d: (e + 0.5) / d is at
-// least 0.5 / d away from an integer, more than its float error (two roundings,
-// at most about 2^-23 (e + 0.5) / d < 0.25 / d). Callers pass e < 4096.
-__device__ __forceinline__ int small_div(int e, float inv)
-{
-    return (int)(((float)e + 0.5f) * inv);
-}
 
 // The NS^3 taps of one voxel from an LDS box with compile-time pitches
 // (o0: byte offset of the first tap), z -> y -> x contraction as
@@ -1199,6 +1209,8 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ float2 win[kW2Win];
     __shared__ float region[kW2Region];
     __shared__ float red[4][kBlock / 64];
+    __shared__ float4 wy_s[kW2TY];
+    __shared__ int sy_s[kW2TY];
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
     const int gi = find_inst(prefix, n_inst, b);
     const int i = ids[gi];
@@ -1214,6 +1226,12 @@ __global__ void __launch_bounds__(kBlock)
     const int wxn = ts[d.tab_x + min(x0 + kW2TX, d.nx) - 1] + 4 - kx0;
     const int wyn = ts[d.tab_y + min(y0 + kW2TY, d.ny) - 1] + 4 - ky0;
     const double *P = params + d.par_off;
+    // the tile's y tables (rows clamped to the image) once into LDS
+    if (threadIdx.x < kW2TY) {
+        const int yy = min(y0 + (int)threadIdx.x, d.ny - 1);
+        wy_s[threadIdx.x] = tw[d.tab_y + yy];
+        sy_s[threadIdx.x] = ts[d.tab_y + yy] - ky0;
+    }
     for (int e = threadIdx.x; e < wxn * wyn; e += kBlock) {
         const int ky = ky0 + e / wxn, kx = kx0 + e % wxn;
         win[e] = (kx < d.cx && ky < d.cy) ? coef2(P, (long long)ky * d.cx + kx) : make_float2(0.0f, 0.0f);
@@ -1226,10 +1244,10 @@ __global__ void __launch_bounds__(kBlock)
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-        const int y = y0 + ly + k * (kBlock / kW2TX);
-        const int yc = min(y, d.ny - 1);
-        const int sy = ts[d.tab_y + yc] - ky0;
-        const float4 wyv = tw[d.tab_y + yc];
+        const int yl = ly + k * (kBlock / kW2TX);
+        const int y = y0 + yl;
+        const int sy = sy_s[yl];
+        const float4 wyv = wy_s[yl];
         const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
         float ux = 0.0f, uy = 0.0f;
 #pragma unroll

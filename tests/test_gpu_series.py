@@ -124,6 +124,31 @@ def test_device_optimizer_batched_equals_per_frame(dim, n, side):
     assert np.all(eb <= 30)
 
 
+def test_device_optimizer_both_search_paths():
+    """Frames above and below the register-resident limit of the device
+    L-BFGS direction search (2048 parameters) in one batch: bitwise equal to
+    one context per frame, and close to the host L-BFGS."""
+    from mia_amd import series, synth
+    frames = synth.series_frames_2d(2, (60, 80), 8.0, 4.0, 31) + synth.series_frames_2d(2, (300, 330), 8.0, 4.0, 32)
+    sizes = [2 * mia_amd_csize(f[1].shape) for f in frames]
+    assert min(sizes) <= 2048 < max(sizes)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        pb, cb, eb, _ = s.run("lbfgs:maxiter=25", series.DEVICE_BATCHED)
+        pf, cf, ef, _ = s.run("lbfgs:maxiter=25", series.DEVICE_PER_FRAME)
+        ph, ch, eh, _ = s.run("lbfgs:maxiter=6", series.BATCHED)
+        pd, cd, ed, _ = s.run("lbfgs:maxiter=6", series.DEVICE_BATCHED)
+    assert np.array_equal(pb, pf)
+    assert np.array_equal(cb, cf)
+    assert np.array_equal(eb, ef)
+    assert np.array_equal(eh, ed)
+    np.testing.assert_allclose(cd, ch, rtol=1e-9)
+
+
+def mia_amd_csize(shape):
+    import mia_amd
+    return mia_amd.spline_csize(shape[1], 8.0, 2) * mia_amd.spline_csize(shape[0], 8.0, 2)
+
+
 def test_device_optimizer_follows_host_lbfgs():
     """Same algorithm on the device and on the host (different summation
     order of the dot products): the first iterations agree tightly and the
