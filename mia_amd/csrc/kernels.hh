@@ -1941,9 +1941,22 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
     const int beg = z ? si.zoff + si.s1z[i] : si.s1[i], end = z ? si.zoff + si.s1z[i + 1] : si.s1[i + 1];
     double s = 0.0;
     unsigned c = 0;
-    for (int k = beg + threadIdx.x; k < end; k += kBlock) {
-        s += ssd_part[k];
-        c += cnt_part[k];
+    // four partials per thread in flight before they are added (same order;
+    // a missing one adds 0)
+    for (int k0 = beg + threadIdx.x; k0 < end; k0 += 4 * kBlock) {
+        double a[4];
+        unsigned n[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u * kBlock;
+            a[u] = k < end ? ssd_part[k] : 0.0;
+            n[u] = k < end ? cnt_part[k] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s += a[u];
+            c += n[u];
+        }
     }
     s = block_sum(s, red);
     c = block_sum_u(c, redu);
@@ -2044,17 +2057,31 @@ __global__ void __launch_bounds__(kBlock)
         const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
         if (e3 >= 2 * ncoef)
             return;
-        const int comp = (int)(e3 % 2);
-        const long long e = e3 / 2;
-        const int kx = (int)(e % d.cx);
-        const int ky = (int)(e / d.cx);
-        double g = 0.0;
+        int comp, kx, ky;
+        if (2 * ncoef < (1LL << 31)) { // 32-bit index arithmetic (block-uniform)
+            const unsigned ue = (unsigned)e3 >> 1;
+            comp = (int)(e3 & 1);
+            ky = (int)(ue / (unsigned)d.cx);
+            kx = (int)(ue - (unsigned)ky * (unsigned)d.cx);
+        } else {
+            comp = (int)(e3 % 2);
+            const long long e = e3 / 2;
+            kx = (int)(e % d.cx);
+            ky = (int)(e / d.cx);
+        }
+        // the four partial sums in flight first (a missing one reads as 0),
+        // then the sum in the fixed t order
+        float v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int band = ky - t;
-            if (band >= 0 && band < d.ncy)
-                g += P[d.part_off + (long long)band * 8 * d.cx + (t * d.cx + kx) * 2 + comp];
+            v[t] = band >= 0 && band < d.ncy ? P[d.part_off + (long long)band * 8 * d.cx + (t * d.cx + kx) * 2 + comp]
+                                             : 0.0f;
         }
+        double g = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            g += v[t];
         grad[d.par_off + e3] = sc * g;
     }
 }
@@ -2101,19 +2128,23 @@ __global__ void __launch_bounds__(kBlock)
     const int ky = (int)(row - (unsigned)kz * (unsigned)d.cy);
     const float *Pi = P + d.part_off + 3 * kx + comp;
     const long long cell_stride = 48LL * d.cx;
-    double g = 0.0;
+    // all 16 partial sums in flight first (a missing one reads as 0), then
+    // the sum in the fixed (k, t) order
+    float v[16];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int cz = kz - k;
-        if (cz < 0 || cz >= d.ncz)
-            continue;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int band = ky - t;
-            if (band >= 0 && band < d.ncy)
-                g += (double)Pi[((long long)cz * d.ncy + band) * cell_stride + (4 * k + t) * 3LL * d.cx];
+            const bool in = cz >= 0 && cz < d.ncz && band >= 0 && band < d.ncy;
+            v[4 * k + t] = in ? Pi[((long long)cz * d.ncy + band) * cell_stride + (4 * k + t) * 3LL * d.cx] : 0.0f;
         }
     }
+    double g = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        g += (double)v[q];
     grad[d.par_off + e3] = sc * g;
 }
 
