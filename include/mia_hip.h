@@ -65,7 +65,9 @@ int mia_hip_ctx_destroy(mia_hip_ctx *ctx);
  * kernel, 3 (`imgkernel=bspline:d=3`, default) or 1 (`bspline:d=1`); boundary
  * is mirror (mia/template/transformfactory.cxx:58-59).  Replaces the grid
  * set-up of C?DSplineTransformation::reinit (3D spline.cc:224-275, 2D
- * :170-203).  Allocates all device buffers. */
+ * :170-203).  Allocates all device buffers.  MIA_HIP_ERR_LIMIT for an x size
+ * beyond the row limit or a 3D image of 2^29 voxels or more (the kernels
+ * address an image with 32-bit byte offsets). */
 int mia_hip_setup(mia_hip_ctx *ctx, int dim, int n_inst, const uint32_t *sizes,
                   const uint32_t *csizes, int degree);
 
