@@ -470,7 +470,10 @@ def main():
                 "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "traffic_unit": "bytes per launch",
                 "alg_bytes": kd["alg_bytes"],
-                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None}
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+                # what bounds the 3D warp instead: LDS gathers and VALU issue (SQ counters)
+                "limiter": "lds+valu" if dom == "warp" else None,
+                "counters_source": "profiles/kernels_cost_grad_3d_256cube_rate8_counters.md"}
     tsum = sum(v["avg_us"] for v in kernels.values())
     log("kernels: %s" % json.dumps(kernels))
 
