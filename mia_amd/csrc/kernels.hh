@@ -15,6 +15,10 @@
 //               registers and with the x weights through LDS -> partial sums
 //               P[z][band][t][kx][c] on the coefficient grid.  In exact mode it
 //               computes the warp and the analytic image gradient inline.
+//               3D reference mode (rows up to 256 voxels): k_stage1z_3d per
+//               (z knot cell, y band) also folds the cell's slices with their z
+//               weights -> P[cell][band][k][t][kx][c], read once by
+//               k_reducez_3d.
 //   k_finalize  (only with autothresh) per instance: SSD sum, mask count, cost
 //               and gradient scale.
 //   k_reduce    per coefficient component: fixed-order FP64 sum over the four y
@@ -880,18 +884,20 @@ __device__ __forceinline__ void fill_box_rows(float *box, const float *__restric
     }
 }
 
-// 3D warp.  Block = 32 x 8 x 8 voxels, thread = one (x, y) column of 8 voxels.
+// 3D warp.  Block = TX x TY x TZ voxels (32 x 8 x 8 by default), thread = one
+// (x, y) column of TZ voxels.
 //   phase 1: the tile's coefficient window (double params -> float, zero
 //            beyond the grid) and the tile's x/y/z spline tables into LDS;
-//   phase 2: displacement, contracted along y (T1[kz][y][kx]), then x
-//            (T2[kz][y][x]) in LDS, then z per voxel; when the intermediates
-//            do not fit, each thread contracts its column along z directly;
+//   phase 2: displacement, contracted along y in LDS (T1[kz][y][kx]), then
+//            along x into the thread's registers (one value per z knot), then
+//            along z per voxel; when the window does not fit, each thread
+//            contracts its column directly from global memory;
 //   phase 3: the bounding box of all interpolation taps of the tile's voxels
 //            (block min/max of the sample positions) is loaded into LDS with
 //            mirror indices applied on the load;
 //   phase 4: every voxel reads its 4^3 taps from LDS at one base address plus
 //            compile-time offsets, clamps and stores W.  A tile whose box
-//            exceeds the 40 x 16 x 15 pitches gathers from global memory with
+//            exceeds the PX x PY x PZ pitches gathers from global memory with
 //            the same arithmetic.
 template <int DEG, class TL>
 __global__ void __launch_bounds__(kBlock)
