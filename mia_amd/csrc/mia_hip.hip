@@ -157,7 +157,7 @@ struct mia_hip_ctx {
     // on-device optimizer state (allocated on first use, sized for `opt_memory`)
     int opt_memory = 0;
     double *d_oX = nullptr, *d_oG = nullptr, *d_oD = nullptr, *d_oBX = nullptr, *d_oS = nullptr, *d_oY = nullptr,
-           *d_oRHO = nullptr, *d_obest = nullptr;
+           *d_oRHO = nullptr, *d_obest = nullptr, *d_oSY = nullptr, *d_oYY = nullptr;
     int *d_odone = nullptr, *h_done = nullptr;
     OptState *d_ost = nullptr;
     hipGraph_t graph = nullptr;
@@ -182,7 +182,8 @@ void free_buffers(mia_hip_ctx *c)
                     c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
-                    c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
+                    c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
+                    c->d_oYY};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -195,6 +196,7 @@ void free_buffers(mia_hip_ctx *c)
     if (c->h_done)
         (void)hipHostFree(c->h_done);
     c->d_oX = c->d_oG = c->d_oD = c->d_oBX = c->d_oS = c->d_oY = c->d_oRHO = c->d_obest = nullptr;
+    c->d_oSY = c->d_oYY = nullptr;
     c->d_odone = c->h_done = nullptr;
     c->d_ost = nullptr;
     c->opt_memory = 0;
@@ -1047,7 +1049,7 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     const int n = c->n_inst;
     if (c->opt_memory != memory) {
         void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone,
-                       c->d_ost};
+                       c->d_ost, c->d_oSY, c->d_oYY};
         for (void *p : old)
             if (p)
                 (void)hipFree(p);
@@ -1059,7 +1061,9 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
             (st = dev_alloc(&c->d_oD, c->n_par)) || (st = dev_alloc(&c->d_oBX, c->n_par)) ||
             (st = dev_alloc(&c->d_oS, (size_t)memory * c->n_par)) ||
             (st = dev_alloc(&c->d_oY, (size_t)memory * c->n_par)) || (st = dev_alloc(&c->d_oRHO, memory * n)) ||
-            (st = dev_alloc(&c->d_obest, n)) || (st = dev_alloc(&c->d_odone, n)) || (st = dev_alloc(&c->d_ost, n)))
+            (st = dev_alloc(&c->d_obest, n)) || (st = dev_alloc(&c->d_odone, n)) || (st = dev_alloc(&c->d_ost, n)) ||
+            (st = dev_alloc(&c->d_oSY, (size_t)memory * memory * n)) ||
+            (st = dev_alloc(&c->d_oYY, (size_t)memory * memory * n)))
             return st;
         HIP_TRY(hipHostMalloc((void **)&c->h_done, sizeof(int) * n, hipHostMallocDefault));
         c->opt_memory = memory;
@@ -1075,7 +1079,8 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     HIP_TRY(hipMemcpyAsync(c->d_ost, st0.data(), sizeof(OptState) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_odone, done0.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    OptBuffers ob{c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost};
+    OptBuffers ob{c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost,
+                  c->d_oSY, c->d_oYY};
     // capture iters_per_graph iterations + the done flags read-back
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
