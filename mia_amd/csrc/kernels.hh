@@ -84,8 +84,8 @@ __host__ __device__ inline void w3_tile_dims(int v, int &tx, int &ty, int &tz)
 }
 // 2D warp tile and LDS budgets
 constexpr int kW2TX = 64, kW2TY = 32;
-constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2
-constexpr int kW2Region = 8192;                     // floats
+constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2, largest coefficient window (rate 1)
+constexpr int kW2Region = 6144;                     // floats
 
 // Per-instance description, identical on host and device.
 struct InstDesc {
@@ -1212,8 +1212,12 @@ __global__ void __launch_bounds__(kBlock)
 {
     constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
     constexpr int RPT = kW2TY / (kBlock / kW2TX);   // rows per thread (8)
-    __shared__ float2 win[kW2Win];
-    __shared__ float region[kW2Region];
+    // dynamic LDS: the image box (kW2Region floats), then the coefficient
+    // window, sized by the host for the launch's largest window (about 1 KB
+    // at rate 8, so that six blocks fit a CU)
+    extern __shared__ float smem[];
+    float *region = smem;
+    float2 *win = (float2 *)(smem + kW2Region);
     __shared__ float red[4][kBlock / 64];
     __shared__ float4 wy_s[kW2TY];
     __shared__ int sy_s[kW2TY];
@@ -1243,6 +1247,27 @@ __global__ void __launch_bounds__(kBlock)
         win[e] = (kx < d.cx && ky < d.cy) ? coef2(P, (long long)ky * d.cx + kx) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
+    // displacement contracted along y once per tile row and knot column
+    // (into the box memory, which is free until the box fill)
+    float2 *vy = (float2 *)region;
+    {
+        const float iwx = 1.0f / (float)wxn;
+        for (int e = threadIdx.x; e < kW2TY * wxn; e += kBlock) {
+            const int yl = small_div(e, iwx), kx = e - yl * wxn;
+            const float4 wyv = wy_s[yl];
+            const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+            const float2 *c = win + sy_s[yl] * wxn + kx;
+            float vx = 0.0f, vyy = 0.0f;
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+                const float2 q = c[bb * wxn];
+                vx = fmaf(wys[bb], q.x, vx);
+                vyy = fmaf(wys[bb], q.y, vyy);
+            }
+            vy[e] = make_float2(vx, vyy);
+        }
+    }
+    __syncthreads();
     const int sx = ts[d.tab_x + xc] - kx0;
     const float4 wxv = tw[d.tab_x + xc];
     const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
@@ -1252,22 +1277,13 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < RPT; ++k) {
         const int yl = ly + k * (kBlock / kW2TX);
         const int y = y0 + yl;
-        const int sy = sy_s[yl];
-        const float4 wyv = wy_s[yl];
-        const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+        const float2 *wr = vy + yl * wxn + sx;
         float ux = 0.0f, uy = 0.0f;
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            float hx = 0.0f, hy = 0.0f;
-            const float2 *wr = win + (sy + bb) * wxn + sx;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const float2 q = wr[a];
-                hx = fmaf(wxs[a], q.x, hx);
-                hy = fmaf(wxs[a], q.y, hy);
-            }
-            ux = fmaf(wys[bb], hx, ux);
-            uy = fmaf(wys[bb], hy, uy);
+        for (int a = 0; a < 4; ++a) {
+            const float2 q = wr[a];
+            ux = fmaf(wxs[a], q.x, ux);
+            uy = fmaf(wxs[a], q.y, uy);
         }
         px[k] = clamp_pos((float)x - ux);
         py[k] = clamp_pos((float)y - uy);

@@ -141,6 +141,7 @@ struct mia_hip_ctx {
     std::vector<char> images_set;
     long long n_img = 0, n_par = 0, n_part = 0, max_vox = 0;
     int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0;
+    int w2_win = 0; // 2D warp: largest coefficient window of a 64 x 32 tile (float2)
 
     InstDesc *d_inst = nullptr;
     int *d_wgroups = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr, *d_s1z_prefix = nullptr,
@@ -395,7 +396,8 @@ int launch_warp(mia_hip_ctx *c, const double *d_params, int g, int boff, int nb,
                c->d_W);
     } else {
         auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
+        const size_t lds = (size_t)kW2Region * sizeof(float) + (size_t)c->w2_win * sizeof(float2);
+        launch(c, ST_WARP, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
                (const float4 *)c->d_tw, (const int *)c->d_ts, (const float *)c->d_M, (const float2 *)c->d_clamp,
                c->d_W);
     }
@@ -633,7 +635,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     std::vector<float> tw;
     std::vector<int> ts, cells;
     long long img = 0, par = 0, part = 0, maxv = 0;
-    int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0, w2_win = 0;
     for (int i = 0; i < n_inst; ++i) {
         InstDesc &d = c->inst[i];
         d.nx = (int)sizes[3 * i];
@@ -704,6 +706,16 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                          : (long long)d.ncy * 8 * d.cx;
         maxv = std::max(maxv, nvox);
         max_nx = std::max(max_nx, d.nx);
+        if (dim == 2) {
+            // knots spanned by the tiles of k_warp2d (the window it loads)
+            auto span = [](const AxisTable &a, int n, int t) {
+                int m = 0;
+                for (int p0 = 0; p0 < n; p0 += t)
+                    m = std::max(m, a.start[std::min(p0 + t, n) - 1] + 4 - a.start[p0]);
+                return m;
+            };
+            w2_win = std::max(w2_win, span(ax, d.nx, kW2TX) * span(ay, d.ny, kW2TY));
+        }
         max_cx = std::max(max_cx, d.cx);
         max_ncx = std::max(max_ncx, d.ncx);
         c->s1_prefix[i + 1] = c->s1_prefix[i] + (dim == 3 ? d.nz * d.ncy : d.ncy);
@@ -764,6 +776,9 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->n_part = part;
     c->max_vox = maxv;
     c->max_nx = max_nx;
+    if (w2_win > kW2Win)
+        return fail(MIA_HIP_ERR_ARG, "2D coefficient window exceeds its bound (spline rate below 1?)");
+    c->w2_win = w2_win;
     c->max_cx = max_cx;
     c->max_ncx = max_ncx;
     c->n_zcell = n_zcell;
