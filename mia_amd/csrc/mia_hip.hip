@@ -1104,7 +1104,7 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
         rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
         if (rc == MIA_HIP_OK) {
-            hipLaunchKernelGGL(k_lbfgs_step, dim3(n), dim3(kBlock), 0, c->stream, c->d_inst, c->dim, ob, c->d_params,
+            hipLaunchKernelGGL(k_lbfgs_step, dim3(n), dim3(kOptBlock), 0, c->stream, c->d_inst, c->dim, ob, c->d_params,
                                c->d_grad, c->d_costs, memory, maxeval, first_step);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess)

@@ -15,6 +15,9 @@
 namespace mia_hip {
 
 constexpr int kMaxMemory = 16;
+// threads per instance: eight waves, so that a CU keeps twice the loads of the
+// pair history in flight that four waves would
+constexpr int kOptBlock = 512;
 
 struct OptState {
     double f, gd, alpha, best_f;
@@ -58,14 +61,14 @@ template <bool MAX>
 __device__ __forceinline__ double block_reduce_all(double v, double *red, int &par)
 {
     v = wave_reduce63<MAX>(v);
-    double *r = red + par * (kBlock / 64);
+    double *r = red + par * (kOptBlock / 64);
     par ^= 1;
     if ((threadIdx.x & 63) == 63)
         r[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = r[0];
 #pragma unroll
-    for (int k = 1; k < kBlock / 64; ++k)
+    for (int k = 1; k < kOptBlock / 64; ++k)
         s = MAX ? fmax(s, r[k]) : s + r[k];
     return s;
 }
@@ -93,17 +96,17 @@ struct OptBuffers {
 };
 
 // Instances of up to kLbRegN parameters keep the block's vectors in registers
-// (element e = thread + k * kBlock, k < kLbK) and compute the direction in the
+// (element e = thread + k * kOptBlock, k < kLbK) and compute the direction in the
 // compact form below; larger ones run the two-loop recursion over global
 // memory in k_lbfgs_step.
-constexpr int kLbK = 8;
-constexpr long long kLbRegN = (long long)kLbK * kBlock;
+constexpr int kLbK = 4;
+constexpr long long kLbRegN = (long long)kLbK * kOptBlock;
 
 __device__ __forceinline__ void lb_load(double *v, const double *__restrict__ src, long long n)
 {
 #pragma unroll
     for (int k = 0; k < kLbK; ++k) {
-        const long long e = threadIdx.x + (long long)k * kBlock;
+        const long long e = threadIdx.x + (long long)k * kOptBlock;
         v[k] = e < n ? src[e] : 0.0;
     }
 }
@@ -126,7 +129,7 @@ __device__ __forceinline__ double part_total(const double (*part)[kPart], int v)
 {
     double t = part[0][v];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w)
+    for (int w = 1; w < kOptBlock / 64; ++w)
         t += part[w][v];
     return t;
 }
@@ -135,7 +138,7 @@ __device__ __forceinline__ double part_max(const double (*part)[kPart], int v)
 {
     double t = part[0][v];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w)
+    for (int w = 1; w < kOptBlock / 64; ++w)
         t = fmax(t, part[w][v]);
     return t;
 }
@@ -149,13 +152,13 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 
 // Pairs are read kLbP at a time, all loads of a group issued before any use,
 // so that a pass over k pairs waits about k / kLbP memory latencies.
-constexpr int kLbP = 4;
+constexpr int kLbP = 8;
 
 __device__ __forceinline__ void lb_store(double *dst, const double *v, long long n)
 {
 #pragma unroll
     for (int k = 0; k < kLbK; ++k) {
-        const long long e = threadIdx.x + (long long)k * kBlock;
+        const long long e = threadIdx.x + (long long)k * kOptBlock;
         if (e < n)
             dst[e] = v[k];
     }
@@ -236,7 +239,7 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
     for (int j0 = 0; j0 < kk; j0 += kLbP) {
         double sc[kLbP][kLbK], yc[kLbP][kLbK];
         lb_load_group(sc, yc, j0, kk, head, memory, S, Y, n);
-        double dd[16];
+        double dd[4 * kLbP];
 #pragma unroll
         for (int t = 0; t < kLbP; ++t) {
             double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
@@ -252,10 +255,13 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
             dd[4 * t + 2] = d2;
             dd[4 * t + 3] = d3;
         }
-        const double t = transpose_sum16(dd, tb);
-        const int j = j0 + (lane >> 4);
-        if ((lane & 3) == 3 && j < kk)
-            part[w][kPPair + 4 * j0 + (lane >> 2)] = t;
+#pragma unroll
+        for (int h = 0; h < kLbP / 4; ++h) {
+            const double t = transpose_sum16(dd + 16 * h, tb);
+            const int j = j0 + 4 * h + (lane >> 4);
+            if ((lane & 3) == 3 && j < kk)
+                part[w][kPPair + 4 * (j0 + 4 * h) + (lane >> 2)] = t;
+        }
     }
 }
 
@@ -343,7 +349,7 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
 #pragma unroll
     for (int k2 = 0; k2 < kLbK; ++k2) {
         dv[k2] = -dv[k2];
-        const long long e = threadIdx.x + (long long)k2 * kBlock;
+        const long long e = threadIdx.x + (long long)k2 * kOptBlock;
         if (e < n)
             gd += g[k2] * dv[k2];
     }
@@ -351,17 +357,17 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
     return block_sum_all(gd, red, rpar);
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kOptBlock)
     k_lbfgs_step(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
                  const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
                  double first_step)
 {
-    __shared__ double red[2 * (kBlock / 64)];
+    __shared__ double red[2 * (kOptBlock / 64)];
     __shared__ double alph[kMaxMemory];
-    __shared__ double part[kBlock / 64][kPart];
+    __shared__ double part[kOptBlock / 64][kPart];
     __shared__ double mSY[kMaxMemory * kMaxMemory], mYY[kMaxMemory * kMaxMemory];
     __shared__ double cu[kMaxMemory], cw[kMaxMemory];
-    __shared__ double tbuf[(kBlock / 64) * 16 * kTbRow];
+    __shared__ double tbuf[(kOptBlock / 64) * 16 * kTbRow];
     int rpar = 0; // which half of red the next block reduction uses
     __shared__ OptState s;
     const int i = blockIdx.x;
@@ -397,7 +403,7 @@ __global__ void __launch_bounds__(kBlock)
     bool finish = false;
     bool backtrack = false;
     if (s.phase == 0) {
-        for (long long e = threadIdx.x; e < n; e += kBlock) {
+        for (long long e = threadIdx.x; e < n; e += kOptBlock) {
             X[e] = XT[e];
             G[e] = GT[e];
             BX[e] = XT[e];
@@ -411,7 +417,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         __syncthreads();
         double gmax = 0.0;
-        for (long long e = threadIdx.x; e < n; e += kBlock)
+        for (long long e = threadIdx.x; e < n; e += kOptBlock)
             gmax = fmax(gmax, fabs(GT[e]));
         gmax = block_max_all(gmax, red, rpar);
         if (!finite || gmax <= 0.0 || s.evals >= maxeval)
@@ -426,7 +432,7 @@ __global__ void __launch_bounds__(kBlock)
         if (better && regs) {
             lb_store(BX, rxt, n);
         } else if (better) {
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 BX[e] = XT[e];
         }
         __syncthreads();
@@ -438,7 +444,7 @@ __global__ void __launch_bounds__(kBlock)
         __syncthreads();
         const bool armijo = finite && ft <= f0 + 1e-4 * a0 * gd0;
         if (armijo && regs) {
-            for (int t = threadIdx.x; t < memory * memory; t += kBlock) {
+            for (int t = threadIdx.x; t < memory * memory; t += kOptBlock) {
                 mSY[t] = SYi[t];
                 mYY[t] = YYi[t];
             }
@@ -455,7 +461,7 @@ __global__ void __launch_bounds__(kBlock)
                 double *Sj = S + (long long)ns * n, *Yj = Y + (long long)ns * n;
 #pragma unroll
                 for (int k = 0; k < kLbK; ++k) {
-                    const long long e = threadIdx.x + (long long)k * kBlock;
+                    const long long e = threadIdx.x + (long long)k * kOptBlock;
                     if (e < n) {
                         Sj[e] = sv[k];
                         Yj[e] = yv[k];
@@ -489,7 +495,7 @@ __global__ void __launch_bounds__(kBlock)
                 search = true;
         } else if (armijo) {
             double sy = 0.0, yy = 0.0, ss = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock) {
+            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
                 const double se = XT[e] - X[e], ye = GT[e] - G[e];
                 sy += se * ye;
                 yy += ye * ye;
@@ -501,7 +507,7 @@ __global__ void __launch_bounds__(kBlock)
             if (sy > 1e-12 * sqrt(ss * yy)) {
                 const int slot = s.k < memory ? (s.head + s.k) % memory : s.head;
                 double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
-                for (long long e = threadIdx.x; e < n; e += kBlock) {
+                for (long long e = threadIdx.x; e < n; e += kOptBlock) {
                     Sj[e] = XT[e] - X[e];
                     Yj[e] = GT[e] - G[e];
                 }
@@ -515,7 +521,7 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
             double gmax = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock) {
+            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
                 X[e] = XT[e];
                 G[e] = GT[e];
                 gmax = fmax(gmax, fabs(GT[e]));
@@ -554,7 +560,7 @@ __global__ void __launch_bounds__(kBlock)
                     xv[k] = rx[k] + an * dv[k];
                 lb_store(XT, xv, n);
             } else {
-                for (long long e = threadIdx.x; e < n; e += kBlock)
+                for (long long e = threadIdx.x; e < n; e += kOptBlock)
                     XT[e] = X[e] + an * D[e];
             }
             __syncthreads();
@@ -580,19 +586,19 @@ __global__ void __launch_bounds__(kBlock)
             gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
         } else {
         // two-loop recursion on q (kept in D), newest pair first
-        for (long long e = threadIdx.x; e < n; e += kBlock)
+        for (long long e = threadIdx.x; e < n; e += kOptBlock)
             D[e] = G[e];
         __syncthreads();
         for (int j = k - 1; j >= 0; --j) {
             const int slot = (head + j) % memory;
             const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
             double a = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 a += Sj[e] * D[e];
             a = block_sum_all(a, red, rpar) * RHO[slot];
             if (threadIdx.x == 0)
                 alph[j] = a;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 D[e] -= a * Yj[e];
             __syncthreads();
         }
@@ -600,11 +606,11 @@ __global__ void __launch_bounds__(kBlock)
             const int slot = (head + k - 1) % memory;
             const double *Yj = Y + (long long)slot * n;
             double yy = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 yy += Yj[e] * Yj[e];
             yy = block_sum_all(yy, red, rpar);
             const double gamma = 1.0 / (RHO[slot] * yy);
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 D[e] *= gamma;
             __syncthreads();
         }
@@ -612,16 +618,16 @@ __global__ void __launch_bounds__(kBlock)
             const int slot = (head + j) % memory;
             const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
             double bsum = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 bsum += Yj[e] * D[e];
             const double b = block_sum_all(bsum, red, rpar) * RHO[slot];
             const double c = alph[j] - b;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 D[e] += c * Sj[e];
             __syncthreads();
         }
         gd = 0.0;
-        for (long long e = threadIdx.x; e < n; e += kBlock) {
+        for (long long e = threadIdx.x; e < n; e += kOptBlock) {
             D[e] = -D[e];
             gd += G[e] * D[e];
         }
@@ -629,7 +635,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
             gd = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock) {
+            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
                 D[e] = -G[e];
                 gd += G[e] * D[e];
             }
@@ -645,7 +651,7 @@ __global__ void __launch_bounds__(kBlock)
         double alpha = 1.0;
         if (first) {
             double dmax = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 dmax = fmax(dmax, fabs(D[e]));
             dmax = block_max_all(dmax, red, rpar);
             if (dmax > 0.0)
@@ -658,7 +664,7 @@ __global__ void __launch_bounds__(kBlock)
                 xv[k2] = rxt[k2] + alpha * dv[k2];
             lb_store(XT, xv, n);
         } else {
-            for (long long e = threadIdx.x; e < n; e += kBlock)
+            for (long long e = threadIdx.x; e < n; e += kOptBlock)
                 XT[e] = X[e] + alpha * D[e];
         }
         __syncthreads();
@@ -669,7 +675,7 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     if (finish) {
-        for (long long e = threadIdx.x; e < n; e += kBlock)
+        for (long long e = threadIdx.x; e < n; e += kOptBlock)
             XT[e] = BX[e];
         __syncthreads();
         if (threadIdx.x == 0) {
