@@ -131,7 +131,9 @@ int mia_hip_graph_run(mia_hip_ctx *ctx);
  * checks the instances' done flags.  Same algorithm as the host `lbfgs`
  * minimizer: memory `memory` (<= 16), `maxeval` evaluations per instance
  * (NLopt's maxeval, which MIA's `maxiter` maps to, addons/nlopt/nlopt.cc:237),
- * first step moving no parameter by more than `first_step`.  x0: starting
+ * first step moving no parameter by more than `first_step`; instances of up
+ * to 2048 parameters form the same direction in the compact representation,
+ * so trajectories agree with the host to rounding.  x0: starting
  * parameters of all instances (host); outputs (host, any may be NULL): best
  * parameters, best costs, evaluations per instance, wall time of the
  * iteration loop in seconds (graph capture excluded).  Replaces the
