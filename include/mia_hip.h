@@ -81,6 +81,15 @@ int mia_hip_set_images(mia_hip_ctx *ctx, int inst, const float *moving, const fl
 /* Same with device pointers (float, HBM-resident, same layout). */
 int mia_hip_set_images_device(mia_hip_ctx *ctx, int inst, const void *d_moving, const void *d_fixed);
 
+/* Which reference-gradient stage-1 layout instance `inst` runs with after
+ * mia_hip_setup: 1 = per (z knot cell, y knot band) with the slices' W in
+ * registers (3D instances with x rows up to 256 voxels and y bands of up to
+ * 9 rows), 0 = per (slice, y band) (2D, exact mode, larger rows).  The
+ * choice depends only on the instance's own sizes, so an instance computes
+ * the same in any batch.  Counterpart of the reference's translate path
+ * (mia/3d/transform/spline.cc:731-791); introspection only. */
+int mia_hip_get_stage1_path(mia_hip_ctx *ctx, int inst, int *path);
+
 /* Cost options: ssd `norm` (default false), `autothresh` (0 = off), the
  * full-cost `weight` (mia/3d/fullcost/image.cc:155-165,
  * mia/template/fullcost.cxx:36-45) and the gradient mode

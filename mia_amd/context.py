@@ -80,6 +80,12 @@ class HipContext:
             self._offsets.append((b.value, a.value))
         self.n_params_total = sum(n for _, n in self._offsets)
 
+    def stage1_path(self, inst):
+        """1: reference-gradient stage 1 per (z knot cell, y band); 0: per (slice, y band)."""
+        v = ctypes.c_int()
+        check(self._L.mia_hip_get_stage1_path(self._h, int(inst), ctypes.byref(v)))
+        return v.value
+
     def set_images(self, inst, moving, fixed):
         moving = np.ascontiguousarray(moving, dtype=np.float32)
         fixed = np.ascontiguousarray(fixed, dtype=np.float32)

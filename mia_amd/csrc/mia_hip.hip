@@ -909,6 +909,18 @@ int mia_hip_n_params(mia_hip_ctx *c, int inst, size_t *n, size_t *offset)
     return MIA_HIP_OK;
 }
 
+int mia_hip_get_stage1_path(mia_hip_ctx *c, int inst, int *path)
+{
+    if (!c || !path)
+        return fail(MIA_HIP_ERR_ARG, "null argument");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    *path = c->inst[inst].zcell;
+    return MIA_HIP_OK;
+}
+
 int mia_hip_cost_grad(mia_hip_ctx *c, const double *params, double *grad, double *costs)
 {
     int st = check_ready(c);

@@ -734,6 +734,71 @@ MO_EXPORT double mo_cost(int dim, const int size[3], const float *moving, const 
                         NULL, NULL, NULL);
 }
 
+/* The part of the unnormalised SSD 0.5 * sum (W - R)^2 (no autothresh) that
+ * comes from the voxels of the box [lo, hi) (x, y, z), for the prefiltered
+ * moving image `coeff` with clamp range [vmin, vmax].  A change of one
+ * coefficient moves W only inside its support (the voxels whose four spline
+ * weights along every axis include it), so the finite difference of mo_cost
+ * in that coefficient equals the difference of this sum over a box holding
+ * the support: the full-size finite-difference check costs one support per
+ * coefficient instead of two whole-image evaluations.  Per voxel it repeats
+ * mo_displacement's sum and mo_interp3 / mo_interp2 (the arithmetic of
+ * mo_cost_grad). */
+MO_EXPORT double mo_cost_box(int dim, const int size[3], const double *coeff, double vmin, double vmax,
+                             const float *fixed, const int csize[3], const double *params, int degree,
+                             const int lo[3], const int hi[3])
+{
+    axis_t ax, ay, az;
+    const int nz = dim == 3 ? size[2] : 1;
+    const int csz = dim == 3 ? csize[2] : 1;
+    axis_init(&ax, size[0], csize[0]);
+    axis_init(&ay, size[1], csize[1]);
+    if (dim == 3)
+        axis_init(&az, size[2], csize[2]);
+    const int z0 = dim == 3 ? lo[2] : 0, z1 = dim == 3 ? hi[2] : 1;
+    double sum = 0.0;
+    for (int z = z0; z < z1 && z < nz; ++z)
+        for (int y = lo[1]; y < hi[1] && y < size[1]; ++y)
+            for (int x = lo[0]; x < hi[0] && x < size[0]; ++x) {
+                double u[3] = {0, 0, 0};
+                const int nwz = dim == 3 ? az.nw[z] : 1;
+                for (int c = 0; c < nwz; ++c) {
+                    const int kz = dim == 3 ? az.start[z] + c : 0;
+                    const double wz = dim == 3 ? az.w[4 * z + c] : 1.0;
+                    if (kz >= csz) break;
+                    for (int b = 0; b < ay.nw[y]; ++b) {
+                        const int ky = ay.start[y] + b;
+                        if (ky >= csize[1]) break;
+                        const double wy = ay.w[4 * y + b];
+                        for (int a = 0; a < ax.nw[x]; ++a) {
+                            const int kx = ax.start[x] + a;
+                            if (kx >= csize[0]) break;
+                            const double wgt = wz * wy * ax.w[4 * x + a];
+                            const double *cc = params + dim * (((size_t)kz * csize[1] + ky) * csize[0] + kx);
+                            for (int d = 0; d < dim; ++d)
+                                u[d] += wgt * cc[d];
+                        }
+                    }
+                }
+                const size_t idx = ((size_t)z * size[1] + y) * size[0] + x;
+                double w;
+                if (dim == 3) {
+                    const double p[3] = {x - u[0], y - u[1], z - u[2]};
+                    w = mo_interp3(coeff, size[0], size[1], size[2], degree, p, vmin, vmax, NULL);
+                } else {
+                    const double p[2] = {x - u[0], y - u[1]};
+                    w = mo_interp2(coeff, size[0], size[1], degree, p, vmin, vmax, NULL);
+                }
+                const double d = w - (double)fixed[idx];
+                sum += d * d;
+            }
+    axis_free(&ax);
+    axis_free(&ay);
+    if (dim == 3)
+        axis_free(&az);
+    return 0.5 * sum;
+}
+
 /* ------------------------------------------------------------------------ */
 /* f3: image resampling (the `scale` filter)                                 */
 /* ------------------------------------------------------------------------ */

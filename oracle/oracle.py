@@ -71,6 +71,9 @@ def lib():
         L.mo_cost.argtypes = [ctypes.c_int, _ip, _fp, _fp, _ip, _dp, ctypes.c_int, ctypes.c_int,
                               ctypes.c_float]
         L.mo_cost.restype = ctypes.c_double
+        L.mo_cost_box.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double, _fp, _ip, _dp,
+                                  ctypes.c_int, _ip, _ip]
+        L.mo_cost_box.restype = ctypes.c_double
         L.mo_gauss_line.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
         L.mo_scale_line.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
         L.mo_scale_image.argtypes = [_fp, _ip, _ip, _fp]
@@ -258,6 +261,53 @@ def cost(moving, fixed, csize, params, degree=3, norm=False, autothresh=0.0):
                          np.ascontiguousarray(fixed, dtype=np.float32), cs,
                          np.ascontiguousarray(params, dtype=np.float64), degree, int(norm),
                          float(autothresh))
+
+
+class PrefilteredPair:
+    """A moving image prefiltered once (FP64 coefficients and clamp range)
+    with its fixed image, for many support-restricted cost evaluations."""
+
+    def __init__(self, moving, fixed, csize, degree=3):
+        self.dim = moving.ndim
+        self.shape = moving.shape
+        self.size = _size3(moving.shape)
+        self.coeff, self.vmin, self.vmax = prefilter(moving, degree)
+        self.fixed = np.ascontiguousarray(fixed, dtype=np.float32)
+        self.csize = np.array(list(csize) + [1] * (3 - len(csize)), dtype=np.int32)
+        self.degree = degree
+        self.tables = [spline_axis_table(int(self.size[a]), int(self.csize[a]))[::2] for a in range(self.dim)]
+
+    def support_box(self, k):
+        """Voxel box [lo, hi) (x, y, z) of the support of coefficient k (its
+        index in the coefficient grid, x fastest)."""
+        lo = np.zeros(3, dtype=np.int32)
+        hi = np.ones(3, dtype=np.int32)
+        kk = [k % self.csize[0], (k // self.csize[0]) % self.csize[1], k // (self.csize[0] * self.csize[1])]
+        for a in range(self.dim):
+            start, nw = self.tables[a]
+            inside = np.nonzero((start <= kk[a]) & (start + nw - 1 >= kk[a]))[0]
+            if inside.size == 0:
+                return None
+            lo[a], hi[a] = inside[0], inside[-1] + 1
+        return lo, hi
+
+    def cost_box(self, params, lo, hi):
+        return lib().mo_cost_box(self.dim, self.size, self.coeff, self.vmin, self.vmax, self.fixed, self.csize,
+                                 np.ascontiguousarray(params, dtype=np.float64), self.degree,
+                                 np.ascontiguousarray(lo, dtype=np.int32), np.ascontiguousarray(hi, dtype=np.int32))
+
+    def fd(self, params, e, h):
+        """Central difference (cost(p + h e_e) - cost(p - h e_e)) / 2h of the
+        unnormalised cost in parameter e, over the support of its coefficient."""
+        box = self.support_box(e // self.dim)
+        if box is None:
+            return 0.0
+        p = np.array(params, dtype=np.float64)
+        p[e] += h
+        cp = self.cost_box(p, *box)
+        p[e] -= 2 * h
+        cm = self.cost_box(p, *box)
+        return (cp - cm) / (2 * h)
 
 
 def gauss_line(values, half_width):

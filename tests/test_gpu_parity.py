@@ -2,8 +2,9 @@
 
 Tolerances (FP32 kernels with FP64 SSD and gradient accumulation vs FP64):
   cost      relative 1e-5
-  gradient  max-norm relative 2e-5 (|g - g_ref|_inf / |g_ref|_inf)
+  gradient  max-norm relative 1e-5 (|g - g_ref|_inf / |g_ref|_inf)
   warped    max abs 2e-5 of the image range
+The measured errors are printed by each test (about 1e-7 to 2e-6).
 Reference-mode results are compared with the oracle's reference mode, exact
 mode with the oracle's exact mode.
 """
@@ -15,7 +16,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 COST_RTOL = 1e-5
-GRAD_RTOL = 2e-5
+GRAD_RTOL = 1e-5
 WARP_ATOL = 2e-5
 
 
@@ -66,12 +67,19 @@ def test_cost_grad_vs_oracle(mia, shape, rate, degree, mode):
         costs, g = c.cost_grad(p)
     c_ref, g_ref = O.cost_grad(mov, fix, cs, p, degree=degree, grad_mode=mode)
     assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
-    # The linear kernel's derivative jumps at integer positions: a sample
-    # within float rounding (~4e-6 at 64) of a knot takes the other one-sided
-    # derivative in FP32 than in FP64 (seen at y = 61.999991 for the (64, 80)
-    # case), which moves a handful of coefficients by one pixel's force.
-    tol = 1e-2 if (degree == 1 and mode == 1) else GRAD_RTOL
-    assert _rel_inf(g, g_ref) < tol
+    err = np.abs(g - g_ref) / np.abs(g_ref).max()
+    print("cost rel %.2e, gradient max-norm rel %.2e" % (abs(costs[0] - c_ref) / abs(c_ref), err.max()))
+    if degree == 1 and mode == 1:
+        # The linear kernel's derivative jumps at integer positions: a sample
+        # within float rounding (~4e-6 at 64) of a knot takes the other
+        # one-sided derivative in FP32 than in FP64 (seen at y = 61.999991 for
+        # the (64, 80) case), which moves the few coefficients whose support
+        # holds that pixel by one pixel's force.  Those may differ by up to
+        # 1e-2; every other coefficient is held to the common bound.
+        assert err.max() < 1e-2
+        assert np.count_nonzero(err >= GRAD_RTOL) <= 16
+    else:
+        assert err.max() < GRAD_RTOL
 
 
 @pytest.mark.parametrize("shape,rate,degree", CASES)
@@ -103,7 +111,8 @@ def test_warp_large_and_small_displacement(mia, shape, rate, sd):
     assert np.abs(W - W_ref).max() < 5e-5
     c_ref, g_ref = O.cost_grad(mov, fix, cs, p)
     assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
-    assert _rel_inf(g, g_ref) < 1e-4
+    print("gradient max-norm rel %.2e" % _rel_inf(g, g_ref))
+    assert _rel_inf(g, g_ref) < GRAD_RTOL
 
 
 def test_shift_warp_known_answer(mia, kat):
@@ -170,7 +179,8 @@ def test_norm_autothresh_weight(mia, dim):
             costs, g = c.cost_grad(p)
         c_ref, g_ref = O.cost_grad(mov, fix, cs, p, norm=norm, autothresh=thr, weight=weight, grad_mode=mode)
         assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL), (norm, thr, weight, mode)
-        assert _rel_inf(g, g_ref) < 1e-4, (norm, thr, weight, mode)
+        print((norm, thr, weight, mode), "gradient max-norm rel %.2e" % _rel_inf(g, g_ref))
+        assert _rel_inf(g, g_ref) < GRAD_RTOL, (norm, thr, weight, mode)
 
 
 def test_autothresh_empty_mask(mia):
@@ -268,7 +278,7 @@ def test_exact_gradient_matches_fd(mia):
                 pp[i] -= 2 * h
                 cm = cost(pp)
                 fds.append((cp - cm) / (2 * h))
-            assert g[i] == pytest.approx(fds[0], rel=1e-4, abs=2e-5 * gmax)
+            assert g[i] == pytest.approx(fds[0], rel=1e-4, abs=GRAD_RTOL * gmax)
             assert g[i] == pytest.approx(fds[1], rel=1e-2, abs=2e-3 * gmax)
 
 

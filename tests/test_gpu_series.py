@@ -144,6 +144,28 @@ def test_device_optimizer_both_search_paths():
     np.testing.assert_allclose(cd, ch, rtol=1e-9)
 
 
+def test_device_lbfgs_ring_wraparound():
+    """memory = 2: from the third accepted step on, every step drops the
+    oldest pair and overwrites its ring slot (and its rows of the pair-product
+    tables), so twelve evaluations wrap the ring several times.  Host and
+    device L-BFGS (same algorithm, different summation order) must agree
+    tightly on frames on both sides of the 2048-parameter limit of the
+    register-resident direction."""
+    from mia_amd import series, synth
+    frames = synth.series_frames_2d(2, (60, 80), 8.0, 4.0, 41) + synth.series_frames_2d(2, (300, 330), 8.0, 4.0, 42)
+    sizes = [2 * mia_amd_csize(f[1].shape) for f in frames]
+    assert min(sizes) <= 2048 < max(sizes)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        ph, ch, eh, _ = s.run("lbfgs:maxiter=12,memory=2", series.BATCHED)
+        pd, cd, ed, _ = s.run("lbfgs:maxiter=12,memory=2", series.DEVICE_BATCHED)
+    print("evaluations", eh, ed, "cost rel", np.abs(cd - ch) / np.abs(ch),
+          "param max abs", np.abs(pd - ph).max())
+    assert np.array_equal(eh, ed)
+    assert np.all(eh >= 10)
+    np.testing.assert_allclose(cd, ch, rtol=1e-9)
+    assert np.abs(pd - ph).max() <= 1e-7 * max(1.0, np.abs(ph).max())
+
+
 def mia_amd_csize(shape):
     import mia_amd
     return mia_amd.spline_csize(shape[1], 8.0, 2) * mia_amd.spline_csize(shape[0], 8.0, 2)

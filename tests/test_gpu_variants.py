@@ -13,7 +13,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 COST_RTOL = 1e-5
-GRAD_RTOL = 2e-5
+GRAD_RTOL = 1e-5
 WARP_ATOL = 2e-5
 
 

@@ -489,6 +489,47 @@ def test_exact_gradient_matches_fd(shape, rate, degree):
         assert g[i] == pytest.approx(fd, rel=2e-4, abs=2e-6 * np.abs(g).max())
 
 
+@pytest.mark.parametrize("shape,rate", [((23, 29), 4.0), ((13, 11, 15), 3.0)])
+def test_support_box_cost_difference_equals_full(shape, rate):
+    """The support-restricted cost (used for the full-size FD check) against
+    the whole-image cost: over the whole image it is the same sum, and for
+    every coefficient (edges included) the difference of the box sums equals
+    the difference of the full costs, so no voxel the coefficient moves lies
+    outside its box.  The same box one voxel short at its high x end fails
+    that check for every coefficient (the last voxel of a support carries a
+    nonzero weight), so the check would catch a box that is too small."""
+    dim = len(shape)
+    mov, fix = _smooth_pair(shape, 5)
+    cs = tuple(O.spline_csize(n, rate, dim) for n in shape[::-1])
+    rng = np.random.default_rng(3)
+    params = rng.normal(scale=0.8, size=dim * int(np.prod(cs)))
+    pair = O.PrefilteredPair(mov, fix, cs)
+    full = np.array(list(shape[::-1]) + [1] * (3 - dim), dtype=np.int32)
+    assert pair.cost_box(params, np.zeros(3, np.int32), full) == pytest.approx(
+        O.cost(mov, fix, cs, params), rel=1e-13)
+    h = 0.05
+    shrunk = 0
+    for k in range(int(np.prod(cs))):
+        e = dim * k + int(rng.integers(dim))
+        pp = params.copy()
+        pp[e] += h
+        cp = O.cost(mov, fix, cs, pp)
+        pp[e] -= 2 * h
+        cm = O.cost(mov, fix, cs, pp)
+        fd_full = (cp - cm) / (2 * h)
+        assert pair.fd(params, e, h) == pytest.approx(fd_full, rel=1e-9, abs=1e-12), k
+        lo, hi = pair.support_box(k)
+        hi = hi.copy()
+        hi[0] -= 1
+        pp = params.copy()
+        pp[e] += h
+        a = pair.cost_box(pp, lo, hi)
+        pp[e] -= 2 * h
+        b = pair.cost_box(pp, lo, hi)
+        shrunk += abs((a - b) / (2 * h) - fd_full) > 1e-9 * max(1.0, abs(fd_full))
+    assert shrunk == np.prod(cs)
+
+
 def test_reference_mode_is_warped_image_gradient():
     """grad_mode 0 is translate(central-difference force); at zero displacement
     the warped image is the moving image itself, so it also equals the exact
