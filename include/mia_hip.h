@@ -224,6 +224,40 @@ int mia_hip_divcurl_create(int device, const uint32_t *csize, const float *range
 int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs, double *value, double *gradient);
 int mia_hip_divcurl_destroy(mia_hip_divcurl *p);
 
+/* ---- allocation-free evaluation ----------------------------------------- */
+
+/* Size of the scratch memory of mia_hip_cost_grad_noalloc: one array of this
+ * many bytes per GPU the library runs on, held until the process ends. */
+#define MIA_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Cost and gradient of ONE instance, as mia_hip_setup +
+ * mia_hip_set_images_device + mia_hip_set_cost + mia_hip_cost_grad_device
+ * compute them for a batch of that one instance (same kernels, same
+ * results), with every buffer beyond the inputs and outputs (the instance's
+ * tables, the prefiltered moving image, the warped image, the stage-1 partial
+ * sums) in the library's file-scope scratch memory of
+ * MIA_HIP_NOALLOC_SCRATCH_BYTES on `device`.  Calls no hipMalloc, hipFree or
+ * allocating library routine, and does not synchronise: everything is
+ * enqueued on `stream` (a hipStream_t, NULL for the null stream).
+ *   size, csize : x, y[, z] image and coefficient grid sizes (dim entries)
+ *   d_moving, d_fixed : device float images (x fastest), not modified
+ *   d_params : device doubles, 2 or 3 per coefficient (interleaved)
+ *   d_grad   : device doubles, same layout, overwritten
+ *   d_cost   : one device double
+ * MIA_HIP_ERR_LIMIT when the instance needs more scratch memory than that
+ * (about 17 bytes per voxel plus the partial sums: a 3D volume of 3.5 M
+ * voxels at spline rate 8, a 2D frame of 1800 x 1800 pixels).  Every call on
+ * one GPU uses that GPU's one array and it keeps the last call's contents
+ * (nothing in it needs to start at zero): calls that overlap on one GPU
+ * (different streams, or host threads) race and must be serialised by the
+ * caller. */
+int mia_hip_cost_grad_noalloc(int device, int dim, const uint32_t *size, const uint32_t *csize, int degree,
+                              int norm, float autothresh, double weight, int grad_mode, const float *d_moving,
+                              const float *d_fixed, const double *d_params, double *d_grad, double *d_cost,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,35 @@ struct InstDesc {
     int zcell;                // 3D reference gradient: stage 1 per (z knot cell, y band)
 };
 
+// Entry i of a spline axis table with the reference's semantics
+// (mia/3d/transform/spline.cc:237-273, mia/2d/transform/spline.cc:181-200):
+// float scale (csize - 3) / (n - 1), float position i*scale + 1, start index
+// fastfloor(pos) - 1 (16.16 rounding, mia/core/splinekernel.cc:164-192),
+// cubic weights at pos - fastfloor(pos) in double stored as float, the last
+// sample's fourth weight dropped.  Returns the start index.  The host set-up
+// and the device set-up of the allocation-free entry point both call it; with
+// contraction off every operation rounds as written, so both build the same
+// bits.
+__host__ __device__ inline int spline_axis_entry(int i, int n, int cs, float w[4])
+{
+#pragma clang fp contract(off)
+    const float scale = (float)(cs - 3) / (float)(n - 1);
+    const float si = (float)i * scale;
+    const float pos = si + 1.0f;
+    const int f = (int)floor(nearbyint((double)pos * 65536.0) / 65536.0);
+    const double x = (double)pos - f;
+    const double xm1 = 1.0 - x;
+    const double w3 = (1.0 / 6.0) * x * x * x;
+    const double w0 = (1.0 / 6.0) * xm1 * xm1 * xm1;
+    const double w2 = x - 2.0 * w3 + w0;
+    const double w1 = xm1 - 2.0 * w0 + w3;
+    w[0] = (float)w0;
+    w[1] = (float)w1;
+    w[2] = (float)w2;
+    w[3] = (i == n - 1) ? 0.0f : (float)w3;
+    return f - 1;
+}
+
 // Where the stage-1 SSD partials of an instance are: per (slice, y band) block
 // at s1[i] .. s1[i + 1], or, for an instance with zcell set while zmode is on
 // (3D reference gradient), per (z knot cell, y band) block at zoff + s1z[i] ..
@@ -2168,6 +2197,71 @@ __global__ void __launch_bounds__(kBlock)
     for (int q = 0; q < 16; ++q)
         g += (double)v[q];
     grad[d.par_off + e3] = sc * g;
+}
+
+// ---------------------------------------------------------------------------
+// set-up of the allocation-free evaluation
+// ---------------------------------------------------------------------------
+
+// Block counts of a one-instance batch (the totals of the per-instance block
+// prefixes that the host set-up builds).
+struct OneInstCounts {
+    int s1, red, s1z, redz, tiles;
+};
+
+// Layout of the small integer arrays of a one-instance batch: the four block
+// prefixes {0, total}, then the warp launch's instance list {0} and tile
+// prefix {0, tiles}.
+constexpr int kOneInstInts = 11;
+
+// One block writes what the host set-up builds and copies for a batch of one
+// instance: the instance description, the block prefixes, the warp launch's
+// lists and the spline axis tables (spline_axis_entry, the host's arithmetic)
+// with their band tables.  Kernel arguments only, so the sequence can be
+// captured in a graph.
+__global__ void __launch_bounds__(kBlock)
+    k_one_inst_setup(InstDesc d, int dim, OneInstCounts nc, InstDesc *__restrict__ inst, int *__restrict__ ints,
+                     float4 *__restrict__ tw, int *__restrict__ ts, int *__restrict__ cells)
+{
+    if (threadIdx.x == 0) {
+        *inst = d;
+        const int v[kOneInstInts] = {0, nc.s1, 0, nc.red, 0, nc.s1z, 0, nc.redz, 0, 0, nc.tiles};
+        for (int k = 0; k < kOneInstInts; ++k)
+            ints[k] = v[k];
+    }
+    const int ns[3] = {d.nx, d.ny, d.nz}, cs[3] = {d.cx, d.cy, d.cz};
+    const int toff[3] = {d.tab_x, d.tab_y, d.tab_z}, coff[3] = {d.cell_x, d.cell_y, d.cell_z};
+    for (int a = 0; a < 3; ++a) {
+        if (a < dim) {
+            const int n = ns[a], c = cs[a];
+            for (int i = threadIdx.x; i < n; i += kBlock) {
+                float w[4];
+                ts[toff[a] + i] = spline_axis_entry(i, n, c, w);
+                tw[toff[a] + i] = make_float4(w[0], w[1], w[2], w[3]);
+            }
+            // band table: first sample whose start index is >= k (n if none);
+            // the start indices ascend, so a binary search over recomputed
+            // entries finds it without reading another thread's stores
+            for (int k = threadIdx.x; k <= c; k += kBlock) {
+                int lo = 0, hi = n;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    float w[4];
+                    if (spline_axis_entry(mid, n, c, w) < k)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                cells[coff[a] + k] = lo;
+            }
+        } else if (threadIdx.x == 0) {
+            // the z axis of a 2D instance: one sample with weight 1, one band
+            ts[toff[a]] = 0;
+            tw[toff[a]] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+            cells[coff[a]] = 0;
+            cells[coff[a] + 1] = 1;
+        }
+    }
 }
 
 } // namespace mia_hip

@@ -47,12 +47,8 @@ using mia_hip_rt::fail;
 enum Stage { ST_WARP = 0, ST_STAGE1, ST_FINALIZE, ST_REDUCE, ST_COUNT };
 const char *kStageNames[ST_COUNT] = {"warp", "ssd_force_adjoint1", "finalize", "adjoint_reduce"};
 
-// Spline axis table with the reference's semantics
-// (mia/3d/transform/spline.cc:237-273, mia/2d/transform/spline.cc:181-200):
-// float scale (csize - 3) / (n - 1), float position i*scale + 1, start index
-// fastfloor(pos) - 1 (16.16 rounding, mia/core/splinekernel.cc:164-192), cubic
-// weights at pos - fastfloor(pos) in double, the last sample's fourth weight
-// dropped.  Band table: cells[c] = first sample whose start index is >= c.
+// Spline axis table (entries from spline_axis_entry) and its band table:
+// cells[c] = first sample whose start index is >= c.
 struct AxisTable {
     std::vector<float> w; // 4 per sample
     std::vector<int> start;
@@ -60,7 +56,13 @@ struct AxisTable {
     int nbands = 0;
 };
 
-int make_axis_table(int n, int cs, AxisTable &t, const char *axis)
+// Sizes an axis of the kernels accepts, checked without building the table:
+// the number of knot bands and the most rows (samples) in one band.
+struct AxisCheck {
+    int nbands = 0, band_rows = 0;
+};
+
+int check_axis(int n, int cs, const char *axis, AxisCheck &out)
 {
     if (n < 2)
         return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": image size must be >= 2");
@@ -69,34 +71,37 @@ int make_axis_table(int n, int cs, AxisTable &t, const char *axis)
     if (cs - 3 > n - 1)
         return fail(MIA_HIP_ERR_LIMIT, std::string("axis ") + axis +
                                            ": coefficient grid denser than the image (scale > 1) is not supported");
-    const float scale = (float)(cs - 3) / (float)(n - 1);
-    t.w.assign(4 * (size_t)n, 0.0f);
-    t.start.assign(n, 0);
-    for (int i = 0; i < n; ++i) {
-        const float si = (float)i * scale;
-        const float pos = si + 1.0f;
-        const int f = (int)std::floor(std::nearbyint((double)pos * 65536.0) / 65536.0);
-        const double x = (double)pos - f;
-        const double xm1 = 1.0 - x;
-        const double w3 = (1.0 / 6.0) * x * x * x;
-        const double w0 = (1.0 / 6.0) * xm1 * xm1 * xm1;
-        const double w2 = x - 2.0 * w3 + w0;
-        const double w1 = xm1 - 2.0 * w0 + w3;
-        t.start[i] = f - 1;
-        t.w[4 * i + 0] = (float)w0;
-        t.w[4 * i + 1] = (float)w1;
-        t.w[4 * i + 2] = (float)w2;
-        t.w[4 * i + 3] = (i == n - 1) ? 0.0f : (float)w3;
-    }
-    if (t.start[0] != 0)
+    float w[4];
+    int prev = spline_axis_entry(0, n, cs, w), run = 1, rows = 1, last_but_one = prev;
+    if (prev != 0)
         return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": unexpected first start index");
     for (int i = 1; i < n; ++i) {
-        const int dlt = t.start[i] - t.start[i - 1];
+        const int s = spline_axis_entry(i, n, cs, w);
+        const int dlt = s - prev;
         if (dlt < 0 || dlt > 1)
             return fail(MIA_HIP_ERR_LIMIT, std::string("axis ") + axis + ": non-unit knot step");
+        run = dlt ? 1 : run + 1;
+        rows = std::max(rows, run);
+        last_but_one = prev;
+        prev = s;
     }
-    if (t.start[n - 1] + 3 > cs || (n > 1 && t.start[n - 2] + 3 > cs - 1))
+    if (prev + 3 > cs || (n > 1 && last_but_one + 3 > cs - 1))
         return fail(MIA_HIP_ERR_ARG, std::string("axis ") + axis + ": knot range exceeds the coefficient grid");
+    out.nbands = prev + 1;
+    out.band_rows = rows;
+    return MIA_HIP_OK;
+}
+
+int make_axis_table(int n, int cs, AxisTable &t, const char *axis)
+{
+    AxisCheck ck;
+    int st = check_axis(n, cs, axis, ck);
+    if (st)
+        return st;
+    t.w.assign(4 * (size_t)n, 0.0f);
+    t.start.assign(n, 0);
+    for (int i = 0; i < n; ++i)
+        t.start[i] = spline_axis_entry(i, n, cs, &t.w[4 * (size_t)i]);
     t.cells.assign(cs + 1, n);
     int c = 0;
     for (int i = 0; i < n; ++i)
@@ -338,20 +343,20 @@ hipEvent_t get_event(mia_hip_ctx *c)
     return e;
 }
 
-// Launch a kernel on the context's stream.  With profiling on, a pair of
-// timing events brackets the dispatch itself (hipExtLaunchKernelGGL) rather
-// than separate markers queued around it.
+// Launch a kernel on stream s.  With a profiling context, a pair of timing
+// events brackets the dispatch itself (hipExtLaunchKernelGGL) rather than
+// separate markers queued around it.
 template <typename K, typename... Args>
-void launch(mia_hip_ctx *c, int stage, bool timed, K kern, dim3 grid, size_t lds, Args... args)
+void launch(hipStream_t s, mia_hip_ctx *prof, int stage, K kern, dim3 grid, size_t lds, Args... args)
 {
     hipEvent_t a = nullptr, b = nullptr;
-    if (timed) {
-        a = get_event(c);
-        b = get_event(c);
+    if (prof) {
+        a = get_event(prof);
+        b = get_event(prof);
     }
-    hipExtLaunchKernelGGL(kern, grid, dim3(kBlock), (uint32_t)lds, c->stream, a, b, 0, args...);
-    if (timed)
-        c->pending.push_back({stage, a, b});
+    hipExtLaunchKernelGGL(kern, grid, dim3(kBlock), (uint32_t)lds, s, a, b, 0, args...);
+    if (prof)
+        prof->pending.push_back({stage, a, b});
 }
 
 template <typename T>
@@ -361,11 +366,6 @@ int dev_alloc(T **p, size_t count)
         count = 1;
     HIP_TRY(hipMalloc((void **)p, sizeof(T) * count));
     return MIA_HIP_OK;
-}
-
-size_t stage1_lds(const mia_hip_ctx *c, bool exact)
-{
-    return stage1_layout(c->dim, c->max_nx, c->max_cx, c->max_ncx, exact).bytes;
 }
 
 template <int DEG>
@@ -379,111 +379,187 @@ const void *warp3d_kernel(int variant)
     }
 }
 
-// Warp the instances of group g, blocks [boff, boff + nb) of its tile prefix.
-int launch_warp(mia_hip_ctx *c, const double *d_params, int g, int boff, int nb, bool timed)
+// One warp launch: the instances of a batch that share a tile variant.
+struct WarpLaunch {
+    int variant = 0, n = 0, tiles = 0;
+    const int *ids = nullptr, *prefix = nullptr; // device: instance ids, tile prefix
+};
+
+// Everything one cost + gradient evaluation launches with: device pointers
+// and host-side counts, no containers.  Built per call from a context, or on
+// the stack by the allocation-free entry point over its scratch memory.
+struct EvalPlan {
+    int dim = 0, degree = 3, n_inst = 0, n_zcell = 0;
+    int norm = 0, grad_mode = MIA_HIP_GRAD_REFERENCE;
+    float thr = 0.0f;
+    double weight = 1.0;
+    int max_nx = 0, max_cx = 0, max_ncx = 0, w2_win = 0;
+    int s1_blocks = 0, red_blocks = 0, s1z_blocks = 0, redz_blocks = 0; // prefix totals
+    int n_warp = 0;
+    WarpLaunch warp[kW3Variants];
+    const InstDesc *inst = nullptr;
+    const int *s1_prefix = nullptr, *red_prefix = nullptr, *s1z_prefix = nullptr, *redz_prefix = nullptr;
+    const float *M = nullptr, *R = nullptr;
+    float *W = nullptr, *P = nullptr;
+    const float4 *tw = nullptr;
+    const int *ts = nullptr, *cells = nullptr;
+    const float2 *clamp = nullptr;
+    double *ssd = nullptr, *gscale = nullptr;
+    unsigned *cnt = nullptr;
+};
+
+EvalPlan plan_of(const mia_hip_ctx *c)
 {
-    const mia_hip_ctx::WarpGroup &G = c->wgroups[g];
-    const int *ids = c->d_wgroups + G.ids_off, *prefix = c->d_wgroups + G.prefix_off;
-    const int n = (int)G.ids.size();
+    EvalPlan p;
+    p.dim = c->dim;
+    p.degree = c->degree;
+    p.n_inst = c->n_inst;
+    p.n_zcell = c->n_zcell;
+    p.norm = c->norm;
+    p.grad_mode = c->grad_mode;
+    p.thr = c->thr;
+    p.weight = c->weight;
+    p.max_nx = c->max_nx;
+    p.max_cx = c->max_cx;
+    p.max_ncx = c->max_ncx;
+    p.w2_win = c->w2_win;
+    const int n = c->n_inst;
+    p.s1_blocks = c->s1_prefix[n];
+    p.red_blocks = c->red_prefix[n];
+    p.s1z_blocks = c->s1z_prefix[n];
+    p.redz_blocks = c->redz_prefix[n];
+    p.n_warp = (int)c->wgroups.size();
+    for (int g = 0; g < p.n_warp; ++g) {
+        const auto &G = c->wgroups[g];
+        p.warp[g].variant = G.variant;
+        p.warp[g].n = (int)G.ids.size();
+        p.warp[g].tiles = G.prefix.back();
+        p.warp[g].ids = c->d_wgroups + G.ids_off;
+        p.warp[g].prefix = c->d_wgroups + G.prefix_off;
+    }
+    p.inst = c->d_inst;
+    p.s1_prefix = c->d_s1_prefix;
+    p.red_prefix = c->d_red_prefix;
+    p.s1z_prefix = c->d_s1z_prefix;
+    p.redz_prefix = c->d_redz_prefix;
+    p.M = c->d_M;
+    p.R = c->d_R;
+    p.W = c->d_W;
+    p.P = c->d_P;
+    p.tw = c->d_tw;
+    p.ts = c->d_ts;
+    p.cells = c->d_cells;
+    p.clamp = c->d_clamp;
+    p.ssd = c->d_ssd;
+    p.gscale = c->d_gscale;
+    p.cnt = c->d_cnt;
+    return p;
+}
+
+// Warp the instances of warp launch g, blocks [boff, boff + nb) of its tile
+// prefix.
+int launch_warp(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const double *d_params, int g, int boff,
+                int nb)
+{
+    const WarpLaunch &G = pl.warp[g];
     if (nb <= 0)
         return MIA_HIP_OK;
-    if (c->dim == 3) {
+    if (pl.dim == 3) {
         using K = void (*)(const InstDesc *, const int *, int, const int *, int, int, const double *, const float4 *,
                            const int *, const float *, const float2 *, float *);
-        auto kern = (K)(c->degree == 3 ? warp3d_kernel<3>(G.variant) : warp3d_kernel<1>(G.variant));
-        launch(c, ST_WARP, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
-               (const float4 *)c->d_tw, (const int *)c->d_ts, (const float *)c->d_M, (const float2 *)c->d_clamp,
-               c->d_W);
+        auto kern = (K)(pl.degree == 3 ? warp3d_kernel<3>(G.variant) : warp3d_kernel<1>(G.variant));
+        launch(s, prof, ST_WARP, kern, dim3(nb), 0, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw, pl.ts,
+               pl.M, pl.clamp, pl.W);
     } else {
-        auto kern = c->degree == 3 ? k_warp2d<3> : k_warp2d<1>;
-        const size_t lds = (size_t)kW2Region * sizeof(float) + (size_t)c->w2_win * sizeof(float2);
-        launch(c, ST_WARP, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, ids, n, prefix, boff, nb, d_params,
-               (const float4 *)c->d_tw, (const int *)c->d_ts, (const float *)c->d_M, (const float2 *)c->d_clamp,
-               c->d_W);
+        auto kern = pl.degree == 3 ? k_warp2d<3> : k_warp2d<1>;
+        const size_t lds = (size_t)kW2Region * sizeof(float) + (size_t)pl.w2_win * sizeof(float2);
+        launch(s, prof, ST_WARP, kern, dim3(nb), lds, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw,
+               pl.ts, pl.M, pl.clamp, pl.W);
     }
     LAUNCH_CHECK("warp");
     return MIA_HIP_OK;
 }
 
 // Enqueue warp (reference mode), stage 1, finalize (only with autothresh) and
-// reduce for all instances.  No memsets: every output element is written.
-int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
+// reduce for all instances of the plan on stream s.  No memsets: every output
+// element is written.
+int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const double *d_params, double *d_grad,
+                 double *d_costs)
 {
-    const int n = c->n_inst;
-    const bool exact = c->grad_mode == MIA_HIP_GRAD_EXACT;
+    const int n = pl.n_inst;
+    const bool exact = pl.grad_mode == MIA_HIP_GRAD_EXACT;
     if (!exact)
-        for (int g = 0; g < (int)c->wgroups.size(); ++g) {
-            int st = launch_warp(c, d_params, g, 0, c->wgroups[g].prefix.back(), timed);
+        for (int g = 0; g < pl.n_warp; ++g) {
+            int st = launch_warp(pl, s, prof, d_params, g, 0, pl.warp[g].tiles);
             if (st)
                 return st;
         }
     // 3D reference gradient: instances with zcell run stage 1 per (z knot
     // cell, y band) and the single-read reduce, the others per (slice, y
     // band); their SSD partials are kept apart (SsdIndex)
-    const int n_z = c->dim == 3 && !exact ? c->n_zcell : 0;
+    const int n_z = pl.dim == 3 && !exact ? pl.n_zcell : 0;
     const bool any_slice = n_z < n;
     SsdIndex si;
-    si.s1 = c->d_s1_prefix;
-    si.s1z = c->d_s1z_prefix;
-    si.zoff = c->s1_prefix[n];
-    si.zmode = c->dim == 3 && !exact ? 1 : 0;
-    const size_t lds = stage1_lds(c, exact);
+    si.s1 = pl.s1_prefix;
+    si.s1z = pl.s1z_prefix;
+    si.zoff = pl.s1_blocks;
+    si.zmode = pl.dim == 3 && !exact ? 1 : 0;
+    const size_t lds = stage1_layout(pl.dim, pl.max_nx, pl.max_cx, pl.max_ncx, exact).bytes;
     if (any_slice) {
-        const int nb = c->s1_prefix[n];
+        const int nb = pl.s1_blocks;
         decltype(&k_stage1_3d<3, 0>) kern;
-        if (c->dim == 3)
-            kern = c->degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
+        if (pl.dim == 3)
+            kern = pl.degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
                                   : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
         else
-            kern = c->degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
+            kern = pl.degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
                                   : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
-        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n,
-               (const int *)c->d_s1_prefix, nb, d_params, (const float4 *)c->d_tw, (const int *)c->d_ts,
-               (const int *)c->d_cells, (const float *)c->d_M, (const float2 *)c->d_clamp, (const float *)c->d_W,
-               (const float *)c->d_R, c->thr, c->d_P, c->d_ssd, c->d_cnt);
+        launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1_prefix, nb, d_params, pl.tw, pl.ts,
+               pl.cells, pl.M, pl.clamp, (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd, pl.cnt);
         LAUNCH_CHECK("stage1");
     }
     if (n_z > 0) {
-        const int nb = c->s1z_prefix[n];
-        auto kern = c->degree == 3 ? k_stage1z_3d<3, kS1zRows> : k_stage1z_3d<1, kS1zRows>;
-        launch(c, ST_STAGE1, timed, kern, dim3(nb), lds, (const InstDesc *)c->d_inst, n,
-               (const int *)c->d_s1z_prefix, nb, (const float4 *)c->d_tw, (const int *)c->d_ts,
-               (const int *)c->d_cells, (const float *)c->d_W, (const float *)c->d_R, c->thr, c->d_P,
-               c->d_ssd + si.zoff, c->d_cnt + si.zoff);
+        const int nb = pl.s1z_blocks;
+        auto kern = pl.degree == 3 ? k_stage1z_3d<3, kS1zRows> : k_stage1z_3d<1, kS1zRows>;
+        launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1z_prefix, nb, pl.tw, pl.ts, pl.cells,
+               (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd + si.zoff, pl.cnt + si.zoff);
         LAUNCH_CHECK("stage1");
     }
-    const bool fused = c->thr == 0.0f;
+    const bool fused = pl.thr == 0.0f;
     if (!fused) {
-        launch(c, ST_FINALIZE, timed, k_finalize, dim3(n), 0, (const InstDesc *)c->d_inst, si,
-               (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs,
-               c->d_gscale);
+        launch(s, prof, ST_FINALIZE, k_finalize, dim3(n), 0, pl.inst, si, (const double *)pl.ssd,
+               (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs, pl.gscale);
         LAUNCH_CHECK("finalize");
     }
     // with a fused finalize, the first reduce launch does it for every instance
     int nfin = fused ? n : 0;
     if (any_slice) {
-        const int nb = c->red_prefix[n] + nfin;
+        const int nb = pl.red_blocks + nfin;
         decltype(&k_reduce<3, true>) kern;
-        if (c->dim == 3)
+        if (pl.dim == 3)
             kern = fused ? k_reduce<3, true> : k_reduce<3, false>;
         else
             kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
-        launch(c, ST_REDUCE, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, n, (const int *)c->d_red_prefix,
-               (const float *)c->d_P, (const float4 *)c->d_tw, (const int *)c->d_ts, (const int *)c->d_cells, si,
-               nfin, (const double *)c->d_ssd, (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs,
-               (const double *)c->d_gscale, d_grad);
+        launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.red_prefix, (const float *)pl.P, pl.tw, pl.ts,
+               pl.cells, si, nfin, (const double *)pl.ssd, (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight,
+               d_costs, (const double *)pl.gscale, d_grad);
         LAUNCH_CHECK("reduce");
         nfin = 0;
     }
     if (n_z > 0) {
-        const int nb = c->redz_prefix[n] + nfin;
+        const int nb = pl.redz_blocks + nfin;
         auto kern = fused ? k_reducez_3d<true> : k_reducez_3d<false>;
-        launch(c, ST_REDUCE, timed, kern, dim3(nb), 0, (const InstDesc *)c->d_inst, n,
-               (const int *)c->d_redz_prefix, (const float *)c->d_P, si, nfin, (const double *)c->d_ssd,
-               (const unsigned *)c->d_cnt, c->norm, c->thr, c->weight, d_costs, (const double *)c->d_gscale, d_grad);
+        launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.redz_prefix, (const float *)pl.P, si, nfin,
+               (const double *)pl.ssd, (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs,
+               (const double *)pl.gscale, d_grad);
         LAUNCH_CHECK("reduce");
     }
     return MIA_HIP_OK;
+}
+
+int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
+{
+    return enqueue_plan(plan_of(c), c->stream, timed ? c : nullptr, d_params, d_grad, d_costs);
 }
 
 int check_ready(mia_hip_ctx *c)
@@ -498,17 +574,19 @@ int check_ready(mia_hip_ctx *c)
     return MIA_HIP_OK;
 }
 
-int prefilter_instance(mia_hip_ctx *c, int i)
+// Clamp range and, for cubic interpolation, the in-place prefilter of a
+// moving image on stream s (the double scratch holds one causal pass: as many
+// doubles as the image has voxels).
+int enqueue_prefilter(const InstDesc &d, int dim, int degree, float *img, float2 *mm, float2 *clamp_out,
+                      double *scratch, hipStream_t s)
 {
-    const InstDesc &d = c->inst[i];
-    float *img = c->d_M + d.img_off;
     const long long nvox = (long long)d.nx * d.ny * d.nz;
     const int nbm = (int)std::min<long long>(1024, (nvox + kBlock * 16 - 1) / (kBlock * 16));
-    hipLaunchKernelGGL(k_minmax_partial, dim3(nbm), dim3(kBlock), 0, c->stream, img, nvox, c->d_mm);
+    hipLaunchKernelGGL(k_minmax_partial, dim3(nbm), dim3(kBlock), 0, s, img, nvox, mm);
     LAUNCH_CHECK("minmax");
-    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, c->stream, c->d_mm, nbm, c->d_clamp + i);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, s, (const float2 *)mm, nbm, clamp_out);
     LAUNCH_CHECK("minmax_final");
-    if (c->degree != 3)
+    if (degree != 3)
         return MIA_HIP_OK;
     struct Pass {
         long long nlines;
@@ -518,13 +596,46 @@ int prefilter_instance(mia_hip_ctx *c, int i)
     } passes[3] = {{(long long)d.ny * d.nz, d.nx, 1, 0},
                    {(long long)d.nx * d.nz, d.ny, d.nx, 1},
                    {(long long)d.nx * d.ny, d.nz, (long long)d.nx * d.ny, 2}};
-    const int npass = c->dim == 3 ? 3 : 2;
+    const int npass = dim == 3 ? 3 : 2;
     for (int k = 0; k < npass; ++k) {
         const Pass &p = passes[k];
         const long long blocks = (p.nlines + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(k_prefilter_lines, dim3(blocks), dim3(kBlock), 0, c->stream, img, p.nlines, p.n, p.stride,
-                           p.mode, d.nx, d.ny, c->d_scratch);
+        hipLaunchKernelGGL(k_prefilter_lines, dim3(blocks), dim3(kBlock), 0, s, img, p.nlines, p.n, p.stride, p.mode,
+                           d.nx, d.ny, scratch);
         LAUNCH_CHECK("prefilter");
+    }
+    return MIA_HIP_OK;
+}
+
+int prefilter_instance(mia_hip_ctx *c, int i)
+{
+    const InstDesc &d = c->inst[i];
+    return enqueue_prefilter(d, c->dim, c->degree, c->d_M + d.img_off, c->d_mm, c->d_clamp + i, c->d_scratch,
+                             c->stream);
+}
+
+// Stage 1 keeps one x row (plus tables and, in exact mode, a coefficient
+// window) in dynamic LDS: allow more than the 64 KiB default where a batch
+// whose longest row is max_nx needs it.
+int set_stage1_lds(int dim, int max_nx, int max_cx, int max_ncx)
+{
+    for (int exact = 0; exact < 2; ++exact) {
+        const size_t lds = stage1_layout(dim, max_nx, max_cx, max_ncx, exact != 0).bytes;
+        if (lds > 160 * 1024)
+            return fail(MIA_HIP_ERR_LIMIT, "stage-1 LDS requirement exceeds 160 KiB");
+        const void *fn[2][2] = {{(const void *)k_stage1_3d<3, 0>, (const void *)k_stage1_3d<1, 0>},
+                                {(const void *)k_stage1_3d<3, 1>, (const void *)k_stage1_3d<1, 1>}};
+        const void *fn2[2][2] = {{(const void *)k_stage1_2d<3, 0>, (const void *)k_stage1_2d<1, 0>},
+                                 {(const void *)k_stage1_2d<3, 1>, (const void *)k_stage1_2d<1, 1>}};
+        const void *fnz[2] = {(const void *)k_stage1z_3d<3, kS1zRows>, (const void *)k_stage1z_3d<1, kS1zRows>};
+        if (lds > 64 * 1024) {
+            for (int k = 0; k < 2; ++k)
+                HIP_TRY(hipFuncSetAttribute(dim == 3 ? fn[exact][k] : fn2[exact][k],
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            if (dim == 3 && !exact)
+                for (int k = 0; k < 2; ++k)
+                    HIP_TRY(hipFuncSetAttribute(fnz[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
     }
     return MIA_HIP_OK;
 }
@@ -782,25 +893,10 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->max_cx = max_cx;
     c->max_ncx = max_ncx;
     c->n_zcell = n_zcell;
-    // stage 1 keeps one x row (plus tables and, in exact mode, a coefficient
-    // window) in dynamic LDS; allow more than the 64 KiB default where needed
-    for (int exact = 0; exact < 2; ++exact) {
-        const size_t lds = stage1_layout(dim, max_nx, max_cx, max_ncx, exact != 0).bytes;
-        if (lds > 160 * 1024)
-            return fail(MIA_HIP_ERR_LIMIT, "stage-1 LDS requirement exceeds 160 KiB");
-        const void *fn[2][2] = {{(const void *)k_stage1_3d<3, 0>, (const void *)k_stage1_3d<1, 0>},
-                                {(const void *)k_stage1_3d<3, 1>, (const void *)k_stage1_3d<1, 1>}};
-        const void *fn2[2][2] = {{(const void *)k_stage1_2d<3, 0>, (const void *)k_stage1_2d<1, 0>},
-                                 {(const void *)k_stage1_2d<3, 1>, (const void *)k_stage1_2d<1, 1>}};
-        const void *fnz[2] = {(const void *)k_stage1z_3d<3, kS1zRows>, (const void *)k_stage1z_3d<1, kS1zRows>};
-        if (lds > 64 * 1024) {
-            for (int k = 0; k < 2; ++k)
-                HIP_TRY(hipFuncSetAttribute(dim == 3 ? fn[exact][k] : fn2[exact][k],
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            if (dim == 3 && !exact)
-                for (int k = 0; k < 2; ++k)
-                    HIP_TRY(hipFuncSetAttribute(fnz[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        }
+    {
+        const int st = set_stage1_lds(dim, max_nx, max_cx, max_ncx);
+        if (st)
+            return st;
     }
     int st;
     if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_wgroups, wg.size())) ||
@@ -983,7 +1079,8 @@ int mia_hip_warp(mia_hip_ctx *c, int inst, const double *params_inst, float *out
                            hipMemcpyHostToDevice, c->stream));
     const auto &G = c->wgroups[c->inst_group[inst]];
     const int boff = G.prefix[c->inst_slot[inst]];
-    int st = launch_warp(c, c->d_params, c->inst_group[inst], boff, G.prefix[c->inst_slot[inst] + 1] - boff, false);
+    int st = launch_warp(plan_of(c), c->stream, nullptr, c->d_params, c->inst_group[inst], boff,
+                         G.prefix[c->inst_slot[inst] + 1] - boff);
     if (st)
         return st;
     HIP_TRY(hipMemcpyAsync(out, c->d_W + d.img_off, sizeof(float) * (size_t)d.nx * d.ny * d.nz, hipMemcpyDeviceToHost,
@@ -1395,4 +1492,220 @@ int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const ch
         return mia_hip_load_warp_tuning(device, json_path);
     }
     return MIA_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free evaluation of one instance
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Scratch memory of mia_hip_cost_grad_noalloc: one array per GPU, held until
+// the process ends.  In host code the array's name is a stand-in, not a
+// device address: each device's address comes from hipGetSymbolAddress, taken
+// once with that device current and kept per device; host code passes only
+// that address on.
+static __device__ __align__(256) unsigned char scratch[MIA_HIP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+
+struct ScratchAddr {
+    std::mutex m;
+    unsigned char *ptr[kMaxScratchDevices] = {};
+};
+
+ScratchAddr &scratch_addr()
+{
+    static ScratchAddr s;
+    return s;
+}
+
+// the device's scratch address (device current, 0 <= device < kMaxScratchDevices)
+int scratch_ptr(int device, unsigned char **out)
+{
+    ScratchAddr &s = scratch_addr();
+    std::lock_guard<std::mutex> lock(s.m);
+    if (!s.ptr[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return fail(MIA_HIP_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch memory");
+        s.ptr[device] = (unsigned char *)p;
+    }
+    *out = s.ptr[device];
+    return MIA_HIP_OK;
+}
+
+} // namespace
+
+int mia_hip_cost_grad_noalloc(int device, int dim, const uint32_t *size, const uint32_t *csize, int degree, int norm,
+                              float autothresh, double weight, int grad_mode, const float *d_moving,
+                              const float *d_fixed, const double *d_params, double *d_grad, double *d_cost,
+                              void *stream)
+{
+    if (dim != 2 && dim != 3)
+        return fail(MIA_HIP_ERR_ARG, "dim must be 2 or 3");
+    if (!size || !csize || !d_moving || !d_fixed || !d_params || !d_grad || !d_cost)
+        return fail(MIA_HIP_ERR_ARG, "null pointer");
+    if (degree != 1 && degree != 3)
+        return fail(MIA_HIP_ERR_ARG, "image interpolation kernel must be bspline:d=1 or bspline:d=3");
+    if (!(autothresh >= 0.0f && autothresh <= 1000.0f))
+        return fail(MIA_HIP_ERR_ARG, "autothresh must be in [0, 1000]");
+    if (grad_mode != MIA_HIP_GRAD_REFERENCE && grad_mode != MIA_HIP_GRAD_EXACT)
+        return fail(MIA_HIP_ERR_ARG, "unknown gradient mode");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count || device >= kMaxScratchDevices)
+        return fail(MIA_HIP_ERR_ARG, "no HIP device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    unsigned char *base = nullptr;
+    int st = scratch_ptr(device, &base);
+    if (st)
+        return st;
+
+    // the instance as mia_hip_setup describes the only instance of a batch
+    InstDesc d{};
+    d.nx = (int)size[0];
+    d.ny = (int)size[1];
+    d.nz = dim == 3 ? (int)size[2] : 1;
+    d.cx = (int)csize[0];
+    d.cy = (int)csize[1];
+    d.cz = dim == 3 ? (int)csize[2] : 1;
+    if (d.nx > kMaxRow)
+        return fail(MIA_HIP_ERR_LIMIT,
+                    "image x size " + std::to_string(d.nx) + " exceeds the limit " + std::to_string(kMaxRow));
+    const long long nvox = (long long)d.nx * d.ny * d.nz;
+    if (dim == 3 && nvox * (long long)sizeof(float) >= (1LL << 31))
+        return fail(MIA_HIP_ERR_LIMIT, "3D image has 2^29 voxels or more (limit of the 32-bit image offsets)");
+    AxisCheck ax, ay, az;
+    if ((st = check_axis(d.nx, d.cx, "x", ax)) || (st = check_axis(d.ny, d.cy, "y", ay)))
+        return st;
+    if (dim == 3) {
+        if ((st = check_axis(d.nz, d.cz, "z", az)))
+            return st;
+    } else {
+        az.nbands = 1;
+        az.band_rows = 1;
+    }
+    d.ncx = ax.nbands;
+    d.ncy = ay.nbands;
+    d.ncz = az.nbands;
+    d.zcell = dim == 3 && d.nx <= kBlock && ay.band_rows + 2 <= kS1zRows && 3LL * d.cx * d.cy * d.cz < (1LL << 31)
+                  ? 1
+                  : 0;
+    d.img_off = d.par_off = d.part_off = 0;
+    d.tab_x = 0;
+    d.tab_y = d.nx;
+    d.tab_z = d.nx + d.ny;
+    d.cell_x = 0;
+    d.cell_y = d.cx + 1;
+    d.cell_z = d.cx + 1 + d.cy + 1;
+    const long long ncoef = (long long)d.cx * d.cy * d.cz;
+    const long long part = dim == 3 ? std::max((long long)d.nz * d.ncy * 12 * d.cx, (long long)d.ncz * d.ncy * 48 * d.cx)
+                                    : (long long)d.ncy * 8 * d.cx;
+    OneInstCounts nc;
+    nc.s1 = dim == 3 ? d.nz * d.ncy : d.ncy;
+    nc.red = (int)((dim * ncoef + (dim == 3 ? 64 : kBlock) - 1) / (dim == 3 ? 64 : kBlock));
+    nc.s1z = d.ncz * d.ncy;
+    nc.redz = (int)((dim * ncoef + kBlock - 1) / kBlock);
+    const int variant = dim == 3 ? tuned_variant(device, degree, nvox) : 0;
+    int w2_win = 0;
+    if (dim == 3) {
+        int tx, ty, tz;
+        w3_tile_dims(variant, tx, ty, tz);
+        nc.tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty) * ((d.nz + tz - 1) / tz);
+    } else {
+        nc.tiles = ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
+        // knots spanned by the tiles of k_warp2d (the window it loads)
+        auto span = [](int n, int cs, int t) {
+            float w[4];
+            int m = 0;
+            for (int p0 = 0; p0 < n; p0 += t)
+                m = std::max(m, spline_axis_entry(std::min(p0 + t, n) - 1, n, cs, w) + 4 -
+                                    spline_axis_entry(p0, n, cs, w));
+            return m;
+        };
+        w2_win = span(d.nx, d.cx, kW2TX) * span(d.ny, d.cy, kW2TY);
+        if (w2_win > kW2Win)
+            return fail(MIA_HIP_ERR_ARG, "2D coefficient window exceeds its bound (spline rate below 1?)");
+    }
+
+    // scratch layout (256-byte aligned pieces); nothing in it needs to start
+    // at zero: every element is written before it is read in each call
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return base + o;
+    };
+    const int ntab = d.nx + d.ny + d.nz, ncell = d.cx + d.cy + d.cz + 3;
+    InstDesc *s_inst = (InstDesc *)take(sizeof(InstDesc));
+    int *s_ints = (int *)take(sizeof(int) * kOneInstInts);
+    float4 *s_tw = (float4 *)take(sizeof(float4) * ntab);
+    int *s_ts = (int *)take(sizeof(int) * ntab);
+    int *s_cells = (int *)take(sizeof(int) * ncell);
+    float2 *s_clamp = (float2 *)take(sizeof(float2));
+    float2 *s_mm = (float2 *)take(sizeof(float2) * 1024);
+    double *s_gscale = (double *)take(sizeof(double));
+    double *s_ssd = (double *)take(sizeof(double) * (nc.s1 + nc.s1z));
+    unsigned *s_cnt = (unsigned *)take(sizeof(unsigned) * (nc.s1 + nc.s1z));
+    float *s_M = (float *)take(sizeof(float) * nvox);
+    float *s_W = (float *)take(sizeof(float) * nvox);
+    float *s_P = (float *)take(sizeof(float) * part);
+    double *s_pf = degree == 3 ? (double *)take(sizeof(double) * nvox) : nullptr;
+    if (off > (size_t)MIA_HIP_NOALLOC_SCRATCH_BYTES)
+        return fail(MIA_HIP_ERR_LIMIT, "allocation-free evaluation needs " + std::to_string(off) +
+                                           " bytes of scratch memory, more than its " +
+                                           std::to_string((size_t)MIA_HIP_NOALLOC_SCRATCH_BYTES));
+    if ((st = set_stage1_lds(dim, d.nx, d.cx, d.ncx)))
+        return st;
+
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_one_inst_setup, dim3(1), dim3(kBlock), 0, s, d, dim, nc, s_inst, s_ints, s_tw, s_ts,
+                       s_cells);
+    LAUNCH_CHECK("one-instance set-up");
+    HIP_TRY(hipMemcpyAsync(s_M, d_moving, sizeof(float) * nvox, hipMemcpyDeviceToDevice, s));
+    if ((st = enqueue_prefilter(d, dim, degree, s_M, s_mm, s_clamp, s_pf, s)))
+        return st;
+
+    EvalPlan pl;
+    pl.dim = dim;
+    pl.degree = degree;
+    pl.n_inst = 1;
+    pl.n_zcell = d.zcell;
+    pl.norm = norm ? 1 : 0;
+    pl.grad_mode = grad_mode;
+    pl.thr = autothresh;
+    pl.weight = weight;
+    pl.max_nx = d.nx;
+    pl.max_cx = d.cx;
+    pl.max_ncx = d.ncx;
+    pl.w2_win = w2_win;
+    pl.s1_blocks = nc.s1;
+    pl.red_blocks = nc.red;
+    pl.s1z_blocks = nc.s1z;
+    pl.redz_blocks = nc.redz;
+    pl.n_warp = 1;
+    pl.warp[0].variant = variant;
+    pl.warp[0].n = 1;
+    pl.warp[0].tiles = nc.tiles;
+    pl.warp[0].ids = s_ints + 8;
+    pl.warp[0].prefix = s_ints + 9;
+    pl.inst = s_inst;
+    pl.s1_prefix = s_ints;
+    pl.red_prefix = s_ints + 2;
+    pl.s1z_prefix = s_ints + 4;
+    pl.redz_prefix = s_ints + 6;
+    pl.M = s_M;
+    pl.R = d_fixed;
+    pl.W = s_W;
+    pl.P = s_P;
+    pl.tw = s_tw;
+    pl.ts = s_ts;
+    pl.cells = s_cells;
+    pl.clamp = s_clamp;
+    pl.ssd = s_ssd;
+    pl.gscale = s_gscale;
+    pl.cnt = s_cnt;
+    return enqueue_plan(pl, s, nullptr, d_params, d_grad, d_cost);
 }
