@@ -91,12 +91,20 @@ int mia_host_nonrigid_register(int device, int dim, const uint32_t *size, const 
  * creator string, size (x, y[, z]), attributes, parameters.  Save creates
  * the transformation from `creator` and `size` and requires n to match its
  * degrees of freedom.  Load re-creates it from the file's creator string and
- * size; outputs (any may be NULL): size, coefficient grid, parameters
- * (params_cap doubles available), their count, the creator string. */
+ * size and restores its attributes; outputs (any may be NULL): size,
+ * coefficient grid, parameters (params_cap doubles available), their count,
+ * the creator string.
+ * Attributes (the transformation's CAttributedData map in string form,
+ * mia/core/attributes.hh:259-349) travel as key/value pairs packed into one
+ * buffer: "key\0value\0key\0value\0\0" (each string NUL-terminated, an empty
+ * key ends the list; keys non-empty).  Save: `attributes` may be NULL (none).
+ * Load: `attributes` (attributes_len bytes, may be NULL) receives the pairs
+ * in key order; MIA_HIP_ERR_SIZE if they do not fit. */
 int mia_host_transform_save(const char *path, int dim, const uint32_t *size, const char *creator,
-                            const double *params, size_t n);
+                            const double *params, size_t n, const char *attributes);
 int mia_host_transform_load(const char *path, int dim, uint32_t *size, uint32_t *coeff_size, double *params,
-                            size_t params_cap, size_t *n_params, char *creator, int creator_len);
+                            size_t params_cap, size_t *n_params, char *creator, int creator_len, char *attributes,
+                            int attributes_len);
 
 #ifdef __cplusplus
 }

@@ -270,7 +270,7 @@ CSplineTransformation::CSplineTransformation(int dim, Size3 range, Vec3 rate, in
 
 CSplineTransformation::CSplineTransformation(const CSplineTransformation &org)
     : m_dim(org.m_dim), m_range(org.m_range), m_rate(org.m_rate), m_img_degree(org.m_img_degree),
-      m_csize(org.m_csize), m_coeffs(org.m_coeffs), m_creator(org.m_creator)
+      m_csize(org.m_csize), m_coeffs(org.m_coeffs), m_creator(org.m_creator), m_attributes(org.m_attributes)
 {
     if (org.m_penalty)
         m_penalty.reset(org.m_penalty->clone());

@@ -171,6 +171,20 @@ public:
     const std::string &get_creator_string() const { return m_creator; }
     void set_creator_string(const std::string &s) { m_creator = s; }
 
+    // Attributes in their string form (CAttributedData,
+    // mia/core/attributes.hh:259-349: set_attribute(key, string),
+    // get_attribute_as_string, the attribute map); the transformation file
+    // stores them as strings (mia/3d/transio/serialization.hh:43-49).
+    void set_attribute(const std::string &key, const std::string &value) { m_attributes[key] = value; }
+    bool has_attribute(const std::string &key) const { return m_attributes.count(key) != 0; }
+    // "" for a key that is not set (get_attribute_as_string of a missing key)
+    std::string get_attribute_as_string(const std::string &key) const
+    {
+        const auto i = m_attributes.find(key);
+        return i == m_attributes.end() ? std::string() : i->second;
+    }
+    const std::map<std::string, std::string> &attributes() const { return m_attributes; }
+
 private:
     void reinit_penalty() const;
     int m_dim;
@@ -182,6 +196,7 @@ private:
     PSplinePenalty m_penalty;
     mutable bool m_penalty_valid = false;
     std::string m_creator;
+    std::map<std::string, std::string> m_attributes;
 };
 using PSplineTransformation = std::shared_ptr<CSplineTransformation>;
 

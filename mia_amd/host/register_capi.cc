@@ -81,13 +81,21 @@ extern "C" int mia_host_nonrigid_register(int device, int dim, const uint32_t *s
 }
 
 extern "C" int mia_host_transform_save(const char *path, int dim, const uint32_t *size, const char *creator,
-                                       const double *params, size_t n)
+                                       const double *params, size_t n, const char *attributes)
 {
     if (!path || (dim != 2 && dim != 3) || !size || !creator || (!params && n))
         return detail::host_fail(MIA_HIP_ERR_ARG, "transform_save: bad arguments");
     try {
         auto t = produce_transform_creator(dim, creator)->create(Size3{size[0], size[1], dim == 3 ? size[2] : 1u});
         t->set_parameters(params, n);
+        // "key\0value\0...\0": pairs up to the empty key
+        for (const char *a = attributes; a && *a;) {
+            const std::string key(a);
+            a += key.size() + 1;
+            const std::string value(a);
+            a += value.size() + 1;
+            t->set_attribute(key, value);
+        }
         save_transformation(path, *t);
         return MIA_HIP_OK;
     } catch (const std::invalid_argument &e) {
@@ -99,7 +107,7 @@ extern "C" int mia_host_transform_save(const char *path, int dim, const uint32_t
 
 extern "C" int mia_host_transform_load(const char *path, int dim, uint32_t *size, uint32_t *coeff_size,
                                        double *params, size_t params_cap, size_t *n_params, char *creator,
-                                       int creator_len)
+                                       int creator_len, char *attributes, int attributes_len)
 {
     if (!path || (dim != 2 && dim != 3))
         return detail::host_fail(MIA_HIP_ERR_ARG, "transform_load: bad arguments");
@@ -126,6 +134,18 @@ extern "C" int mia_host_transform_load(const char *path, int dim, uint32_t *size
         if (creator && creator_len > 0) {
             std::strncpy(creator, t->get_creator_string().c_str(), (size_t)creator_len - 1);
             creator[creator_len - 1] = 0;
+        }
+        if (attributes) {
+            std::string packed;
+            for (const auto &kv : t->attributes()) {
+                packed.append(kv.first).push_back('\0');
+                packed.append(kv.second).push_back('\0');
+            }
+            packed.push_back('\0');
+            if (attributes_len < 0 || packed.size() > (size_t)attributes_len)
+                return detail::host_fail(MIA_HIP_ERR_SIZE, "transform_load: attributes buffer too small (" +
+                                                               std::to_string(packed.size()) + " bytes needed)");
+            std::memcpy(attributes, packed.data(), packed.size());
         }
         return MIA_HIP_OK;
     } catch (const std::invalid_argument &e) {

@@ -13,6 +13,7 @@
 namespace mia_amd {
 
 namespace {
+// the five predefined XML entities (what a Boost XML archive escapes)
 std::string xml_escape(const std::string &s)
 {
     std::string r;
@@ -23,24 +24,42 @@ std::string xml_escape(const std::string &s)
             r += "&lt;";
         else if (c == '>')
             r += "&gt;";
+        else if (c == '"')
+            r += "&quot;";
+        else if (c == '\'')
+            r += "&apos;";
         else
             r += c;
     }
     return r;
 }
 
+// inverse of xml_escape; an '&' without its ';' or an unknown entity is a
+// malformed file
 std::string xml_unescape(const std::string &s)
 {
+    static const char *const names[5][2] = {
+        {"&amp;", "&"}, {"&lt;", "<"}, {"&gt;", ">"}, {"&quot;", "\""}, {"&apos;", "'"}};
     std::string r;
     for (size_t i = 0; i < s.size(); ++i) {
-        if (s[i] == '&') {
-            const size_t e = s.find(';', i);
-            const std::string ent = s.substr(i, e - i + 1);
-            r += ent == "&amp;" ? "&" : ent == "&lt;" ? "<" : ent == "&gt;" ? ">" : ent;
-            i = e;
-        } else {
+        if (s[i] != '&') {
             r += s[i];
+            continue;
         }
+        const size_t e = s.find(';', i);
+        if (e == std::string::npos)
+            throw std::runtime_error("load_transformation: malformed file, '&' without ';'");
+        const std::string ent = s.substr(i, e - i + 1);
+        bool known = false;
+        for (const auto &n : names)
+            if (ent == n[0]) {
+                r += n[1];
+                known = true;
+                break;
+            }
+        if (!known)
+            throw std::runtime_error("load_transformation: malformed file, unknown entity " + ent);
+        i = e;
     }
     return r;
 }
@@ -96,8 +115,19 @@ void save_transformation(const std::string &path, const CSplineTransformation &t
       << "<size_y>" << t.get_size().y << "</size_y>\n";
     if (t.get_dim() == 3)
         f << "<size_z>" << t.get_size().z << "</size_z>\n";
+    // attributes as strings, the std::map<string, string> of
+    // serialization.hh:43-49: count, item_version, then (first, second) items
+    const auto &attr = t.attributes();
     f << "<attributes class_id=\"0\" tracking_level=\"0\" version=\"0\">\n"
-      << "\t<count>0</count>\n\t<item_version>0</item_version>\n</attributes>\n"
+      << "\t<count>" << attr.size() << "</count>\n\t<item_version>0</item_version>\n";
+    bool first_item = true;
+    for (const auto &kv : attr) {
+        f << (first_item ? "\t<item class_id=\"1\" tracking_level=\"0\" version=\"0\">\n" : "\t<item>\n")
+          << "\t\t<first>" << xml_escape(kv.first) << "</first>\n"
+          << "\t\t<second>" << xml_escape(kv.second) << "</second>\n\t</item>\n";
+        first_item = false;
+    }
+    f << "</attributes>\n"
       << "<params>\n\t<count>" << p.size() << "</count>\n\t<item_version>0</item_version>\n";
     f << std::setprecision(17);
     for (double v : p)
@@ -121,6 +151,28 @@ PSplineTransformation load_transformation(const std::string &path, int dim)
         throw std::runtime_error("load_transformation: '" + path + "' is not a transformation file");
     if (dim == 3 && !element(x, "size_z", pos, sz))
         throw std::runtime_error("load_transformation: '" + path + "' has no size_z (a 2D transformation?)");
+    // attributes (serialization.hh:66-77: read as strings, restored after
+    // the parameters)
+    std::vector<std::pair<std::string, std::string>> attr;
+    {
+        std::string block, acount, item;
+        size_t ap = pos;
+        if (!element(x, "attributes", ap, block))
+            throw std::runtime_error("load_transformation: '" + path + "' has no attributes");
+        pos = ap;
+        size_t bp = 0;
+        if (!element(block, "count", bp, acount))
+            throw std::runtime_error("load_transformation: attributes without count in '" + path + "'");
+        while (element(block, "item", bp, item)) {
+            size_t ip = 0;
+            std::string k, v;
+            if (!element(item, "first", ip, k) || !element(item, "second", ip, v))
+                throw std::runtime_error("load_transformation: malformed attribute in '" + path + "'");
+            attr.emplace_back(xml_unescape(k), xml_unescape(v));
+        }
+        if (attr.size() != std::stoul(acount))
+            throw std::runtime_error("load_transformation: attribute count and items differ in '" + path + "'");
+    }
     if (!element(x, "params", pos, params))
         throw std::runtime_error("load_transformation: '" + path + "' has no params");
     size_t pp = 0;
@@ -139,6 +191,8 @@ PSplineTransformation load_transformation(const std::string &path, int dim)
                                  std::to_string(t->degrees_of_freedom()) + " parameters, got " +
                                  std::to_string(v.size()));
     t->set_parameters(v);
+    for (const auto &kv : attr)
+        t->set_attribute(kv.first, kv.second);
     return t;
 }
 

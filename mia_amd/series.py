@@ -42,10 +42,11 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_uint32), _vp, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_double)]),
     "mia_host_transform_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
-                                               ctypes.c_char_p, _vp, ctypes.c_size_t]),
+                                               ctypes.c_char_p, _vp, ctypes.c_size_t, ctypes.c_char_p]),
     "mia_host_transform_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(ctypes.c_uint32), _vp, ctypes.c_size_t,
-                                               ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_int]),
+                                               ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_int,
+                                               ctypes.c_char_p, ctypes.c_int]),
 }
 
 
@@ -189,28 +190,52 @@ def nonrigid_register(src, ref, transform="spline:rate=8", cost="image:cost=[ssd
     return params[:n.value].copy(), tuple(cs), st, fc.value
 
 
-def save_transform(path, creator, shape, params):
+def _pack_attributes(attributes):
+    """{key: value} -> b"key\\0value\\0...\\0" (keys non-empty, no NUL inside)."""
+    out = b""
+    for k, v in sorted((attributes or {}).items()):
+        k, v = str(k).encode(), str(v).encode()
+        if not k or b"\0" in k or b"\0" in v:
+            raise ValueError("attribute keys must be non-empty and keys and values free of NUL")
+        out += k + b"\0" + v + b"\0"
+    return out + b"\0"
+
+
+def _unpack_attributes(buf):
+    parts = buf.split(b"\0")
+    out = {}
+    k = 0
+    while k + 1 < len(parts) and parts[k]:
+        out[parts[k].decode()] = parts[k + 1].decode()
+        k += 2
+    return out
+
+
+def save_transform(path, creator, shape, params, attributes=None):
     """Write a spline transformation file (MIA's xml layout).  shape: numpy
-    image shape; creator: e.g. "spline:rate=8"."""
+    image shape; creator: e.g. "spline:rate=8"; attributes: {str: str}."""
     L = load()
     dim = len(shape)
     sz = np.array(list(shape[::-1]) + [1] * (3 - dim), dtype=np.uint32)
     p = np.ascontiguousarray(params, dtype=np.float64)
     _check(L.mia_host_transform_save(str(path).encode(), dim, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-                                     creator.encode(), p.ctypes.data, p.size))
+                                     creator.encode(), p.ctypes.data, p.size, _pack_attributes(attributes)))
 
 
-def load_transform(path, dim):
+def load_transform(path, dim, with_attributes=False):
     """Read a spline transformation file; returns (creator, numpy shape,
-    coefficient grid (x,y,z), params)."""
+    coefficient grid (x,y,z), params[, attributes {str: str}])."""
     L = load()
     sz = (ctypes.c_uint32 * 3)()
     cs = (ctypes.c_uint32 * 3)()
     n = ctypes.c_size_t()
     cr = ctypes.create_string_buffer(1024)
-    _check(L.mia_host_transform_load(str(path).encode(), int(dim), sz, cs, None, 0, ctypes.byref(n), None, 0))
+    _check(L.mia_host_transform_load(str(path).encode(), int(dim), sz, cs, None, 0, ctypes.byref(n), None, 0,
+                                     None, 0))
     params = np.zeros(n.value)
+    attr = ctypes.create_string_buffer(1 << 16)
     _check(L.mia_host_transform_load(str(path).encode(), int(dim), sz, cs, params.ctypes.data, params.size,
-                                     ctypes.byref(n), cr, 1024))
+                                     ctypes.byref(n), cr, 1024, attr, len(attr)))
     shape = tuple(int(v) for v in list(sz)[:dim][::-1])
-    return cr.value.decode(), shape, tuple(cs), params
+    out = (cr.value.decode(), shape, tuple(cs), params)
+    return out + (_unpack_attributes(attr.raw),) if with_attributes else out

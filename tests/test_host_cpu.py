@@ -93,3 +93,36 @@ def test_transform_file_round_trip(tmp_path):
     (tmp_path / "cut.x3dt").write_text(t)
     with pytest.raises(RuntimeError):
         series.load_transform(tmp_path / "cut.x3dt", 3)
+
+
+def test_transform_file_attributes(tmp_path):
+    """Attributes are written as the string map of
+    mia/3d/transio/serialization.hh:43-49 (count, item_version, first/second
+    items, XML-escaped) and restored on load (:66-77); malformed entities
+    raise instead of looping."""
+    import numpy as np
+    import mia_amd
+    from mia_amd import series
+    shape = (30, 34, 38)
+    cs = [mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1]]
+    p = np.linspace(-1, 1, 3 * int(np.prod(cs))).astype(np.float32).astype(np.float64)
+    attr = {"affine": "1 0 0 0 1 0 0 0 1", "note": "a<b & \"c\" 'd'>", "z": ""}
+    f = tmp_path / "a.x3dt"
+    series.save_transform(f, "spline:rate=8", shape, p, attr)
+    text = f.read_text()
+    assert "<count>3</count>" in text
+    assert "<first>note</first>" in text and "a&lt;b &amp; &quot;c&quot; &apos;d&apos;&gt;" in text
+    cr, sh, grid, q, back = series.load_transform(f, 3, with_attributes=True)
+    assert back == attr and np.array_equal(q, p)
+    # no attributes: an empty map, as before
+    series.save_transform(tmp_path / "e.x3dt", "spline:rate=8", shape, p)
+    assert series.load_transform(tmp_path / "e.x3dt", 3, with_attributes=True)[4] == {}
+    # '&' without ';' (once looped forever), an unknown entity, a bad count
+    for bad in (text.replace("&amp;", "&x", 1), text.replace("&amp;", "&bogus;", 1),
+                text.replace("<count>3</count>", "<count>4</count>", 1)):
+        (tmp_path / "bad.x3dt").write_text(bad)
+        with pytest.raises(RuntimeError):
+            series.load_transform(tmp_path / "bad.x3dt", 3)
+    (tmp_path / "c.x3dt").write_text(text.replace("<creator>spline:rate=8", "<creator>spline:rate=8&x"))
+    with pytest.raises(RuntimeError):
+        series.load_transform(tmp_path / "c.x3dt", 3)
