@@ -147,7 +147,7 @@ def cpu_baseline(mov, fix, cs, params, threads):
     return statistics.median(times), t1, cost, g.copy(), tm.copy()
 
 
-def series_cpu_baseline(frames, maxeval, threads):
+def series_cpu_baseline(frames, maxeval, threads, rate=8.0):
     """CPU baseline of the series: the FP32 port of MIA's 2D evaluation with
     the same L-BFGS, frames in a pfor over `threads` workers
     (src/2dmany2one-nonrigid.cc:185).  Returns (seconds, costs)."""
@@ -162,7 +162,7 @@ def series_cpu_baseline(frames, maxeval, threads):
     movs = np.ascontiguousarray(np.concatenate([m.ravel() for m, _ in frames]), dtype=np.float32)
     fixs = np.ascontiguousarray(np.concatenate([f.ravel() for _, f in frames]), dtype=np.float32)
     costs = np.zeros(n)
-    sec = L.mcp_series_2d(n, sizes, movs, fixs, 8.0, maxeval, threads, costs.ctypes.data, None)
+    sec = L.mcp_series_2d(n, sizes, movs, fixs, float(rate), maxeval, threads, costs.ctypes.data, None)
     return sec, costs
 
 
@@ -275,6 +275,52 @@ def batched_3d(steps, n_vol=16):
             "voxels": nvox, "ms_per_evaluation": round(ms, 4), "mvoxel_per_s": round(nvox / ms / 1e3, 1),
             "tile_per_bucket": dict(sorted(buckets.items(), key=lambda kv: int(kv[0]))),
             "tuning_table": os.path.relpath(src, ROOT) if src else None}
+
+
+def registration_2d(reps=5):
+    """Plain 2D registration (single level): 128x128 Gaussian-blob fixed
+    image, moving image warped by a seeded cubic B-spline displacement of knot
+    spacing 16 (coefficients N(0, 1.5) clipped to 3 px); `spline:rate=16`
+    (10 x 10 x 2 = 200 parameters), `image:cost=[ssd]`, L-BFGS with 50
+    evaluations, one level (src/2dimageregistration.cc:46-106 with one
+    multiresolution level).  Wall time of the whole registration
+    (normalisation, set-up, minimisation) through the multiresolution driver
+    and of the minimisation loop alone with the on-device L-BFGS; the CPU
+    port (FP32 MIA data flow, same L-BFGS, one thread) on the same inputs.
+    Medians of `reps` runs."""
+    from mia_amd import series, synth
+    mov, fix, cs, _ = synth.blob_pair_2d()
+    a, b = series.normalize_pair(mov, fix)
+    c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
+    walls, finals = [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        params, grid, stats, final = series.nonrigid_register(mov, fix, "spline:rate=16", "image:cost=[ssd]",
+                                                              "lbfgs:maxiter=50", levels=1)
+        walls.append(time.perf_counter() - t0)
+        finals.append(final)
+    dev = []
+    with series.SeriesRegistration([(mov, fix)], rate=16.0) as s:
+        s.run("lbfgs:maxiter=3", series.DEVICE_BATCHED)
+        for _ in range(reps):
+            p_d, c_d, e_d, sec = s.run("lbfgs:maxiter=50", series.DEVICE_BATCHED)
+            dev.append(sec)
+        _, c_h, e_h, sec_h = s.run("lbfgs:maxiter=50", series.PER_FRAME)
+    cpu = [series_cpu_baseline([(mov, fix)], 50, 1, rate=16.0) for _ in range(reps)]
+    cpu_sec = statistics.median([c[0] for c in cpu])
+    wall = statistics.median(walls[1:])
+    log("registration 2D 128^2: driver %.4f s, device loop %.5f s, cpu port %.4f s, final %.6g (identity %.6g)"
+        % (wall, statistics.median(dev), cpu_sec, finals[-1], c_id))
+    return {"workload": "2D 128x128 Gaussian blobs, spline rate 16 (10x10 x2), ssd, lbfgs maxiter=50, 1 level",
+            "seconds_driver": round(wall, 5), "seconds_device_loop": round(statistics.median(dev), 6),
+            "regs_per_sec_device_loop": round(1.0 / statistics.median(dev), 1),
+            "final_cost_driver": finals[-1], "final_cost_device": float(c_d[0]), "final_cost_host": float(c_h[0]),
+            "evaluations_device": int(e_d[0]), "identity_cost": c_id,
+            "cpu_port": {"seconds": round(cpu_sec, 5), "threads": 1, "kind": "port",
+                         "final_cost": float(cpu[-1][1][0]),
+                         "sample": "the same registration, FP32 port of MIA's 2D evaluation with the same L-BFGS"},
+            "speedup_driver_vs_cpu_port": round(cpu_sec / wall, 2),
+            "speedup_device_loop_vs_cpu_port": round(cpu_sec / statistics.median(dev), 1)}
 
 
 def registration_3d(threads):
@@ -500,8 +546,9 @@ def main():
     bat = None
     if not args.no_batched:
         bat = batched_3d(20)
-    reg = None
+    reg = reg2 = None
     if not args.no_registration:
+        reg2 = registration_2d()
         reg = registration_3d(args.cpu_threads or min(16, os.cpu_count() or 1))
 
     out = {
@@ -528,6 +575,7 @@ def main():
         "parity_vs_cpu_port": parity,
         "series_2d": ser,
         "batched_3d": bat,
+        "registration_2d": reg2,
         "registration_3d": reg,
     }
     for c in ctxs:

@@ -224,6 +224,20 @@ int mia_hip_divcurl_create(int device, const uint32_t *csize, const float *range
 int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs, double *value, double *gradient);
 int mia_hip_divcurl_destroy(mia_hip_divcurl *p);
 
+/* Attach penalty p (a divcurl object created for the coefficient grid of 3D
+ * instance `inst`) to a context, or detach with p = NULL.  From then on every
+ * evaluation of the context (mia_hip_cost_grad, _device, the captured graph,
+ * mia_hip_optimize) adds weight * c^T P c to the instance's cost and
+ * -weight * d(c^T P c)/dc to its gradient on the device, c being the
+ * parameters rounded to float: the penalty term of
+ * TNonrigRegGradientProblem::evaluate_fdf (mia/template/nonrigidregister.cxx:
+ * 422-429) with C3DSplineTransformPenalty::value_and_gradient's weight and
+ * sign (mia/3d/splinetransformpenalty.cc:52-71; weight already divided by the
+ * range's volume for normalize=1).  The context does not own p: it must
+ * outlive the attachment, and evaluations of one penalty object must not
+ * overlap (its partial sums live in the object). */
+int mia_hip_set_penalty(mia_hip_ctx *ctx, int inst, mia_hip_divcurl *p, double weight);
+
 /* ---- allocation-free evaluation ----------------------------------------- */
 
 /* Size of the scratch memory of mia_hip_cost_grad_noalloc: one array of this

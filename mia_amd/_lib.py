@@ -66,6 +66,7 @@ SIGNATURES = {
                                               ctypes.POINTER(_vp)]),
     "mia_hip_divcurl_evaluate": (ctypes.c_int, [_vp, _flt_p, _dbl_p, _vp]),
     "mia_hip_divcurl_destroy": (ctypes.c_int, [_vp]),
+    "mia_hip_set_penalty": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_double]),
     "mia_hip_cost_grad_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32_p, _u32_p, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_int,
                                                  _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -98,6 +98,14 @@ class HipContext:
         check(self._L.mia_hip_set_images_device(self._h, int(inst), ctypes.c_void_p(d_moving),
                                                 ctypes.c_void_p(d_fixed)))
 
+    def set_penalty(self, inst, penalty, weight):
+        """Attach a level.DivcurlPenalty (grid = the instance's) evaluated with
+        every evaluation (weight * value added to the cost, -weight * its
+        gradient to the gradient), or detach with penalty=None."""
+        h = penalty._h if penalty is not None else None
+        check(self._L.mia_hip_set_penalty(self._h, int(inst), h, float(weight)))
+        self._penalty = penalty  # keep it alive while attached
+
     def set_cost(self, norm=False, autothresh=0.0, weight=1.0, grad_mode=GRAD_REFERENCE):
         check(self._L.mia_hip_set_cost(self._h, int(bool(norm)), float(autothresh), float(weight), int(grad_mode)))
 

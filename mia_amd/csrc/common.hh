@@ -9,6 +9,17 @@
 
 namespace mia_hip_rt {
 int fail(int code, const std::string &msg);
+
+// The divcurl penalty object's coefficient grid (x, y, z).
+int divcurl_grid(const mia_hip_divcurl *p, uint32_t cs[3]);
+
+// Enqueue on stream s: the penalty of the double parameters `params` (one
+// instance's, rounded to float as the transformation stores them), its
+// weighted value added to *cost and -weight times its gradient added to
+// grad.  Uses the penalty object's partial-sum buffer: evaluations of one
+// penalty object must not overlap.
+int divcurl_enqueue_add(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *grad,
+                        double *cost);
 } // namespace mia_hip_rt
 
 #define HIP_TRY(expr)                                                                                   \

@@ -188,11 +188,28 @@ struct PPCell {
 constexpr int kPK = 4;                  // kernel support of bspline:d=3
 constexpr int kPW = 2 * kPK + 1;        // cells per axis
 
+// A coefficient component as the penalty reads it: the transformation keeps
+// float coefficients (the double parameters rounded to float,
+// mia/3d/transform/spline.cc:344-358), used in double.
+__device__ __forceinline__ double coef_value(const float *c, long long k)
+{
+    return (double)c[k];
+}
+
+__device__ __forceinline__ double coef_value(const double *c, long long k)
+{
+    return (double)(float)c[k];
+}
+
 // One thread per coefficient i: g_i and the row sum of the quadratic form
 // over the neighbours j with i - 4 <= j <= i + 3 per axis (the reference's
 // loop bounds, ppmatrix.cc:330-345); per-block partial sums in fixed order.
-__global__ void __launch_bounds__(kLB) k_divcurl(const float *__restrict__ coef, const PPCell *__restrict__ cells,
-                                                 int nx, int ny, int nz, double *__restrict__ grad,
+// ADD: grad[3i + k] += -w * g_k (the penalty's weighted gradient with the
+// reference's sign, splinetransformpenalty.cc:60-71, added to a gradient
+// already there); otherwise grad = g (unweighted, may be NULL).
+template <typename T, bool ADD>
+__global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, const PPCell *__restrict__ cells,
+                                                 int nx, int ny, int nz, double w, double *__restrict__ grad,
                                                  double *__restrict__ part)
 {
     __shared__ PPCell P[kPW * kPW * kPW];
@@ -205,17 +222,20 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const float *__restrict__ coef,
     double v = 0.0;
     if (i < n) {
         const int xi = (int)(i % nx), yi = (int)((i / nx) % ny), zi = (int)(i / ((long long)nx * ny));
-        const double cix = coef[3 * i], ciy = coef[3 * i + 1], ciz = coef[3 * i + 2];
+        const double cix = coef_value(coef, 3 * i), ciy = coef_value(coef, 3 * i + 1),
+                     ciz = coef_value(coef, 3 * i + 2);
         double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, g0 = 0, g1 = 0, g2 = 0;
         for (int zj = max(0, zi - kPK); zj < min(zi + kPK, nz); ++zj) {
             const int dz = zi - zj + kPK;
             for (int yj = max(0, yi - kPK); yj < min(yi + kPK, ny); ++yj) {
                 const int dy = yi - yj + kPK;
                 const int xs = max(0, xi - kPK), xe = min(xi + kPK, nx);
-                const float *cj = coef + 3 * (((long long)zj * ny + yj) * nx + xs);
+                const long long j0 = 3 * (((long long)zj * ny + yj) * nx + xs);
                 const PPCell *p = P + (dz * kPW + dy) * kPW + (xi - xs + kPK);
-                for (int xj = xs; xj < xe; ++xj, cj += 3, --p) {
-                    const double cx = cj[0], cy = cj[1], cz = cj[2];
+                for (int xj = xs; xj < xe; ++xj, --p) {
+                    const long long j = j0 + 3LL * (xj - xs);
+                    const double cx = coef_value(coef, j), cy = coef_value(coef, j + 1),
+                                 cz = coef_value(coef, j + 2);
                     const double xx = cx * p->vxx, yy = cy * p->vyy, zz = cz * p->vzz;
                     const double yxy = cy * p->vxy, xxz = cx * p->vxz, zyz = cz * p->vyz;
                     r0 += cix * xx;
@@ -230,7 +250,14 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const float *__restrict__ coef,
                 }
             }
         }
-        if (grad) {
+        if (ADD) {
+            // the product rounded before the sum, as the host adds the
+            // penalty's gradient vector (no fused multiply-add)
+#pragma clang fp contract(off)
+            grad[3 * i] = grad[3 * i] + -w * g0;
+            grad[3 * i + 1] = grad[3 * i + 1] + -w * g1;
+            grad[3 * i + 2] = grad[3 * i + 2] + -w * g2;
+        } else if (grad) {
             grad[3 * i] = g0;
             grad[3 * i + 1] = g1;
             grad[3 * i + 2] = g2;
@@ -244,13 +271,17 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const float *__restrict__ coef,
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int w = 0; w < kLB / 64; ++w)
-            s += red[w];
+        for (int wv = 0; wv < kLB / 64; ++wv)
+            s += red[wv];
         part[blockIdx.x] = s;
     }
 }
 
-__global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ part, int n, double *__restrict__ out)
+// Sum of the block partials in a fixed order: *out = sum, or with ADD
+// *out = *out + w * sum (the weighted penalty added to a cost already there).
+template <bool ADD>
+__global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ part, int n, double w,
+                                                   double *__restrict__ out)
 {
     __shared__ double red[kLB / 64];
     double s = 0.0;
@@ -263,9 +294,14 @@ __global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ pa
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-        for (int w = 0; w < kLB / 64; ++w)
-            t += red[w];
-        *out = t;
+        for (int wv = 0; wv < kLB / 64; ++wv)
+            t += red[wv];
+        if (ADD) {
+#pragma clang fp contract(off)
+            *out = *out + w * t;
+        } else {
+            *out = t;
+        }
     }
 }
 
@@ -573,10 +609,12 @@ extern "C" int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs,
         return fail(MIA_HIP_ERR_ARG, "divcurl_evaluate: null argument");
     HIP_TRY(hipSetDevice(p->device));
     HIP_TRY(hipMemcpyAsync(p->d_coef, coeffs, 3 * p->n * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    hipLaunchKernelGGL(k_divcurl, dim3(p->nblocks), dim3(kLB), 0, p->stream, p->d_coef, p->d_cells, (int)p->cs[0],
-                       (int)p->cs[1], (int)p->cs[2], gradient ? p->d_grad : nullptr, p->d_part);
+    hipLaunchKernelGGL((k_divcurl<float, false>), dim3(p->nblocks), dim3(kLB), 0, p->stream,
+                       (const float *)p->d_coef, (const PPCell *)p->d_cells, (int)p->cs[0], (int)p->cs[1],
+                       (int)p->cs[2], 0.0, gradient ? p->d_grad : nullptr, p->d_part);
     LAUNCH_CHECK("divcurl");
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kLB), 0, p->stream, p->d_part, p->nblocks, p->d_value);
+    hipLaunchKernelGGL(k_sum_parts<false>, dim3(1), dim3(kLB), 0, p->stream, (const double *)p->d_part, p->nblocks,
+                       0.0, p->d_value);
     LAUNCH_CHECK("divcurl sum");
     HIP_TRY(hipMemcpyAsync(value, p->d_value, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     if (gradient)
@@ -584,3 +622,28 @@ extern "C" int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs,
     HIP_TRY(hipStreamSynchronize(p->stream));
     return MIA_HIP_OK;
 }
+
+namespace mia_hip_rt {
+
+int divcurl_grid(const mia_hip_divcurl *p, uint32_t cs[3])
+{
+    if (!p)
+        return fail(MIA_HIP_ERR_ARG, "null divcurl penalty");
+    std::copy(p->cs, p->cs + 3, cs);
+    return MIA_HIP_OK;
+}
+
+int divcurl_enqueue_add(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *grad,
+                        double *cost)
+{
+    hipLaunchKernelGGL((k_divcurl<double, true>), dim3(p->nblocks), dim3(kLB), 0, s, params,
+                       (const PPCell *)p->d_cells, (int)p->cs[0], (int)p->cs[1], (int)p->cs[2], weight, grad,
+                       p->d_part);
+    LAUNCH_CHECK("divcurl");
+    hipLaunchKernelGGL(k_sum_parts<true>, dim3(1), dim3(kLB), 0, s, (const double *)p->d_part, p->nblocks, weight,
+                       cost);
+    LAUNCH_CHECK("divcurl sum");
+    return MIA_HIP_OK;
+}
+
+} // namespace mia_hip_rt

@@ -169,6 +169,10 @@ struct mia_hip_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
+    // spline penalties evaluated with every evaluation, per instance (none: null)
+    std::vector<mia_hip_divcurl *> penalty;
+    std::vector<double> penalty_weight;
+
     bool profile = false;
     struct Ev {
         int stage;
@@ -559,7 +563,31 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
 
 int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
 {
-    return enqueue_plan(plan_of(c), c->stream, timed ? c : nullptr, d_params, d_grad, d_costs);
+    int st = enqueue_plan(plan_of(c), c->stream, timed ? c : nullptr, d_params, d_grad, d_costs);
+    if (st)
+        return st;
+    // attached penalties, after the image cost has written cost and gradient
+    for (int i = 0; i < (int)c->penalty.size(); ++i)
+        if (c->penalty[i]) {
+            const InstDesc &d = c->inst[i];
+            if ((st = mia_hip_rt::divcurl_enqueue_add(c->penalty[i], c->stream, d_params + d.par_off,
+                                                      c->penalty_weight[i], d_grad + d.par_off, d_costs + i)))
+                return st;
+        }
+    return MIA_HIP_OK;
+}
+
+// A captured graph holds the launches of the settings it was captured with.
+void drop_graph(mia_hip_ctx *c)
+{
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    if (c->graph) {
+        (void)hipGraphDestroy(c->graph);
+        c->graph = nullptr;
+    }
 }
 
 int check_ready(mia_hip_ctx *c)
@@ -743,6 +771,8 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->s1z_prefix.assign(n_inst + 1, 0);
     c->redz_prefix.assign(n_inst + 1, 0);
     c->images_set.assign(n_inst, 0);
+    c->penalty.assign(n_inst, nullptr);
+    c->penalty_weight.assign(n_inst, 0.0);
     std::vector<float> tw;
     std::vector<int> ts, cells;
     long long img = 0, par = 0, part = 0, maxv = 0;
@@ -980,14 +1010,7 @@ int mia_hip_set_cost(mia_hip_ctx *c, int norm, float autothresh, double weight, 
     c->thr = autothresh;
     c->weight = weight;
     c->grad_mode = grad_mode;
-    if (c->gexec) {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
-    if (c->graph) {
-        (void)hipGraphDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graph(c);
     return MIA_HIP_OK;
 }
 
@@ -1002,6 +1025,33 @@ int mia_hip_n_params(mia_hip_ctx *c, int inst, size_t *n, size_t *offset)
         *n = (size_t)c->dim * d.cx * d.cy * d.cz;
     if (offset)
         *offset = (size_t)d.par_off;
+    return MIA_HIP_OK;
+}
+
+int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double weight)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    if (p) {
+        if (c->dim != 3)
+            return fail(MIA_HIP_ERR_ARG, "set_penalty: the divcurl penalty is 3D");
+        uint32_t cs[3];
+        int st = mia_hip_rt::divcurl_grid(p, cs);
+        if (st)
+            return st;
+        const InstDesc &d = c->inst[inst];
+        if ((int)cs[0] != d.cx || (int)cs[1] != d.cy || (int)cs[2] != d.cz)
+            return fail(MIA_HIP_ERR_SIZE, "set_penalty: the penalty's coefficient grid differs from the instance's");
+        if (!std::isfinite(weight))
+            return fail(MIA_HIP_ERR_ARG, "set_penalty: weight must be finite");
+    }
+    c->penalty[inst] = p;
+    c->penalty_weight[inst] = p ? weight : 0.0;
+    drop_graph(c);
     return MIA_HIP_OK;
 }
 

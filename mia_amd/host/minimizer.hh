@@ -324,4 +324,10 @@ private:
 // and sets `err` for unknown names or keys.
 ReverseMinimizer *produce_minimizer(const std::string &descr, size_t n, std::string &err);
 
+// Whether `descr` is an lbfgs the on-device L-BFGS (mia_hip_optimize) runs:
+// keys maxiter, memory, step, gtol = 0, ftolr = 0 and host = 0 (host=1 asks
+// for the host minimizer).  Sets maxeval, memory and step; false otherwise
+// (err set only for a malformed descriptor or a bad value).
+bool device_lbfgs_params(const std::string &descr, int &maxeval, int &memory, double &step, std::string &err);
+
 } // namespace mia_amd

@@ -145,17 +145,20 @@ void CSplinePenalty::initialize(int dim, Size3 coeff_size, Vec3 range)
     do_initialize();
 }
 
-double CSplinePenalty::value(const std::vector<float> &coefficients) const
+double CSplinePenalty::effective_weight() const
 {
     const double vol = (double)m_range[0] * m_range[1] * (m_dim == 3 ? m_range[2] : 1.0f);
-    const double w = m_normalize ? m_weight / vol : m_weight;
-    return w * do_value(coefficients);
+    return m_normalize ? m_weight / vol : m_weight;
+}
+
+double CSplinePenalty::value(const std::vector<float> &coefficients) const
+{
+    return effective_weight() * do_value(coefficients);
 }
 
 double CSplinePenalty::value_and_gradient(const std::vector<float> &coefficients, std::vector<double> &gradient) const
 {
-    const double vol = (double)m_range[0] * m_range[1] * (m_dim == 3 ? m_range[2] : 1.0f);
-    const double w = m_normalize ? m_weight / vol : m_weight;
+    const double w = effective_weight();
     gradient.assign(coefficients.size(), 0.0);
     const double v = w * do_value_and_gradient(coefficients, gradient);
     for (double &g : gradient)
@@ -189,6 +192,11 @@ CSplinePenalty *CDivcurlSplinePenalty::clone() const
     if (m_impl->p)
         r->initialize(m_dim, m_size, m_range);
     return r;
+}
+
+mia_hip_divcurl *CDivcurlSplinePenalty::device_handle() const
+{
+    return m_impl->p;
 }
 
 void CDivcurlSplinePenalty::do_initialize()
@@ -567,6 +575,18 @@ double CSplineTransformation::get_energy_penalty() const
     return m_penalty->value(m_coeffs);
 }
 
+bool CSplineTransformation::device_penalty(mia_hip_divcurl **p, double *weight) const
+{
+    *p = nullptr;
+    *weight = 0.0;
+    if (!m_penalty)
+        return true;
+    reinit_penalty();
+    *p = m_penalty->device_handle();
+    *weight = m_penalty->effective_weight();
+    return *p != nullptr;
+}
+
 double CSplineTransformation::get_energy_penalty_and_gradient(std::vector<double> &gradient) const
 {
     if (!m_penalty) {
@@ -748,21 +768,27 @@ void CImageFullCost::prepare(const CSplineTransformation &t) const
     const uint32_t c3[3] = {cs.x, cs.y, cs.z};
     hip_check(mia_hip_setup(m_ctx, m_dim, 1, sz, c3, t.get_image_degree()), "image cost: setup");
     hip_check(mia_hip_set_images(m_ctx, 0, m_src_scaled.data.data(), m_ref_scaled.data.data()), "image cost: images");
-    hip_check(mia_hip_set_cost(m_ctx, m_cost.norm ? 1 : 0, m_cost.autothresh, 1.0, m_grad_mode), "image cost: cost");
+    hip_check(mia_hip_set_cost(m_ctx, m_cost.norm ? 1 : 0, m_cost.autothresh, m_weight, m_grad_mode),
+              "image cost: cost");
     m_ctx_size = m_current;
     m_ctx_csize = cs;
     m_ctx_degree = t.get_image_degree();
 }
 
+mia_hip_ctx *CImageFullCost::device_context(const CSplineTransformation &t) const
+{
+    prepare(t);
+    return m_ctx;
+}
+
+// the weight is applied on the device (the cost and the gradient's scale)
 double CImageFullCost::evaluate(const CSplineTransformation &t, double *gradient) const
 {
     prepare(t);
     const std::vector<double> p = t.get_parameters();
     double cost = 0.0;
     hip_check(mia_hip_cost_grad(m_ctx, p.data(), gradient, &cost), "image cost: evaluate");
-    for (size_t k = 0; k < p.size(); ++k)
-        gradient[k] *= m_weight;
-    return m_weight * cost;
+    return cost;
 }
 
 double CImageFullCost::evaluate(const CSplineTransformation &t, std::vector<double> &gradient) const
@@ -898,6 +924,40 @@ double CNonrigidRegister::minimize(CNonrigRegGradientProblem &p, std::vector<dou
     return opt->best_f();
 }
 
+bool CNonrigidRegister::minimize_on_device(CSplineTransformation &t, std::vector<double> &x, LevelStats &st,
+                                           double &best) const
+{
+    int maxeval = 0, memory = 0;
+    double step = 0.0;
+    std::string err;
+    if (!device_lbfgs_params(m_minimizer, maxeval, memory, step, err)) {
+        if (!err.empty())
+            throw std::invalid_argument(err);
+        return false;
+    }
+    mia_hip_divcurl *pen = nullptr;
+    double pen_w = 0.0;
+    if (m_costs.size() != 1 || !t.device_penalty(&pen, &pen_w))
+        return false;
+    mia_hip_ctx *ctx = m_costs.at(0)->device_context(t);
+    // the penalty is part of this minimisation's evaluations only
+    struct Detach {
+        mia_hip_ctx *c;
+        ~Detach() { (void)mia_hip_set_penalty(c, 0, nullptr, 0.0); }
+    } detach{ctx};
+    hip_check(mia_hip_set_penalty(ctx, 0, pen, pen_w), "nonrigid register: penalty");
+    std::vector<double> xo(x.size());
+    double cost = 0.0;
+    int evals = 0;
+    hip_check(mia_hip_optimize(ctx, x.data(), maxeval, memory, step, 10, xo.data(), &cost, &evals, nullptr),
+              "nonrigid register: device L-BFGS");
+    if (evals > 0 && std::isfinite(cost))
+        x = xo;
+    st.evaluations += evals;
+    best = cost;
+    return true;
+}
+
 PSplineTransformation CNonrigidRegister::run(const CImage &src_in, const CImage &ref_in)
 {
     if (src_in.size != ref_in.size || src_in.dim != ref_in.dim)
@@ -932,12 +992,14 @@ PSplineTransformation CNonrigidRegister::run(const CImage &src_in, const CImage 
         st.size = local;
         CNonrigRegGradientProblem gp(m_costs, *t);
         std::vector<double> x = t->get_parameters();
-        st.cost = minimize(gp, x, st);
+        if (!minimize_on_device(*t, x, st, st.cost))
+            st.cost = minimize(gp, x, st);
         t->set_parameters(x);
         if (t->refine()) {
             gp.reset_counters();
             x = t->get_parameters();
-            st.cost = minimize(gp, x, st);
+            if (!minimize_on_device(*t, x, st, st.cost))
+                st.cost = minimize(gp, x, st);
             t->set_parameters(x);
         }
         st.coeff_size = t->get_coeff_size();

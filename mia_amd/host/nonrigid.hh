@@ -86,6 +86,11 @@ public:
     virtual CSplinePenalty *clone() const = 0;
     double get_weight() const { return m_weight; }
     bool get_normalize() const { return m_normalize; }
+    // the weight value() applies (divided by the image volume with norm=1)
+    double effective_weight() const;
+    // the penalty's device object once initialized, for evaluation inside a
+    // context (mia_hip_set_penalty); nullptr where there is none
+    virtual mia_hip_divcurl *device_handle() const { return nullptr; }
 
 protected:
     virtual void do_initialize() = 0;
@@ -109,6 +114,7 @@ public:
     CDivcurlSplinePenalty(double weight, bool normalize, double div, double curl, int device = 0);
     ~CDivcurlSplinePenalty() override;
     CSplinePenalty *clone() const override;
+    mia_hip_divcurl *device_handle() const override;
 
 private:
     void do_initialize() override;
@@ -164,6 +170,10 @@ public:
     bool refine();
 
     bool has_energy_penalty() const { return (bool)m_penalty; }
+    // The penalty as a device object for the current coefficient grid and
+    // the weight it is applied with: true with *p = nullptr without a
+    // penalty, false if the penalty has no device form.
+    bool device_penalty(mia_hip_divcurl **p, double *weight) const;
     double get_energy_penalty() const;
     double get_energy_penalty_and_gradient(std::vector<double> &gradient) const;
 
@@ -263,6 +273,9 @@ public:
     double cost_value(const CSplineTransformation &t) const;
     double get_weight() const { return m_weight; }
     int get_dim() const { return m_dim; }
+    // the device context that evaluates this cost for t's size and grid
+    // (images at the current size, weight applied on the device)
+    mia_hip_ctx *device_context(const CSplineTransformation &t) const;
 
 private:
     void prepare(const CSplineTransformation &t) const;
@@ -289,6 +302,7 @@ public:
     void push(PImageFullCost c) { m_costs.push_back(std::move(c)); }
     bool empty() const { return m_costs.empty(); }
     size_t size() const { return m_costs.size(); }
+    const PImageFullCost &at(size_t i) const { return m_costs.at(i); }
     void reinit();
     bool get_full_size(Size3 &size) const;
     void set_size(Size3 size);
@@ -337,6 +351,10 @@ public:
 
 private:
     double minimize(CNonrigRegGradientProblem &p, std::vector<double> &x, LevelStats &st) const;
+    // the same minimisation with the on-device L-BFGS (one image cost, an
+    // optional divcurl penalty, lbfgs without tolerances); false if the
+    // problem does not qualify
+    bool minimize_on_device(CSplineTransformation &t, std::vector<double> &x, LevelStats &st, double &best) const;
     CFullCostList &m_costs;
     std::string m_minimizer;
     const CSplineTransformCreator &m_creator;
@@ -347,9 +365,9 @@ private:
 
 // Transformation files in the layout of MIA's "xml" transformation IO
 // (mia/3d/transio/xml.cc, mia/3d/transio/serialization.hh:36-79; 2D
-// without size_z): creator string, size, attributes (none written, ignored
-// on load) and the parameter vector.  load_transformation re-creates the
-// transformation from its creator string and size and sets the parameters;
+// without size_z): creator string, size, attributes (the string map,
+// restored on load) and the parameter vector.  load_transformation re-creates
+// the transformation from its creator string and size and sets the parameters;
 // it throws std::runtime_error on a malformed file or a parameter count that
 // does not match the creator (as the reference's loaders do).
 void save_transformation(const std::string &path, const CSplineTransformation &t);

@@ -80,6 +80,8 @@ ReverseMinimizer *produce_minimizer(const std::string &descr, size_t n, std::str
                     ftolr = std::stod(p.second);
                 else if (p.first == "step")
                     step = std::stod(p.second);
+                else if (p.first == "host")
+                    (void)std::stoi(p.second); // where it runs, not how
                 else {
                     err = "lbfgs: unknown parameter '" + p.first + "'";
                     return nullptr;
@@ -166,18 +168,22 @@ struct mia_host_series {
     std::vector<char> frame_set;
 };
 
-// On-device L-BFGS through mia_hip_optimize: all frames in the batched context
-// (one graph covers every frame), or each frame in its own context, one frame
-// after the other.  Only `lbfgs` with gtol = ftolr = 0 runs on the device.
-static int run_device(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
-                      int *evals_out, double *seconds)
+bool mia_amd::device_lbfgs_params(const std::string &descr, int &maxeval, int &memory, double &step,
+                                   std::string &err)
 {
     std::string name;
     std::vector<std::pair<std::string, std::string>> kv;
-    if (!parse_descr(minimizer, name, kv) || name != "lbfgs")
-        return fail(MIA_HIP_ERR_ARG, "series: the device modes run the lbfgs minimizer only");
-    int maxeval = 100, memory = 10;
-    double step = 1.0;
+    err.clear();
+    if (!parse_descr(descr, name, kv)) {
+        err = "malformed minimizer descriptor '" + descr + "'";
+        return false;
+    }
+    if (name != "lbfgs")
+        return false;
+    maxeval = 100;
+    memory = 10;
+    step = 1.0;
+    bool device = true;
     try {
         for (auto &p : kv) {
             if (p.first == "maxiter")
@@ -186,14 +192,33 @@ static int run_device(mia_host_series *s, const char *minimizer, int mode, doubl
                 memory = std::stoi(p.second);
             else if (p.first == "step")
                 step = std::stod(p.second);
-            else if ((p.first == "gtol" || p.first == "ftolr") && std::stod(p.second) == 0.0)
-                continue;
+            else if (p.first == "host")
+                device = device && std::stoi(p.second) == 0;
+            else if (p.first == "gtol" || p.first == "ftolr")
+                device = device && std::stod(p.second) == 0.0;
             else
-                return fail(MIA_HIP_ERR_ARG, "series: device lbfgs does not take '" + p.first + "'");
+                device = false;
         }
     } catch (const std::exception &e) {
-        return fail(MIA_HIP_ERR_ARG, std::string("series: bad minimizer parameter value: ") + e.what());
+        err = std::string("bad minimizer parameter value: ") + e.what();
+        return false;
     }
+    return device && maxeval >= 1 && memory >= 1 && memory <= 16 && step > 0.0;
+}
+
+// On-device L-BFGS through mia_hip_optimize: all frames in the batched context
+// (one graph covers every frame), or each frame in its own context, one frame
+// after the other.  Only `lbfgs` with gtol = ftolr = 0 runs on the device.
+static int run_device(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
+                      int *evals_out, double *seconds)
+{
+    int maxeval = 100, memory = 10;
+    double step = 1.0;
+    std::string err;
+    if (!device_lbfgs_params(minimizer ? minimizer : "", maxeval, memory, step, err))
+        return fail(MIA_HIP_ERR_ARG, err.empty() ? "series: the device modes run lbfgs (maxiter, memory, step; "
+                                                   "gtol = ftolr = 0, memory <= 16)"
+                                                 : "series: " + err);
     const int chunk = 10;
     int st;
     if (mode == MIA_HOST_DEVICE_BATCHED) {

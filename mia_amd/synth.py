@@ -117,6 +117,19 @@ def series_frames_2d(n_frames=64, side=(96, 224), rate=8.0, max_disp=4.0, seed=0
     return frames
 
 
+def blob_pair_2d(shape=(128, 128), rate=16.0, sd=1.5, clip=3.0, seed=1):
+    """Plain 2D registration input: the fixed image is 8 Gaussian blobs
+    (centres uniform in the image, sigma U[6, 14] px, amplitude U[0.3, 1.0])
+    plus N(0, 0.01) noise; the moving image is the fixed one warped by a cubic
+    B-spline displacement of knot spacing `rate` (coefficients N(0, sd)
+    clipped to +-clip px).  Returns (moving, fixed, coefficient grid (x, y),
+    true coefficients)."""
+    fixed = gaussian_blobs(shape, 8, (6.0, 14.0), (0.3, 1.0), 0.01, seed=seed)
+    cs = (spline_csize(shape[1], rate, 2), spline_csize(shape[0], rate, 2))
+    p = random_spline_params(cs, sd, clip, seed=seed + 1)
+    return spline_warp(fixed, cs, p), fixed, cs, p
+
+
 def scale_to_unit_range(img, bits=16):
     """uint16 intensities scaled to float32 (config of the batched 2D series)."""
     q = np.round(np.clip(img, 0, None) / max(float(img.max()), 1e-12) * (2 ** bits - 1)).astype(np.uint16)

@@ -110,3 +110,64 @@ def test_nonrigid_register_python_2d():
     assert final < 0.3 * c_id
     with pytest.raises(ValueError):
         series.nonrigid_register(mov, fix, "spline:rate=8", "image:cost=[ncc]", "lbfgs:maxiter=5", levels=1)
+
+
+def test_penalty_in_context_equals_host_sum():
+    """The divcurl penalty attached to a context: every evaluation (eager and
+    the captured graph) returns the image cost plus weight * value and the
+    image gradient plus -weight * the penalty's gradient, bitwise as the
+    host sums them (TNonrigRegGradientProblem::evaluate_fdf,
+    mia/template/nonrigidregister.cxx:422-429); detached, the image cost alone."""
+    import mia_amd
+    from mia_amd import level, synth
+    shape = (34, 30, 38)
+    fix = synth.smoothed_noise(shape, 2.0, 3)
+    mov = synth.smoothed_noise(shape, 2.0, 4)
+    cs = tuple(mia_amd.spline_csize(n, 6.0, 3) for n in shape[::-1])
+    p = synth.random_spline_params(cs, 1.5, 3.0, 5)
+    w = 0.0137
+    with mia_amd.HipContext(0) as c, level.DivcurlPenalty(cs, tuple(float(v) for v in shape[::-1])) as pen:
+        c.setup(3, [shape], [cs], 3)
+        c.set_images(0, mov, fix)
+        c_img, g_img = c.cost_grad(p)
+        v, g = pen.evaluate(p.astype(np.float32))
+        c.set_penalty(0, pen, w)
+        c1, g1 = c.cost_grad(p)
+        assert c1[0] == c_img[0] + w * v
+        assert np.array_equal(g1, g_img + (-w * g))
+        c.graph_capture()
+        for _ in range(3):
+            cg, gg = c.graph_run(p)
+            assert cg[0] == c1[0] and np.array_equal(gg, g1)
+        with pytest.raises(RuntimeError):
+            with level.DivcurlPenalty((cs[0] + 1, cs[1], cs[2]), (1.0, 1.0, 1.0)) as bad:
+                c.set_penalty(0, bad, w)
+        c.set_penalty(0, None, 0.0)
+        c2, g2 = c.cost_grad(p)
+        assert c2[0] == c_img[0] and np.array_equal(g2, g_img)
+
+
+def test_nonrigid_register_device_matches_host_3d():
+    """Two-level 3D registration with the divcurl penalty: the on-device
+    L-BFGS (evaluation, penalty and optimizer step in one captured graph per
+    ten iterations) against the host L-BFGS driving the same evaluations
+    ("host=1"); same algorithm, different summation order of its dot
+    products, so the runs agree to rounding."""
+    import mia_amd
+    from mia_amd import series, synth
+    shape = (40, 44, 48)
+    fix = synth.smoothed_noise(shape, 3.0, 31)
+    cs = tuple(mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1])
+    q = synth.random_spline_params(cs, 1.5, 3.0, 32)
+    mov = synth.spline_warp(fix, cs, q)
+    tr = "spline:rate=8,penalty=[divcurl:weight=0.01]"
+    pd, gd, sd, fd = series.nonrigid_register(mov, fix, tr, "image:cost=[ssd]", "lbfgs:maxiter=8", levels=2)
+    ph, gh, sh, fh = series.nonrigid_register(mov, fix, tr, "image:cost=[ssd]", "lbfgs:maxiter=8,host=1", levels=2)
+    print("device", [(s["evaluations"], s["cost"]) for s in sd], fd)
+    print("host  ", [(s["evaluations"], s["cost"]) for s in sh], fh)
+    assert gd == gh
+    assert [s["evaluations"] for s in sd] == [s["evaluations"] for s in sh]
+    for a, b in zip(sd, sh):
+        assert a["cost"] == pytest.approx(b["cost"], rel=1e-9)
+    assert fd == pytest.approx(fh, rel=1e-9)
+    assert np.abs(pd - ph).max() <= 1e-7 * max(1.0, np.abs(ph).max())
