@@ -135,14 +135,18 @@ int mia_hip_graph_run(mia_hip_ctx *ctx);
 
 /* Minimise every instance's cost with L-BFGS on the device, all instances in
  * lockstep: each iteration is the batched evaluation followed by one
- * optimizer step per instance (one thread block each), `iters_per_graph`
- * iterations captured in one HIP graph; the host only replays the graph and
- * checks the instances' done flags.  Same algorithm as the host `lbfgs`
- * minimizer: memory `memory` (<= 16), `maxeval` evaluations per instance
- * (NLopt's maxeval, which MIA's `maxiter` maps to, addons/nlopt/nlopt.cc:237),
- * first step moving no parameter by more than `first_step`; instances of up
- * to 2048 parameters form the same direction in the compact representation,
- * so trajectories agree with the host to rounding.  x0: starting
+ * optimizer step per instance (one thread block each for instances of up to
+ * 2048 parameters; larger ones in segments of 512 parameters over three
+ * launches, so that their pair history streams through many CUs), with
+ * `iters_per_graph` iterations captured in one HIP graph; the host only
+ * replays the graph and checks the instances' done flags.  Same algorithm
+ * as the host `lbfgs` minimizer: memory `memory` (<= 16), `maxeval`
+ * evaluations per instance (NLopt's maxeval, which MIA's `maxiter` maps to,
+ * addons/nlopt/nlopt.cc:237), first step moving no parameter by more than
+ * `first_step`; the direction is the two-loop recursion's in the compact
+ * representation, so trajectories agree with the host to rounding, and an
+ * instance's trajectory does not depend on the batch.  Penalties attached
+ * with mia_hip_set_penalty are part of every evaluation.  x0: starting
  * parameters of all instances (host); outputs (host, any may be NULL): best
  * parameters, best costs, evaluations per instance, wall time of the
  * iteration loop in seconds (graph capture excluded).  Replaces the

@@ -201,9 +201,16 @@ __device__ __forceinline__ double coef_value(const double *c, long long k)
     return (double)(float)c[k];
 }
 
-// One thread per coefficient i: g_i and the row sum of the quadratic form
-// over the neighbours j with i - 4 <= j <= i + 3 per axis (the reference's
-// loop bounds, ppmatrix.cc:330-345); per-block partial sums in fixed order.
+// Threads per coefficient (one per z plane of its neighbourhood) and
+// coefficients per block of the divcurl kernel.
+constexpr int kDcPlanes = 2 * kPK;
+constexpr int kDcCoef = kLB / kDcPlanes;
+
+// Coefficient i's gradient g_i and row sum of the quadratic form over the
+// neighbours j with i - 4 <= j <= i + 3 per axis (the reference's loop
+// bounds, ppmatrix.cc:330-345): thread (i, q) takes the z plane
+// zj = zi - 4 + q, the eight planes' sums are added in ascending order,
+// and the block's coefficients' row sums in a fixed order into part[block].
 // ADD: grad[3i + k] += -w * g_k (the penalty's weighted gradient with the
 // reference's sign, splinetransformpenalty.cc:60-71, added to a gradient
 // already there); otherwise grad = g (unweighted, may be NULL).
@@ -213,19 +220,23 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, con
                                                  double *__restrict__ part)
 {
     __shared__ PPCell P[kPW * kPW * kPW];
+    __shared__ double acc[kDcCoef][kDcPlanes][10];
     __shared__ double red[kLB / 64];
     for (int k = threadIdx.x; k < kPW * kPW * kPW; k += kLB)
         P[k] = cells[k];
     __syncthreads();
     const long long n = (long long)nx * ny * nz;
-    const long long i = (long long)blockIdx.x * kLB + threadIdx.x;
-    double v = 0.0;
+    const int cl = threadIdx.x / kDcPlanes, q = threadIdx.x % kDcPlanes;
+    const long long i = (long long)blockIdx.x * kDcCoef + cl;
+    double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, g0 = 0, g1 = 0, g2 = 0;
+    double cix = 0.0, ciy = 0.0, ciz = 0.0;
     if (i < n) {
         const int xi = (int)(i % nx), yi = (int)((i / nx) % ny), zi = (int)(i / ((long long)nx * ny));
-        const double cix = coef_value(coef, 3 * i), ciy = coef_value(coef, 3 * i + 1),
-                     ciz = coef_value(coef, 3 * i + 2);
-        double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, g0 = 0, g1 = 0, g2 = 0;
-        for (int zj = max(0, zi - kPK); zj < min(zi + kPK, nz); ++zj) {
+        cix = coef_value(coef, 3 * i);
+        ciy = coef_value(coef, 3 * i + 1);
+        ciz = coef_value(coef, 3 * i + 2);
+        const int zj = zi - kPK + q;
+        if (zj >= 0 && zj < nz) {
             const int dz = zi - zj + kPK;
             for (int yj = max(0, yi - kPK); yj < min(yi + kPK, ny); ++yj) {
                 const int dy = yi - yj + kPK;
@@ -250,19 +261,41 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, con
                 }
             }
         }
+    }
+    double *a = acc[cl][q];
+    a[0] = r0;
+    a[1] = r1;
+    a[2] = r2;
+    a[3] = r3;
+    a[4] = r4;
+    a[5] = r5;
+    a[6] = g0;
+    a[7] = g1;
+    a[8] = g2;
+    __syncthreads();
+    double v = 0.0;
+    if (q == 0 && i < n) {
+        double t[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            t[k] = acc[cl][0][k];
+#pragma unroll
+            for (int pl = 1; pl < kDcPlanes; ++pl)
+                t[k] += acc[cl][pl][k];
+        }
         if (ADD) {
             // the product rounded before the sum, as the host adds the
             // penalty's gradient vector (no fused multiply-add)
 #pragma clang fp contract(off)
-            grad[3 * i] = grad[3 * i] + -w * g0;
-            grad[3 * i + 1] = grad[3 * i + 1] + -w * g1;
-            grad[3 * i + 2] = grad[3 * i + 2] + -w * g2;
+            grad[3 * i] = grad[3 * i] + -w * t[6];
+            grad[3 * i + 1] = grad[3 * i + 1] + -w * t[7];
+            grad[3 * i + 2] = grad[3 * i + 2] + -w * t[8];
         } else if (grad) {
-            grad[3 * i] = g0;
-            grad[3 * i + 1] = g1;
-            grad[3 * i + 2] = g2;
+            grad[3 * i] = t[6];
+            grad[3 * i + 1] = t[7];
+            grad[3 * i + 2] = t[8];
         }
-        v = r0 + r1 + r2 + r3 + r4 + r5;
+        v = t[0] + t[1] + t[2] + t[3] + t[4] + t[5];
     }
     for (int o = 32; o > 0; o >>= 1)
         v += __shfl_down(v, o, 64);
@@ -270,10 +303,10 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, con
         red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0.0;
+        double sum = 0.0;
         for (int wv = 0; wv < kLB / 64; ++wv)
-            s += red[wv];
-        part[blockIdx.x] = s;
+            sum += red[wv];
+        part[blockIdx.x] = sum;
     }
 }
 
@@ -587,7 +620,7 @@ extern "C" int mia_hip_divcurl_create(int device, const uint32_t *csize, const f
     p->device = device;
     std::copy(csize, csize + 3, p->cs);
     p->n = (long long)csize[0] * csize[1] * csize[2];
-    p->nblocks = (int)((p->n + kLB - 1) / kLB);
+    p->nblocks = (int)((p->n + kDcCoef - 1) / kDcCoef);
     make_pp_cells(csize, range, wdiv, wcurl, p->cells);
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreate(&p->stream));

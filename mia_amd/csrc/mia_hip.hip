@@ -163,9 +163,11 @@ struct mia_hip_ctx {
     // on-device optimizer state (allocated on first use, sized for `opt_memory`)
     int opt_memory = 0;
     double *d_oX = nullptr, *d_oG = nullptr, *d_oD = nullptr, *d_oBX = nullptr, *d_oS = nullptr, *d_oY = nullptr,
-           *d_oRHO = nullptr, *d_obest = nullptr, *d_oSY = nullptr, *d_oYY = nullptr;
+           *d_obest = nullptr, *d_oSY = nullptr, *d_oYY = nullptr, *d_oPART = nullptr, *d_oPART2 = nullptr;
+    int *d_oseg = nullptr;
+    int opt_segments = 0; // segments of the optimizer's passes over all instances
     int *d_odone = nullptr, *h_done = nullptr;
-    OptState *d_ost = nullptr;
+    OptState *d_ost = nullptr, *d_ost2 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
@@ -192,8 +194,8 @@ void free_buffers(mia_hip_ctx *c)
                     c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
-                    c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
-                    c->d_oYY};
+                    c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
+                    c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -205,8 +207,11 @@ void free_buffers(mia_hip_ctx *c)
         (void)hipHostFree(c->h_costs);
     if (c->h_done)
         (void)hipHostFree(c->h_done);
-    c->d_oX = c->d_oG = c->d_oD = c->d_oBX = c->d_oS = c->d_oY = c->d_oRHO = c->d_obest = nullptr;
-    c->d_oSY = c->d_oYY = nullptr;
+    c->d_oX = c->d_oG = c->d_oD = c->d_oBX = c->d_oS = c->d_oY = c->d_obest = nullptr;
+    c->d_oSY = c->d_oYY = c->d_oPART = c->d_oPART2 = nullptr;
+    c->d_oseg = nullptr;
+    c->d_ost2 = nullptr;
+    c->opt_segments = 0;
     c->d_odone = c->h_done = nullptr;
     c->d_ost = nullptr;
     c->opt_memory = 0;
@@ -1221,9 +1226,21 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const int n = c->n_inst;
+    // instances of up to kLbRegN parameters step in one block each
+    // (k_lbfgs_step_small), larger ones in segments of kSeg parameters
+    // (k_lbfgs_pass1..3): the segment prefix counts none for the small ones
+    std::vector<int> seg(n + 1, 0);
+    int n_small = 0;
+    for (int i = 0; i < n; ++i) {
+        const InstDesc &d = c->inst[i];
+        const long long np = (long long)c->dim * d.cx * d.cy * d.cz;
+        const bool small = np <= kLbRegN;
+        n_small += small;
+        seg[i + 1] = seg[i] + (small ? 0 : (int)((np + kSeg - 1) / kSeg));
+    }
     if (c->opt_memory != memory) {
-        void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone,
-                       c->d_ost, c->d_oSY, c->d_oYY};
+        void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone,
+                       c->d_ost, c->d_oSY, c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
         for (void *p : old)
             if (p)
                 (void)hipFree(p);
@@ -1231,30 +1248,35 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
             (void)hipHostFree(c->h_done);
         c->h_done = nullptr;
         c->opt_memory = 0;
+        const size_t R = (size_t)memory + 1; // ring: memory pairs and the candidate's slot
         if ((st = dev_alloc(&c->d_oX, c->n_par)) || (st = dev_alloc(&c->d_oG, c->n_par)) ||
             (st = dev_alloc(&c->d_oD, c->n_par)) || (st = dev_alloc(&c->d_oBX, c->n_par)) ||
-            (st = dev_alloc(&c->d_oS, (size_t)memory * c->n_par)) ||
-            (st = dev_alloc(&c->d_oY, (size_t)memory * c->n_par)) || (st = dev_alloc(&c->d_oRHO, memory * n)) ||
+            (st = dev_alloc(&c->d_oS, R * c->n_par)) || (st = dev_alloc(&c->d_oY, R * c->n_par)) ||
             (st = dev_alloc(&c->d_obest, n)) || (st = dev_alloc(&c->d_odone, n)) || (st = dev_alloc(&c->d_ost, n)) ||
-            (st = dev_alloc(&c->d_oSY, (size_t)memory * memory * n)) ||
-            (st = dev_alloc(&c->d_oYY, (size_t)memory * memory * n)))
+            (st = dev_alloc(&c->d_oSY, R * R * n)) || (st = dev_alloc(&c->d_oYY, R * R * n)) ||
+            (st = dev_alloc(&c->d_oPART, (size_t)kPart * seg[n])) ||
+            (st = dev_alloc(&c->d_oPART2, (size_t)kPart * seg[n])) || (st = dev_alloc(&c->d_ost2, n)) ||
+            (st = dev_alloc(&c->d_oseg, n + 1)))
             return st;
         HIP_TRY(hipHostMalloc((void **)&c->h_done, sizeof(int) * n, hipHostMallocDefault));
         c->opt_memory = memory;
+        c->opt_segments = seg[n];
     }
     // initial state: parameters, phase 0, not done (plain copies, outside the graph)
     std::vector<OptState> st0(n);
     for (auto &o : st0) {
         o = OptState();
         o.phase = 0;
+        o.mode = 0;
     }
-    std::vector<int> done0(n, 0);
+    std::vector<int> zeros(n, 0);
     HIP_TRY(hipMemcpyAsync(c->d_params, x0, sizeof(double) * c->n_par, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_ost, st0.data(), sizeof(OptState) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_odone, done0.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_odone, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_oseg, seg.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    OptBuffers ob{c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_oRHO, c->d_obest, c->d_odone, c->d_ost,
-                  c->d_oSY, c->d_oYY};
+    OptBuffers ob{c->d_oX,   c->d_oG,   c->d_oD,   c->d_oBX,    c->d_oS,     c->d_oY,   c->d_obest, c->d_odone,
+                  c->d_ost,  c->d_ost2, c->d_oSY,  c->d_oYY,    c->d_oPART,  c->d_oPART2, c->d_oseg};
     // capture iters_per_graph iterations + the done flags read-back
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -1263,11 +1285,32 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
         rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
         if (rc == MIA_HIP_OK) {
-            hipLaunchKernelGGL(k_lbfgs_step, dim3(n), dim3(kOptBlock), 0, c->stream, c->d_inst, c->dim, ob, c->d_params,
-                               c->d_grad, c->d_costs, memory, maxeval, first_step);
-            hipError_t e = hipGetLastError();
+            hipError_t e = hipSuccess;
+            if (n_small > 0) {
+                hipLaunchKernelGGL(k_lbfgs_step_small, dim3(n), dim3(kOptBlock), 0, c->stream,
+                                   (const InstDesc *)c->d_inst, c->dim, ob, c->d_params, (const double *)c->d_grad,
+                                   (const double *)c->d_costs, memory, maxeval, first_step);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess && seg[n] > 0) {
+                hipLaunchKernelGGL(k_lbfgs_pass1, dim3(seg[n]), dim3(kSegT), 0, c->stream,
+                                   (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params,
+                                   (const double *)c->d_grad, (const double *)c->d_costs, memory, maxeval);
+                e = hipGetLastError();
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL(k_lbfgs_pass2, dim3(seg[n]), dim3(kSegT), 0, c->stream,
+                                       (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params,
+                                       (const double *)c->d_costs, memory, maxeval, first_step);
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL(k_lbfgs_pass3, dim3(seg[n]), dim3(kSegT), 0, c->stream,
+                                       (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params, first_step);
+                    e = hipGetLastError();
+                }
+            }
             if (e != hipSuccess)
-                rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs_step: ") + hipGetErrorString(e));
+                rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs passes: ") + hipGetErrorString(e));
         }
     }
     if (rc == MIA_HIP_OK) {

@@ -1,30 +1,80 @@
-// On-device batched L-BFGS: one thread block per registration instance runs
-// one step of the minimizer after each batched cost + gradient evaluation, so
-// a whole optimisation iteration (warp, stage 1, reduce, step) stays on the
-// GPU and can be captured in a graph.  The algorithm is the one of the host
-// LBFGSMinimizer (mia_amd/host/minimizer.hh): the L-BFGS direction (two-loop
-// recursion of Nocedal & Wright, Alg. 7.4/7.5, or for instances of up to
-// kLbRegN parameters the equal compact form), backtracking Armijo line search with safeguarded
-// quadratic interpolation, first step scaled to move no parameter by more
-// than `first_step`, `maxeval` counting every evaluation.  Dot products are
-// block reductions in a fixed order, so an instance's trajectory does not
-// depend on the batch it runs in.
+// On-device batched L-BFGS: after each batched cost + gradient evaluation,
+// one step of the minimizer runs for every instance, so a whole
+// optimisation iteration (warp, stage 1, reduce, step) stays on the GPU and can
+// be captured in a graph.  The algorithm is the one of the host
+// LBFGSMinimizer (mia_amd/host/minimizer.hh): the L-BFGS direction (written
+// in the compact form of Byrd, Nocedal & Schnabel 1994, equal to the host's
+// two-loop recursion of Nocedal & Wright, Alg. 7.4/7.5), backtracking Armijo
+// line search with safeguarded quadratic interpolation, first step scaled to
+// move no parameter by more than `first_step`, `maxeval` counting every
+// evaluation.
+//
+// Instances of up to kLbRegN parameters take the whole step in one block
+// (k_lbfgs_step_small, at the end of this file): for them a step is a chain
+// of short dependent phases that one block runs faster than several launches.
+//
+// Larger instances: the parameter vector is cut into segments of kSeg
+// elements, one block each, over three launches, so that a large instance (44,616 parameters at
+// the full level of the 3D registration) streams its pair history through
+// many CUs instead of one.  No block waits for or signals another: each pass
+// writes per-segment partial sums, and in the next pass every block of the
+// instance adds them up in ascending segment order and takes the same scalar
+// decisions (the launch boundary makes the sums visible).  An instance's
+// segments depend only on its own size, so its trajectory does not depend on
+// the batch it runs in.
+//
+//   pass 1: the evaluated point's element updates (best point, X/G, the
+//           candidate pair s, y in the ring's spare slot, or the backtracking
+//           point) and partial sums of every product the step needs (s.y,
+//           y.y, s.s, s.g, y.g, max |g|, and s_c.y, y_c.y, s_c.g, y_c.g per
+//           stored pair c);
+//   pass 2: pair acceptance, finish, the compact coefficients; elements of
+//           the direction D = -H g and the trial point (or the best point
+//           when finished); partial sums of g.D;
+//   pass 3: the descent check (restart from steepest descent when it fails)
+//           and the line-search state for the next step.
 #pragma once
 #include "kernels.hh"
 
 namespace mia_hip {
 
 constexpr int kMaxMemory = 16;
-// threads per instance: eight waves, so that a CU keeps twice the loads of the
-// pair history in flight that four waves would
-constexpr int kOptBlock = 512;
+constexpr int kSegT = 256;           // threads of a segment block
+constexpr int kSegK = 2;             // elements per thread
+constexpr int kSeg = kSegT * kSegK;  // elements per segment
+constexpr int kSegW = kSegT / 64;    // waves per segment block
+constexpr int kRing = kMaxMemory + 1; // ring slots: memory pairs and a spare for the candidate
+constexpr int kOptBlock = 512;       // threads of the one-block step of small instances
 
 struct OptState {
     double f, gd, alpha, best_f;
-    int phase;  // 0: first evaluation pending, 1: line search, 2: done
+    double gamma, next_alpha; // newest pair's s.y / y.y; the next step length (pass 1 -> pass 2)
+    int phase;                // 0: first evaluation pending, 1: line search, 2: done
     int evals;
-    int k, head; // memory pairs stored, index of the oldest
+    int k, head;              // pairs stored, ring slot of the oldest
+    int mode;                 // pass 2: 0 nothing, 1 new direction, 2 finished (best point back)
+    int first;                // the direction is the first (steepest descent) one
 };
+
+// Buffers (doubles) per instance i with n = dim * ncoef parameters at offset
+// par_off: X, G, D, BX laid out like the parameters; S and Y hold `memory + 1`
+// vectors (ring slots) per instance at offset (memory + 1) * par_off; SY and
+// YY (memory + 1)^2 pair products per instance by ring slot; ST the state
+// between steps, ST2 the state decided by pass 2; PART and PART2 kPart
+// partial sums per segment (segment prefix) of passes 1 and 2.
+struct OptBuffers {
+    double *X, *G, *D, *BX, *S, *Y, *best_cost;
+    int *done;
+    OptState *st, *ST2;
+    double *SY, *YY, *PART, *PART2;
+    const int *seg_prefix;
+};
+
+constexpr int kPart = 6 + 4 * kMaxMemory;
+
+// partial-sum indices: the new pair's s.y, y.y, s.s, s.g, y.g, max |g|, then per
+// stored pair c (oldest first) s_c.y, y_c.y, s_c.g, y_c.g; pass 2 uses kPSY for g.D
+enum { kPSY = 0, kPYY, kPSS, kPSG, kPYG, kPGMAX, kPPair };
 
 // A double moved across lanes by DPP (both halves; lanes that the row mask
 // leaves out, or whose source is out of range, receive `fill`)
@@ -52,6 +102,589 @@ __device__ __forceinline__ double wave_reduce63(double v)
     v = op(v, dpp64<0x143, 0xc>(v, id));
     return v;
 }
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave sums of 16 values at once (fixed order): each lane stores its 16 values into its wave's tb
+// (16 rows of 65 doubles), then lane 4v + q adds row v's 16 entries of lanes 16q .. 16q + 15 (starting
+// at a lane-dependent rotation, to spread the banks) and the quad combines its four partial sums.
+// Returns the wave sum of value lane / 4 (in all four lanes of the quad).  Contains two block barriers,
+// so every thread of the block calls it the same number of times.
+constexpr int kTbRow = 65;
+
+__device__ __forceinline__ double transpose_sum16(const double *v16, double *tb)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+        tb[v * kTbRow + lane] = v16[v];
+    __syncthreads();
+    const int v = lane >> 2, q = lane & 3;
+    const double *row = tb + v * kTbRow + q * 16;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        acc += row[(j + 2 * q) & 15];
+    acc += dpp64<0xB1, 0xf>(acc, 0.0);
+    acc += dpp64<0x4E, 0xf>(acc, 0.0);
+    __syncthreads(); // tb free for the next call
+    return acc;
+}
+
+// this thread's elements of a segment (element base + thread + k * kSegT)
+__device__ __forceinline__ void seg_load(double *v, const double *__restrict__ src, long long base, long long n)
+{
+#pragma unroll
+    for (int k = 0; k < kSegK; ++k) {
+        const long long e = base + threadIdx.x + (long long)k * kSegT;
+        v[k] = e < n ? src[e] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void seg_store(double *dst, const double *v, long long base, long long n)
+{
+#pragma unroll
+    for (int k = 0; k < kSegK; ++k) {
+        const long long e = base + threadIdx.x + (long long)k * kSegT;
+        if (e < n)
+            dst[e] = v[k];
+    }
+}
+
+// Pairs are read kLbP at a time, all loads of a group issued before any use.
+constexpr int kLbP = 8;
+
+// stored pairs j0 .. j0 + kLbP - 1 (those below kk, ring slots from head) into sc, yc
+__device__ __forceinline__ void seg_load_group(double (*sc)[kSegK], double (*yc)[kSegK], int j0, int kk, int head,
+                                               int R, const double *S, const double *Y, long long base, long long n)
+{
+#pragma unroll
+    for (int t = 0; t < kLbP; ++t)
+        if (j0 + t < kk) {
+            const int slot = (head + j0 + t) % R;
+            seg_load(sc[t], S + (long long)slot * n, base, n);
+            seg_load(yc[t], Y + (long long)slot * n, base, n);
+        }
+}
+
+// The segment's partial sums: the block's wave totals of each value (in wp,
+// complete after a barrier) added over the waves in order by thread v.
+__device__ __forceinline__ void seg_partials_out(const double (*wp)[kPart], int nvals, double *__restrict__ out)
+{
+    const int v = threadIdx.x;
+    if (v < nvals) {
+        double t = wp[0][v];
+#pragma unroll
+        for (int w = 1; w < kSegW; ++w)
+            t = v == kPGMAX ? fmax(t, wp[w][v]) : t + wp[w][v];
+        out[v] = t;
+    }
+}
+
+// Totals over an instance's segments in ascending order (thread v: value v);
+// every block of the instance computes the same totals.
+__device__ __forceinline__ void instance_totals(const double *__restrict__ part, int nseg, int nvals, double *tot)
+{
+    const int v = threadIdx.x;
+    if (v < nvals) {
+        double t = part[v];
+        for (int sgi = 1; sgi < nseg; ++sgi) {
+            const double x = part[(long long)sgi * kPart + v];
+            t = v == kPGMAX ? fmax(t, x) : t + x;
+        }
+        tot[v] = t;
+    }
+    __syncthreads();
+}
+
+// Coefficients of the compact direction for k >= 1 pairs from ring slot head (wave 0, lane i for pair i,
+// oldest first): cu[i] = p_i, cw[i] = gamma w_i, so that H g = gamma g + sum_i (cu_i s_i - cw_i y_i):
+//   w = R^-1 a,  p = R^-T((Dg + gamma Y^T Y) w - gamma b),
+// with a = S^T g, b = Y^T g, R the upper triangle of S^T Y (R_ij = s_i.y_j, i older or equal), Dg its
+// diagonal and gamma = s.y / y.y of the newest pair (the two-loop's scaling).  Pair i's products with g
+// come from the totals at stored index i + drop, or from the new pair's s.g, y.g when `fresh` and
+// i == k - 1.  mSY, mYY: R x R tables by ring slot.
+__device__ __forceinline__ void lbfgs_compact_coeffs(int R, int k, int head, int drop, bool fresh,
+                                                     const double *tot, const double *mSY, const double *mYY,
+                                                     double gamma, double *cu, double *cw)
+{
+    const int i = threadIdx.x;
+    const int il = i < k ? i : k - 1;
+    const int si = (head + il) % R;
+    const bool isnew = fresh && il == k - 1;
+    const int po = kPPair + 4 * (il + drop);
+    const double a = tot[isnew ? kPSG : po + 2];
+    const double b = tot[isnew ? kPYG : po + 3];
+    const double rinv = 1.0 / mSY[si * R + si];
+    // w = R^-1 a (back substitution, column by column)
+    double r = a, wi = 0.0;
+#pragma unroll
+    for (int j = kMaxMemory - 1; j >= 0; --j)
+        if (j < k) {
+            const int sj = (head + j) % R;
+            const double wj = readlane_d(r * rinv, j);
+            if (il == j)
+                wi = wj;
+            if (il < j)
+                r -= mSY[si * R + sj] * wj;
+        }
+    // t = (Dg + gamma Y^T Y) w - gamma b
+    double yw = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxMemory; ++j)
+        if (j < k) {
+            const int sj = (head + j) % R;
+            yw += mYY[si * R + sj] * readlane_d(wi, j);
+        }
+    // p = R^-T t (forward substitution)
+    r = mSY[si * R + si] * wi + gamma * yw - gamma * b;
+    double pi = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxMemory; ++j)
+        if (j < k) {
+            const int sj = (head + j) % R;
+            const double pj = readlane_d(r * rinv, j);
+            if (il == j)
+                pi = pj;
+            if (il > j)
+                r -= mSY[sj * R + si] * pj;
+        }
+    if (i < k) {
+        cu[i] = pi;
+        cw[i] = gamma * wi;
+    }
+}
+
+// Block of segment b: its instance, segment index and count.
+struct SegPos {
+    int i, sg, nseg;
+    long long n, base;
+};
+
+__device__ __forceinline__ SegPos seg_pos(const InstDesc *__restrict__ inst, const int *__restrict__ prefix,
+                                          int n_inst, int dim)
+{
+    SegPos p;
+    const int b = blockIdx.x;
+    p.i = find_inst(prefix, n_inst, b);
+    p.sg = b - prefix[p.i];
+    p.nseg = prefix[p.i + 1] - prefix[p.i];
+    const InstDesc d = inst[p.i];
+    p.n = (long long)dim * d.cx * d.cy * d.cz;
+    p.base = (long long)p.sg * kSeg;
+    return p;
+}
+
+// What the evaluated point means for the line search, the same in every
+// block: 0 first evaluation, 1 accepted (Armijo), 2 backtrack to step *an,
+// 3 finished (evaluations spent, or the step became negligible).
+__device__ __forceinline__ int step_kind(const OptState &st, double ft, int maxeval, double *an, bool *better)
+{
+    *an = 0.0;
+    *better = false;
+    if (st.phase == 0)
+        return 0;
+    const bool finite = isfinite(ft);
+    *better = finite && ft < st.best_f;
+    if (finite && ft <= st.f + 1e-4 * st.alpha * st.gd)
+        return 1;
+    if (st.evals + 1 >= maxeval)
+        return 3;
+    const double a = st.alpha;
+    double b = 0.5 * a;
+    if (finite) {
+        const double den = 2.0 * (ft - st.f - a * st.gd);
+        if (den > 0.0)
+            b = -st.gd * a * a / den;
+    }
+    b = fmin(fmax(b, 0.1 * a), 0.5 * a);
+    *an = b;
+    return b < 1e-20 ? 3 : 2;
+}
+
+// number of pass-1 partial sums of a step of this kind with kk stored pairs
+__device__ __forceinline__ int pass1_nvals(int kind, int kk)
+{
+    return kind == 0 ? kPGMAX + 1 : kind == 1 ? kPPair + 4 * kk : 0;
+}
+
+// Pass 1 (segments): the evaluated point's element updates and the partial
+// sums of every product the step needs.
+__global__ void __launch_bounds__(kSegT)
+    k_lbfgs_pass1(const InstDesc *__restrict__ inst, int n_inst, int dim, OptBuffers ob, double *__restrict__ xt,
+                  const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval)
+{
+    __shared__ double wp[kSegW][kPart];
+    __shared__ double tbuf[kSegW * 16 * kTbRow];
+    const SegPos q = seg_pos(inst, ob.seg_prefix, n_inst, dim);
+    const int i = q.i;
+    const long long n = q.n, base = q.base;
+    const OptState st = ob.st[i];
+    if (st.phase == 2)
+        return;
+    const int R = memory + 1;
+    const long long off = inst[i].par_off;
+    double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off, *BX = ob.BX + off;
+    double *S = ob.S + (long long)R * off, *Y = ob.Y + (long long)R * off;
+    double *XT = xt + off;
+    const double *GT = gt + off;
+    double an;
+    bool better;
+    const int kind = step_kind(st, costs[i], maxeval, &an, &better);
+    double rxt[kSegK], rgt[kSegK];
+    seg_load(rxt, XT, base, n);
+    seg_load(rgt, GT, base, n);
+    if (better)
+        seg_store(BX, rxt, base, n);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *tb = tbuf + w * 16 * kTbRow;
+    const int kk = st.k, head = st.head;
+    if (kind == 0) {
+        double gm = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSegK; ++k)
+            gm = fmax(gm, fabs(rgt[k]));
+        seg_store(X, rxt, base, n);
+        seg_store(G, rgt, base, n);
+        seg_store(BX, rxt, base, n);
+        gm = wave_reduce63<true>(gm);
+        if (lane < kPGMAX)
+            wp[w][lane] = 0.0; // the pair sums are unused here: zero, not stale
+        if (lane == 63)
+            wp[w][kPGMAX] = gm;
+    } else if (kind == 1) {
+        double rx[kSegK], rg[kSegK], sv[kSegK], yv[kSegK];
+        seg_load(rx, X, base, n);
+        seg_load(rg, G, base, n);
+        double sy = 0.0, yy = 0.0, ss = 0.0, sg = 0.0, yg = 0.0, gm = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSegK; ++k) {
+            sv[k] = rxt[k] - rx[k];
+            yv[k] = rgt[k] - rg[k];
+            sy += sv[k] * yv[k];
+            yy += yv[k] * yv[k];
+            ss += sv[k] * sv[k];
+            sg += sv[k] * rgt[k];
+            yg += yv[k] * rgt[k];
+            gm = fmax(gm, fabs(rgt[k]));
+        }
+        // the candidate pair into the spare slot (part of the ring only if
+        // pass 2 accepts it)
+        const int ns = (head + kk) % R;
+        seg_store(S + (long long)ns * n, sv, base, n);
+        seg_store(Y + (long long)ns * n, yv, base, n);
+        seg_store(X, rxt, base, n);
+        seg_store(G, rgt, base, n);
+        {
+            const double h[16] = {sy, yy, ss, sg, yg, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const double t = transpose_sum16(h, tb);
+            gm = wave_reduce63<true>(gm);
+            if ((lane & 3) == 3 && lane < 4 * kPGMAX)
+                wp[w][lane >> 2] = t;
+            if (lane == 63)
+                wp[w][kPGMAX] = gm;
+        }
+        for (int j0 = 0; j0 < kk; j0 += kLbP) {
+            double sc[kLbP][kSegK], yc[kLbP][kSegK];
+            seg_load_group(sc, yc, j0, kk, head, R, S, Y, base, n);
+            double dd[4 * kLbP];
+#pragma unroll
+            for (int t = 0; t < kLbP; ++t) {
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+                for (int k = 0; k < kSegK; ++k) {
+                    d0 += sc[t][k] * yv[k];
+                    d1 += yc[t][k] * yv[k];
+                    d2 += sc[t][k] * rgt[k];
+                    d3 += yc[t][k] * rgt[k];
+                }
+                dd[4 * t] = d0;
+                dd[4 * t + 1] = d1;
+                dd[4 * t + 2] = d2;
+                dd[4 * t + 3] = d3;
+            }
+#pragma unroll
+            for (int h = 0; h < kLbP / 4; ++h) {
+                const double t = transpose_sum16(dd + 16 * h, tb);
+                const int j = j0 + 4 * h + (lane >> 4);
+                if ((lane & 3) == 3 && j < kk)
+                    wp[w][kPPair + 4 * (j0 + 4 * h) + (lane >> 2)] = t;
+            }
+        }
+    } else if (kind == 2) {
+        double rx[kSegK], dv[kSegK], xv[kSegK];
+        seg_load(rx, X, base, n);
+        seg_load(dv, D, base, n);
+#pragma unroll
+        for (int k = 0; k < kSegK; ++k)
+            xv[k] = rx[k] + an * dv[k];
+        seg_store(XT, xv, base, n);
+    }
+    __syncthreads();
+    seg_partials_out(wp, pass1_nvals(kind, kk), ob.PART + ((long long)ob.seg_prefix[i] + q.sg) * kPart);
+}
+
+// Pass 2 (segments): every block adds up pass 1's partial sums and takes the
+// step's decisions (pair acceptance, finish, the compact coefficients), the
+// same in each; then its elements of the direction and the trial point (or
+// the best point when finished) and the partial sums of g.D.  Block 0 of
+// the instance stores the decided state in ST2 and the pair-product entries
+// of the new pair's slot (the spare slot, which no block reads in this pass).
+__global__ void __launch_bounds__(kSegT)
+    k_lbfgs_pass2(const InstDesc *__restrict__ inst, int n_inst, int dim, OptBuffers ob, double *__restrict__ xt,
+                  const double *__restrict__ costs, int memory, int maxeval, double first_step)
+{
+    __shared__ double wp[kSegW][kPart];
+    __shared__ double tot[kPart];
+    __shared__ double mSY[kRing * kRing], mYY[kRing * kRing];
+    __shared__ double cu[kMaxMemory], cw[kMaxMemory];
+    __shared__ OptState s;
+    __shared__ int flags[3]; // fresh (pair stored), drop (oldest pair replaced), new slot
+    const SegPos q = seg_pos(inst, ob.seg_prefix, n_inst, dim);
+    const int i = q.i;
+    const long long n = q.n, base = q.base;
+    const OptState st = ob.st[i];
+    if (st.phase == 2) {
+        if (q.sg == 0 && threadIdx.x == 0) {
+            OptState s2 = st;
+            s2.mode = 0;
+            ob.ST2[i] = s2;
+        }
+        return;
+    }
+    const int R = memory + 1;
+    const long long off = inst[i].par_off;
+    double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off, *BX = ob.BX + off;
+    const double *S = ob.S + (long long)R * off, *Y = ob.Y + (long long)R * off;
+    double *XT = xt + off;
+    const double ft = costs[i];
+    const bool finite = isfinite(ft);
+    double an;
+    bool better;
+    const int kind = step_kind(st, ft, maxeval, &an, &better);
+    const int kk = st.k, head = st.head;
+    instance_totals(ob.PART + (long long)ob.seg_prefix[i] * kPart, q.nseg, pass1_nvals(kind, kk), tot);
+    for (int t = threadIdx.x; t < R * R; t += kSegT) {
+        mSY[t] = ob.SY[(long long)R * R * i + t];
+        mYY[t] = ob.YY[(long long)R * R * i + t];
+    }
+    if (threadIdx.x == 0) {
+        s = st;
+        flags[0] = flags[1] = 0;
+        flags[2] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool finish = false, search = false;
+        if (kind == 0) {
+            s.f = ft;
+            s.evals = 1;
+            s.best_f = finite ? ft : INFINITY;
+            s.k = 0;
+            s.head = 0;
+            const double gmax = tot[kPGMAX];
+            if (!finite || gmax <= 0.0 || s.evals >= maxeval) {
+                finish = true;
+            } else {
+                search = true;
+                s.first = 1;
+                s.next_alpha = fmin(1.0, first_step / gmax);
+            }
+        } else {
+            const double f0 = s.f;
+            s.evals += 1;
+            if (better)
+                s.best_f = ft;
+            if (kind == 1) {
+                const double sy = tot[kPSY], yy = tot[kPYY], ss = tot[kPSS], gmax = tot[kPGMAX];
+                if (sy > 1e-12 * sqrt(ss * yy)) {
+                    const int drop = kk < memory ? 0 : 1;
+                    const int ns = (head + kk) % R;
+                    // the new pair's products with the pairs that stay
+                    for (int jo = drop; jo < kk; ++jo) {
+                        const int sc = (head + jo) % R;
+                        mSY[sc * R + ns] = tot[kPPair + 4 * jo];
+                        mYY[sc * R + ns] = mYY[ns * R + sc] = tot[kPPair + 4 * jo + 1];
+                    }
+                    mSY[ns * R + ns] = sy;
+                    mYY[ns * R + ns] = yy;
+                    if (kk < memory)
+                        s.k = kk + 1;
+                    else
+                        s.head = (head + 1) % R;
+                    flags[0] = 1;
+                    flags[1] = drop;
+                    flags[2] = ns;
+                }
+                s.f = ft;
+                if (s.evals >= maxeval || gmax <= 0.0 || f0 == ft) {
+                    finish = true;
+                } else {
+                    search = true;
+                    s.first = 0;
+                    s.next_alpha = 1.0;
+                }
+            } else if (kind == 3) {
+                finish = true;
+            } else {
+                s.alpha = an;
+            }
+        }
+        s.mode = finish ? 2 : search ? 1 : 0;
+        s.gamma = 1.0;
+        if (search && s.k > 0) {
+            const int sl = (s.head + s.k - 1) % R;
+            s.gamma = mSY[sl * R + sl] / mYY[sl * R + sl];
+        }
+    }
+    __syncthreads();
+    if (s.mode == 1 && s.k > 0 && threadIdx.x < 64)
+        lbfgs_compact_coeffs(R, s.k, s.head, flags[1], flags[0] != 0, tot, mSY, mYY, s.gamma, cu, cw);
+    if (q.sg == 0) {
+        const int ns = flags[2];
+        if (ns >= 0)
+            for (int t = threadIdx.x; t < R; t += kSegT) {
+                ob.SY[(long long)R * R * i + t * R + ns] = mSY[t * R + ns];
+                ob.YY[(long long)R * R * i + t * R + ns] = mYY[t * R + ns];
+                ob.YY[(long long)R * R * i + ns * R + t] = mYY[ns * R + t];
+            }
+        if (threadIdx.x == 0)
+            ob.ST2[i] = s;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int nvals = 0;
+    if (s.mode == 2) {
+        double bx[kSegK];
+        seg_load(bx, BX, base, n);
+        seg_store(XT, bx, base, n);
+    } else if (s.mode == 1) {
+        // D = -(gamma g + sum_j (cu_j s_j - cw_j y_j)), the trial point X + alpha D
+        const int k = s.k, hd = s.head;
+        double g[kSegK], x[kSegK], dv[kSegK];
+        seg_load(g, G, base, n);
+        seg_load(x, X, base, n);
+#pragma unroll
+        for (int k2 = 0; k2 < kSegK; ++k2)
+            dv[k2] = s.gamma * g[k2];
+        for (int j0 = 0; j0 < k; j0 += kLbP) {
+            double sc[kLbP][kSegK], yc[kLbP][kSegK];
+            seg_load_group(sc, yc, j0, k, hd, R, S, Y, base, n);
+#pragma unroll
+            for (int t = 0; t < kLbP; ++t)
+                if (j0 + t < k) {
+                    const double u = cu[j0 + t], v = cw[j0 + t];
+#pragma unroll
+                    for (int k2 = 0; k2 < kSegK; ++k2) {
+                        dv[k2] += u * sc[t][k2];
+                        dv[k2] -= v * yc[t][k2];
+                    }
+                }
+        }
+        double gd = 0.0, xv[kSegK];
+#pragma unroll
+        for (int k2 = 0; k2 < kSegK; ++k2) {
+            dv[k2] = -dv[k2];
+            const long long e = base + threadIdx.x + (long long)k2 * kSegT;
+            if (e < n)
+                gd += g[k2] * dv[k2];
+            xv[k2] = x[k2] + s.next_alpha * dv[k2];
+        }
+        seg_store(D, dv, base, n);
+        seg_store(XT, xv, base, n);
+        gd = wave_reduce63<false>(gd);
+        if (lane == 63)
+            wp[w][0] = gd;
+        nvals = 1;
+    }
+    __syncthreads();
+    seg_partials_out(wp, nvals, ob.PART2 + ((long long)ob.seg_prefix[i] + q.sg) * kPart);
+}
+
+// Pass 3 (segments): every block adds up g.D; a direction that does not
+// descend is replaced by steepest descent (D = -G, the first step's length
+// from max |g| of pass 1, g.D = -g.g), each block rewriting its elements.
+// Block 0 stores the line-search state for the next step.
+__global__ void __launch_bounds__(kSegT)
+    k_lbfgs_pass3(const InstDesc *__restrict__ inst, int n_inst, int dim, OptBuffers ob, double *__restrict__ xt,
+                  double first_step)
+{
+    __shared__ double tot[kPart];
+    __shared__ double red[kSegW];
+    const SegPos q = seg_pos(inst, ob.seg_prefix, n_inst, dim);
+    const int i = q.i;
+    const long long n = q.n, base = q.base;
+    OptState s = ob.ST2[i];
+    if (s.phase == 2)
+        return;
+    const long long off = inst[i].par_off;
+    double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off;
+    double *XT = xt + off;
+    if (s.mode == 1) {
+        instance_totals(ob.PART2 + (long long)ob.seg_prefix[i] * kPart, q.nseg, 1, tot);
+        if (tot[0] < 0.0) {
+            s.gd = tot[0];
+            s.alpha = s.next_alpha;
+        } else {
+            // restart: g.g and max |g| over the instance (the instance's G,
+            // one pass in every block), then this segment's elements
+            double gg = 0.0, gm = 0.0;
+            for (long long e = threadIdx.x; e < n; e += kSegT) {
+                gg += G[e] * G[e];
+                gm = fmax(gm, fabs(G[e]));
+            }
+            const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            gg = wave_reduce63<false>(gg);
+            gm = wave_reduce63<true>(gm);
+            if (lane == 63) {
+                red[w] = gg;
+                tot[1 + w] = gm;
+            }
+            __syncthreads();
+            double tg = red[0], tm = tot[1];
+#pragma unroll
+            for (int v = 1; v < kSegW; ++v) {
+                tg += red[v];
+                tm = fmax(tm, tot[1 + v]);
+            }
+            const double alpha = tm > 0.0 ? fmin(1.0, first_step / tm) : 1.0;
+            double g[kSegK], x[kSegK], dv[kSegK], xv[kSegK];
+            seg_load(g, G, base, n);
+            seg_load(x, X, base, n);
+#pragma unroll
+            for (int k2 = 0; k2 < kSegK; ++k2) {
+                dv[k2] = -g[k2];
+                xv[k2] = x[k2] + alpha * dv[k2];
+            }
+            seg_store(D, dv, base, n);
+            seg_store(XT, xv, base, n);
+            s.k = 0;
+            s.head = 0;
+            s.first = 1;
+            s.gd = -tg;
+            s.alpha = alpha;
+        }
+        s.phase = 1;
+    } else if (s.mode == 2) {
+        s.phase = 2;
+    }
+    if (q.sg == 0 && threadIdx.x == 0) {
+        s.mode = 0;
+        ob.st[i] = s;
+        ob.best_cost[i] = s.best_f;
+        if (s.phase == 2)
+            ob.done[i] = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small instances: the whole step in one block
+// ---------------------------------------------------------------------------
 
 // Block sum / max with the result in every thread (fixed order).  red holds
 // two sets of per-wave results used alternately (par flips per call, the same
@@ -83,22 +716,10 @@ __device__ __forceinline__ double block_max_all(double v, double *red, int &par)
     return block_reduce_all<true>(v, red, par);
 }
 
-// Buffers (doubles) per instance i with n = dim * ncoef parameters at offset
-// par_off: X, G, D, BX laid out like the parameters; S and Y hold `memory`
-// vectors per instance at offset memory * par_off; RHO memory scalars at
-// offset memory * i.  xt (the evaluation's parameter buffer), gt (its gradient
-// output) and costs are the evaluation's own buffers.
-struct OptBuffers {
-    double *X, *G, *D, *BX, *S, *Y, *RHO, *best_cost;
-    int *done;
-    OptState *st;
-    double *SY, *YY; // memory x memory pair products per instance (offset memory^2 * i), by ring slot
-};
-
-// Instances of up to kLbRegN parameters keep the block's vectors in registers
-// (element e = thread + k * kOptBlock, k < kLbK) and compute the direction in the
-// compact form below; larger ones run the two-loop recursion over global
-// memory in k_lbfgs_step.
+// Instances of up to kLbRegN parameters run the whole step in one block of
+// kOptBlock threads with the vectors in registers (element e = thread +
+// k * kOptBlock, k < kLbK): for them the step is a chain of short dependent
+// phases, faster within one block than split over launches.
 constexpr int kLbK = 4;
 constexpr long long kLbRegN = (long long)kLbK * kOptBlock;
 
@@ -119,12 +740,6 @@ __device__ __forceinline__ void lb_load(double *v, const double *__restrict__ sr
 // evaluation come from one pass (lbfgs_accept_pass) and one batched reduction, and the small triangular
 // systems are solved by the lanes of wave 0, instead of 2k + 1 dependent block reductions.  The pair
 // products s_i.y_j and y_i.y_j are kept per instance in memory x memory tables indexed by ring slot.
-constexpr int kPart = 6 + 4 * kMaxMemory;
-
-// part[] value indices: the new pair's s.y, y.y, s.s, s.g, y.g, max |g|, then per stored pair c (oldest
-// first) s_c.y, y_c.y, s_c.g, y_c.g
-enum { kPSY = 0, kPYY, kPSS, kPSG, kPYG, kPGMAX, kPPair };
-
 __device__ __forceinline__ double part_total(const double (*part)[kPart], int v)
 {
     double t = part[0][v];
@@ -143,16 +758,6 @@ __device__ __forceinline__ double part_max(const double (*part)[kPart], int v)
     return t;
 }
 
-__device__ __forceinline__ double readlane_d(double v, int lane)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
-// Pairs are read kLbP at a time, all loads of a group issued before any use,
-// so that a pass over k pairs waits about k / kLbP memory latencies.
-constexpr int kLbP = 8;
 
 __device__ __forceinline__ void lb_store(double *dst, const double *v, long long n)
 {
@@ -175,32 +780,6 @@ __device__ __forceinline__ void lb_load_group(double (*sc)[kLbK], double (*yc)[k
             lb_load(sc[t], S + (long long)slot * n, n);
             lb_load(yc[t], Y + (long long)slot * n, n);
         }
-}
-
-// Wave sums of 16 values at once (fixed order): each lane stores its 16 values into its wave's tb
-// (16 rows of 65 doubles), then lane 4v + q adds row v's 16 entries of lanes 16q .. 16q + 15 (starting
-// at a lane-dependent rotation, to spread the banks) and the quad combines its four partial sums.
-// Returns the wave sum of value lane / 4 (in all four lanes of the quad).  Contains two block barriers,
-// so every thread of the block calls it the same number of times.
-constexpr int kTbRow = 65;
-
-__device__ __forceinline__ double transpose_sum16(const double *v16, double *tb)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-        tb[v * kTbRow + lane] = v16[v];
-    __syncthreads();
-    const int v = lane >> 2, q = lane & 3;
-    const double *row = tb + v * kTbRow + q * 16;
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j)
-        acc += row[(j + 2 * q) & 15];
-    acc += dpp64<0xB1, 0xf>(acc, 0.0);
-    acc += dpp64<0x4E, 0xf>(acc, 0.0);
-    __syncthreads(); // tb free for the next call
-    return acc;
 }
 
 // Accepted step, vectors in registers (xt, x, gv, g: this thread's elements of XT, X, GT, G):
@@ -269,7 +848,7 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
 // oldest first): cu[i] = p_i, cw[i] = gamma w_i, so that H g = gamma g + sum_i (cu_i s_i - cw_i y_i).
 // Pair i's products with g come from part[] at stored index i + drop, or from the new pair's s.g, y.g
 // when `fresh` and i == k - 1.
-__device__ __forceinline__ void lbfgs_compact_coeffs(int memory, int k, int head, int drop, bool fresh,
+__device__ __forceinline__ void block_compact_coeffs(int memory, int k, int head, int drop, bool fresh,
                                                      const double (*part)[kPart], const double *mSY,
                                                      const double *mYY, double gamma, double *cu, double *cw)
 {
@@ -357,13 +936,17 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
     return block_sum_all(gd, red, rpar);
 }
 
+
+// One step for an instance of up to kLbRegN parameters, one block each (the
+// other instances return at once: they take k_lbfgs_pass1..3).  Same step as
+// the segmented passes, with the ring of `memory` slots, products summed
+// over the block's waves.
 __global__ void __launch_bounds__(kOptBlock)
-    k_lbfgs_step(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
-                 const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
-                 double first_step)
+    k_lbfgs_step_small(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
+                       const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
+                       double first_step)
 {
     __shared__ double red[2 * (kOptBlock / 64)];
-    __shared__ double alph[kMaxMemory];
     __shared__ double part[kOptBlock / 64][kPart];
     __shared__ double mSY[kMaxMemory * kMaxMemory], mYY[kMaxMemory * kMaxMemory];
     __shared__ double cu[kMaxMemory], cw[kMaxMemory];
@@ -373,24 +956,23 @@ __global__ void __launch_bounds__(kOptBlock)
     const int i = blockIdx.x;
     const InstDesc d = inst[i];
     const long long n = (long long)dim * d.cx * d.cy * d.cz;
-    const bool regs = n <= kLbRegN; // compact direction with the vectors in registers
-    double *SYi = ob.SY + (long long)memory * memory * i, *YYi = ob.YY + (long long)memory * memory * i;
-    bool fresh = false; // a pair was stored by this step (regs path)
+    if (n > kLbRegN)
+        return;
+    const int R = memory + 1; // allocation stride of the ring and the tables
+    double *SYi = ob.SY + (long long)R * R * i, *YYi = ob.YY + (long long)R * R * i;
+    bool fresh = false; // a pair was stored by this step
     int drop = 0;       // ... and replaced the oldest one
     const long long off = d.par_off;
     double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off, *BX = ob.BX + off;
-    double *S = ob.S + (long long)memory * off, *Y = ob.Y + (long long)memory * off;
-    double *RHO = ob.RHO + (long long)memory * i;
+    double *S = ob.S + (long long)R * off, *Y = ob.Y + (long long)R * off;
     double *XT = xt + off;
     const double *GT = gt + off;
-    // this thread's elements of XT, X, GT, G (regs path), loaded while the state is read
+    // this thread's elements of XT, X, GT, G, loaded while the state is read
     double rxt[kLbK], rx[kLbK], rgt[kLbK], rg[kLbK];
-    if (regs) {
-        lb_load(rxt, XT, n);
-        lb_load(rgt, GT, n);
-        lb_load(rx, X, n);
-        lb_load(rg, G, n);
-    }
+    lb_load(rxt, XT, n);
+    lb_load(rgt, GT, n);
+    lb_load(rx, X, n);
+    lb_load(rg, G, n);
     const double ft = costs[i];
     if (threadIdx.x == 0)
         s = ob.st[i];
@@ -398,16 +980,18 @@ __global__ void __launch_bounds__(kOptBlock)
     if (s.phase == 2)
         return;
     const bool finite = isfinite(ft);
-    bool search = false;      // compute a new direction
+    bool search = false; // compute a new direction
     bool first = false;
     bool finish = false;
     bool backtrack = false;
     if (s.phase == 0) {
-        for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-            X[e] = XT[e];
-            G[e] = GT[e];
-            BX[e] = XT[e];
-        }
+        lb_store(X, rxt, n);
+        lb_store(G, rgt, n);
+        lb_store(BX, rxt, n);
+        double gmax = 0.0;
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k)
+            gmax = fmax(gmax, fabs(rgt[k]));
         if (threadIdx.x == 0) {
             s.f = ft;
             s.evals = 1;
@@ -415,12 +999,8 @@ __global__ void __launch_bounds__(kOptBlock)
             s.k = 0;
             s.head = 0;
         }
-        __syncthreads();
-        double gmax = 0.0;
-        for (long long e = threadIdx.x; e < n; e += kOptBlock)
-            gmax = fmax(gmax, fabs(GT[e]));
         gmax = block_max_all(gmax, red, rpar);
-        if (!finite || gmax <= 0.0 || s.evals >= maxeval)
+        if (!finite || gmax <= 0.0 || 1 >= maxeval)
             finish = true;
         else {
             search = true;
@@ -429,12 +1009,8 @@ __global__ void __launch_bounds__(kOptBlock)
     } else {
         const double f0 = s.f, a0 = s.alpha, gd0 = s.gd;
         const bool better = finite && ft < s.best_f;
-        if (better && regs) {
+        if (better)
             lb_store(BX, rxt, n);
-        } else if (better) {
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                BX[e] = XT[e];
-        }
         __syncthreads();
         if (threadIdx.x == 0) {
             s.evals += 1;
@@ -443,7 +1019,7 @@ __global__ void __launch_bounds__(kOptBlock)
         }
         __syncthreads();
         const bool armijo = finite && ft <= f0 + 1e-4 * a0 * gd0;
-        if (armijo && regs) {
+        if (armijo) {
             for (int t = threadIdx.x; t < memory * memory; t += kOptBlock) {
                 mSY[t] = SYi[t];
                 mYY[t] = YYi[t];
@@ -458,15 +1034,8 @@ __global__ void __launch_bounds__(kOptBlock)
                 fresh = true;
                 drop = kk < memory ? 0 : 1;
                 const int ns = kk < memory ? (head + kk) % memory : head;
-                double *Sj = S + (long long)ns * n, *Yj = Y + (long long)ns * n;
-#pragma unroll
-                for (int k = 0; k < kLbK; ++k) {
-                    const long long e = threadIdx.x + (long long)k * kOptBlock;
-                    if (e < n) {
-                        Sj[e] = sv[k];
-                        Yj[e] = yv[k];
-                    }
-                }
+                lb_store(S + (long long)ns * n, sv, n);
+                lb_store(Y + (long long)ns * n, yv, n);
                 // the new pair's products with the pairs that stay
                 if (threadIdx.x < kk - drop) {
                     const int jo = threadIdx.x + drop;
@@ -479,54 +1048,12 @@ __global__ void __launch_bounds__(kOptBlock)
                 if (threadIdx.x == 0) {
                     mSY[ns * memory + ns] = SYi[ns * memory + ns] = sy;
                     mYY[ns * memory + ns] = YYi[ns * memory + ns] = yy;
-                    RHO[ns] = 1.0 / sy;
                     if (s.k < memory)
                         s.k += 1;
                     else
                         s.head = (s.head + 1) % memory;
                 }
             }
-            if (threadIdx.x == 0)
-                s.f = ft;
-            __syncthreads();
-            if (s.evals >= maxeval || gmax <= 0.0 || f0 == ft)
-                finish = true;
-            else
-                search = true;
-        } else if (armijo) {
-            double sy = 0.0, yy = 0.0, ss = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-                const double se = XT[e] - X[e], ye = GT[e] - G[e];
-                sy += se * ye;
-                yy += ye * ye;
-                ss += se * se;
-            }
-            sy = block_sum_all(sy, red, rpar);
-            yy = block_sum_all(yy, red, rpar);
-            ss = block_sum_all(ss, red, rpar);
-            if (sy > 1e-12 * sqrt(ss * yy)) {
-                const int slot = s.k < memory ? (s.head + s.k) % memory : s.head;
-                double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
-                for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-                    Sj[e] = XT[e] - X[e];
-                    Yj[e] = GT[e] - G[e];
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    RHO[slot] = 1.0 / sy;
-                    if (s.k < memory)
-                        s.k += 1;
-                    else
-                        s.head = (s.head + 1) % memory;
-                }
-            }
-            double gmax = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-                X[e] = XT[e];
-                G[e] = GT[e];
-                gmax = fmax(gmax, fabs(GT[e]));
-            }
-            gmax = block_max_all(gmax, red, rpar);
             if (threadIdx.x == 0)
                 s.f = ft;
             __syncthreads();
@@ -541,7 +1068,7 @@ __global__ void __launch_bounds__(kOptBlock)
         }
     }
     if (backtrack) {
-        double a = s.alpha;
+        const double a = s.alpha;
         double an = 0.5 * a;
         if (finite) {
             const double den = 2.0 * (ft - s.f - a * s.gd);
@@ -552,17 +1079,12 @@ __global__ void __launch_bounds__(kOptBlock)
         if (an < 1e-20) {
             finish = true;
         } else {
-            if (regs) {
-                double xv[kLbK], dv[kLbK];
-                lb_load(dv, D, n);
+            double xv[kLbK], dv[kLbK];
+            lb_load(dv, D, n);
 #pragma unroll
-                for (int k = 0; k < kLbK; ++k)
-                    xv[k] = rx[k] + an * dv[k];
-                lb_store(XT, xv, n);
-            } else {
-                for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                    XT[e] = X[e] + an * D[e];
-            }
+            for (int k = 0; k < kLbK; ++k)
+                xv[k] = rx[k] + an * dv[k];
+            lb_store(XT, xv, n);
             __syncthreads();
             if (threadIdx.x == 0)
                 s.alpha = an;
@@ -570,103 +1092,51 @@ __global__ void __launch_bounds__(kOptBlock)
     }
     if (search) {
         const int k = s.k, head = s.head;
-        double gd;
-        double dv[kLbK]; // the direction's elements of this thread (regs path)
-        if (regs) {
-            double sc[kLbP][kLbK], yc[kLbP][kLbK];
-            lb_load_group(sc, yc, 0, k, head, memory, S, Y, n);
-            double gamma = 1.0;
-            if (k > 0) {
-                const int sl = (head + k - 1) % memory;
-                gamma = mSY[sl * memory + sl] / mYY[sl * memory + sl];
-                if (threadIdx.x < 64)
-                    lbfgs_compact_coeffs(memory, k, head, drop, fresh, part, mSY, mYY, gamma, cu, cw);
-                __syncthreads();
-            }
-            gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
-        } else {
-        // two-loop recursion on q (kept in D), newest pair first
-        for (long long e = threadIdx.x; e < n; e += kOptBlock)
-            D[e] = G[e];
-        __syncthreads();
-        for (int j = k - 1; j >= 0; --j) {
-            const int slot = (head + j) % memory;
-            const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
-            double a = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                a += Sj[e] * D[e];
-            a = block_sum_all(a, red, rpar) * RHO[slot];
-            if (threadIdx.x == 0)
-                alph[j] = a;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                D[e] -= a * Yj[e];
-            __syncthreads();
-        }
+        double dv[kLbK]; // the direction's elements of this thread
+        double sc[kLbP][kLbK], yc[kLbP][kLbK];
+        lb_load_group(sc, yc, 0, k, head, memory, S, Y, n);
+        double gamma = 1.0;
         if (k > 0) {
-            const int slot = (head + k - 1) % memory;
-            const double *Yj = Y + (long long)slot * n;
-            double yy = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                yy += Yj[e] * Yj[e];
-            yy = block_sum_all(yy, red, rpar);
-            const double gamma = 1.0 / (RHO[slot] * yy);
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                D[e] *= gamma;
+            const int sl = (head + k - 1) % memory;
+            gamma = mSY[sl * memory + sl] / mYY[sl * memory + sl];
+            if (threadIdx.x < 64)
+                block_compact_coeffs(memory, k, head, drop, fresh, part, mSY, mYY, gamma, cu, cw);
             __syncthreads();
         }
-        for (int j = 0; j < k; ++j) {
-            const int slot = (head + j) % memory;
-            const double *Sj = S + (long long)slot * n, *Yj = Y + (long long)slot * n;
-            double bsum = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                bsum += Yj[e] * D[e];
-            const double b = block_sum_all(bsum, red, rpar) * RHO[slot];
-            const double c = alph[j] - b;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                D[e] += c * Sj[e];
-            __syncthreads();
-        }
-        gd = 0.0;
-        for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-            D[e] = -D[e];
-            gd += G[e] * D[e];
-        }
-        gd = block_sum_all(gd, red, rpar);
-        }
+        double gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
-            gd = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock) {
-                D[e] = -G[e];
-                gd += G[e] * D[e];
+            double g2 = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 < kLbK; ++k2) {
+                dv[k2] = -rgt[k2];
+                const long long e = threadIdx.x + (long long)k2 * kOptBlock;
+                if (e < n)
+                    g2 += rgt[k2] * dv[k2];
             }
-            gd = block_sum_all(gd, red, rpar);
+            lb_store(D, dv, n);
+            gd = block_sum_all(g2, red, rpar);
             if (threadIdx.x == 0) {
                 s.k = 0;
                 s.head = 0;
             }
             first = true;
-            if (regs)
-                lb_load(dv, D, n);
         }
         double alpha = 1.0;
         if (first) {
             double dmax = 0.0;
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                dmax = fmax(dmax, fabs(D[e]));
+#pragma unroll
+            for (int k2 = 0; k2 < kLbK; ++k2)
+                dmax = fmax(dmax, fabs(dv[k2]));
             dmax = block_max_all(dmax, red, rpar);
             if (dmax > 0.0)
                 alpha = fmin(1.0, first_step / dmax);
         }
-        if (regs) { // X is XT of this evaluation here (accepted step or first evaluation)
-            double xv[kLbK];
+        // X is XT of this evaluation here (accepted step or first evaluation)
+        double xv[kLbK];
 #pragma unroll
-            for (int k2 = 0; k2 < kLbK; ++k2)
-                xv[k2] = rxt[k2] + alpha * dv[k2];
-            lb_store(XT, xv, n);
-        } else {
-            for (long long e = threadIdx.x; e < n; e += kOptBlock)
-                XT[e] = X[e] + alpha * D[e];
-        }
+        for (int k2 = 0; k2 < kLbK; ++k2)
+            xv[k2] = rxt[k2] + alpha * dv[k2];
+        lb_store(XT, xv, n);
         __syncthreads();
         if (threadIdx.x == 0) {
             s.alpha = alpha;
@@ -675,9 +1145,10 @@ __global__ void __launch_bounds__(kOptBlock)
         }
     }
     if (finish) {
-        for (long long e = threadIdx.x; e < n; e += kOptBlock)
-            XT[e] = BX[e];
+        double bx[kLbK];
+        lb_load(bx, BX, n);
         __syncthreads();
+        lb_store(XT, bx, n);
         if (threadIdx.x == 0) {
             s.phase = 2;
             ob.done[i] = 1;
