@@ -1674,18 +1674,23 @@ __device__ __forceinline__ float lane_prev(float v, float edge)
 // their four z knots are cell .. cell + 3) and the rows of the band; one
 // thread per x column (nx <= kBlock).  Each thread keeps its column of W for
 // the band's rows and the rows either side in registers for three slices
-// (z - 1, z, z + 1) and walks z, loading one new slice of W and of R per
-// slice, a slice ahead of its use (buffer loads at scalar row offsets).  The
-// x neighbours come from the neighbouring lanes (DPP); the columns at a
-// wave's edges are exchanged through LDS.  The gradient rule, residual and
-// force are those of k_stage1_3d.  f * wy_t(j) accumulates over the band's
-// rows; at the end of the slice the 12 sums (t, component) are folded with
+// (z - 1, z, z + 1) and walks z, loading at each step the next slice of W and
+// the slice's rows of R (buffer loads at a scalar slice offset plus a scalar
+// row offset fixed for the block).  Nothing is loaded ahead: the four waves
+// per SIMD cover each other's loads, and the registers a prefetch would hold
+// are what keeps the kernel free of spills.  The x neighbours come from the
+// neighbouring lanes (DPP); the wave's edge lanes load the column beyond the
+// wave themselves, so the waves of a block never wait for each other during
+// the walk.  The gradient rule, residual and force are those of k_stage1_3d.
+// f * wy_t(j) accumulates over the band's rows (y weights read from LDS per
+// row); at the end of the slice the 12 sums (t, component) are folded with
 // the slice's four z weights into 48 per column (k = kz - cell, t,
 // component), kept in registers across the cell.  The x contraction then runs
 // through LDS once per k with contract_x's two fixed-order steps.  Output of
 // the block: P[cell][band][k][t][kx][comp] (48 * cx floats), so the reduce
-// reads every partial sum once.  MAXR >= band rows + 2.
-template <int DEG, int MAXR>
+// reads every partial sum once.  MAXR >= band rows + 2.  THR: autothresh on
+// (the mask and its count; without it every voxel counts and no count is kept).
+template <int DEG, int MAXR, bool THR>
 __global__ void __launch_bounds__(kBlock, 4)
     k_stage1z_3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int nblocks,
                  const float4 *__restrict__ tw, const int *__restrict__ ts, const int *__restrict__ cells,
@@ -1696,7 +1701,7 @@ __global__ void __launch_bounds__(kBlock, 4)
     extern __shared__ float smem[];
     __shared__ double red[NW];
     __shared__ unsigned redu[NW];
-    __shared__ float edge_s[2][NW][2][MAXR]; // [slice parity][wave][lane 0 / lane 63][window row]
+    __shared__ float4 wy_s[MAXR];            // y weights of the window rows (read per row, not held)
     const int b = xcd_remap(blockIdx.x, nblocks);
     const int i = find_inst(prefix, n_inst, b);
     const InstDesc d = inst[i];
@@ -1721,27 +1726,36 @@ __global__ void __launch_bounds__(kBlock, 4)
     }
     for (int k = threadIdx.x; k <= cx; k += kBlock)
         cl_s[k] = cells[d.cell_x + k];
-    const int x = threadIdx.x;
-    const int lane = x & 63, wv = x >> 6;
-    const bool ix = x > 0 && x < nx - 1;
     const int nrw = j1 - j0 + 2; // window rows j0 - 1 .. j1 (clamped to the image)
+    if (threadIdx.x >= 1 && threadIdx.x + 1 < nrw)
+        wy_s[threadIdx.x] = tw[d.tab_y + j0 + threadIdx.x - 1];
+    const int x = threadIdx.x;
+    const int lane = x & 63;
+    const bool ix = x > 0 && x < nx - 1;
     const __amdgpu_buffer_rsrc_t rwi = raw_rsrc(W + d.img_off), rri = raw_rsrc(R + d.img_off);
     const unsigned vo = (unsigned)min(x, nx - 1) * 4u;
-    // byte offset of window row r of slice z (slices and rows clamped)
-    auto soff = [&](int z, int r) {
-        const int zc = min(max(z, 0), nz - 1), jr = min(max(j0 - 1 + r, 0), ny - 1);
-        return (unsigned)((zc * ny + jr) * nx) * 4u;
-    };
-    float wm[MAXR], wc[MAXR], wp[MAXR], wn[MAXR], rc[MAXR], rn[MAXR];
+    // the column beyond the wave for its edge lanes (x - 1 for lane 0, x + 1
+    // for lane 63, clamped; the others read their own column)
+    const unsigned veo = lane == 0 ? (unsigned)max(x - 1, 0) * 4u
+                                   : lane == 63 ? (unsigned)min(x + 1, nx - 1) * 4u : vo;
+    // byte offset of window row r within a slice (rows clamped; the same for
+    // every slice of the block, so these stay in scalar registers) and of
+    // slice z (clamped): each load adds the two
+    const unsigned planeb = (unsigned)(nx * ny) * 4u;
+    unsigned roff[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r)
+        roff[r] = (unsigned)(min(max(j0 - 1 + r, 0), ny - 1) * nx) * 4u;
+    auto zoff = [&](int z) { return (unsigned)min(max(z, 0), nz - 1) * planeb; };
+    auto soff = [&](int z, int r) { return zoff(z) + roff[r]; };
+    float wm[MAXR], wc[MAXR], wp[MAXR], rc[MAXR];
 #pragma unroll
     for (int r = 0; r < MAXR; ++r) {
-        wm[r] = wc[r] = wp[r] = wn[r] = rc[r] = rn[r] = 0.0f;
+        wm[r] = wc[r] = wp[r] = rc[r] = 0.0f;
         if (r < nrw) {
             wm[r] = ld_buf(rwi, vo, soff(zb - 1, r));
             wc[r] = ld_buf(rwi, vo, soff(zb, r));
             wp[r] = ld_buf(rwi, vo, soff(zb + 1, r));
-            if (r >= 1 && r < nrw - 1)
-                rc[r] = ld_buf(rri, vo, soff(zb, r));
         }
     }
     double ssd = 0.0;
@@ -1752,38 +1766,31 @@ __global__ void __launch_bounds__(kBlock, 4)
         a48[k] = 0.0f;
     // lanes beyond the image count nothing and their force is never stored
     const bool xin = x < nx;
-    // edge column read by this lane: lane 0 the previous wave's lane 63,
-    // lane 63 the next wave's lane 0 (the DPP shifts below use it only there;
-    // at the block's edges the lane is on the image border, where the x
-    // difference is zeroed, so any value will do)
-    const int ew = lane == 0 ? max(wv - 1, 0) : (lane == 63 ? min(wv + 1, NW - 1) : wv);
-    const int es = lane == 0 ? 1 : 0;
     for (int z = zb; z < ze; ++z) {
-        // next slice's loads: W of z + 2, R of z + 1
-        if (z + 1 < ze) {
-#pragma unroll
-            for (int r = 0; r < MAXR; ++r) {
-                if (r < nrw) {
-                    wn[r] = ld_buf(rwi, vo, soff(z + 2, r));
-                    if (r >= 1 && r < nrw - 1)
-                        rn[r] = ld_buf(rri, vo, soff(z + 1, r));
-                }
-            }
-        }
-        // the wave's edge columns of slice z for the neighbouring waves (two
-        // buffers by slice parity: a buffer is rewritten only after the
-        // barrier of the following slice, when every read of it is done)
-        const int par = (z - zb) & 1;
-        if (lane == 0 || lane == 63) {
+        // this step's loads: W of slice z + 1 (the cell's first step has
+        // it from the prologue), R of slice z and the edge lanes' columns
+        if (z > zb) {
 #pragma unroll
             for (int r = 0; r < MAXR; ++r)
-                edge_s[par][wv][lane == 63][r] = wc[r];
+                if (r < nrw)
+                    wp[r] = ld_buf(rwi, vo, soff(z + 1, r));
         }
-        __syncthreads();
+#pragma unroll
+        for (int r = 1; r + 1 < MAXR; ++r)
+            if (r + 1 < nrw)
+                rc[r] = ld_buf(rri, vo, soff(z, r));
+        // lane 0: W at x - 1, lane 63: W at x + 1 (the DPP shifts below use
+        // it only there; at the image's edges the x difference is zeroed)
+        float ev[MAXR];
+#pragma unroll
+        for (int r = 1; r + 1 < MAXR; ++r)
+            ev[r] = r + 1 < nrw ? ld_buf(rwi, veo, soff(z, r)) : 0.0f;
+        if (z == zb)
+            __syncthreads(); // the y weights in LDS
         const float4 wzv = tw[d.tab_z + z];
         const float wzs[4] = {wzv.x, wzv.y, wzv.z, wzv.w};
         const bool iz = z > 0 && z < nz - 1;
-        const float *er = edge_s[par][ew][es];
+
         float acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k)
@@ -1793,15 +1800,17 @@ __global__ void __launch_bounds__(kBlock, 4)
             if (r + 1 < nrw) {
                 const int j = j0 + r - 1;
                 const float w = wc[r];
-                const float e = er[r];
+                const float e = ev[r];
                 const float wxm = lane_prev(w, e), wxp = lane_next(w, e);
-                const float4 wyv = tw[d.tab_y + j];
+                const float4 wyv = wy_s[r];
                 const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
                 const bool iy = j > 0 && j < ny - 1;
-                const bool on = xin && (thr == 0.0f || w > thr);
+                // without autothresh every voxel counts and the count is unused
+                const bool on = THR ? xin && w > thr : xin;
                 const float dl = on ? w - rc[r] : 0.0f;
                 ssd += (double)dl * (double)dl;
-                cnt_w += (unsigned)__popcll(__ballot(on));
+                if constexpr (THR)
+                    cnt_w += (unsigned)__popcll(__ballot(on));
                 // force = gradient * difference with the reference's 0.5 on
                 // the difference: the same products as (0.5 * (w+ - w-)) * dl,
                 // 0.5 being exact; the y and z components are zero on the
@@ -1823,12 +1832,12 @@ __global__ void __launch_bounds__(kBlock, 4)
 #pragma unroll
             for (int m = 0; m < 12; ++m)
                 a48[12 * k + m] = fmaf(wzs[k], acc[m], a48[12 * k + m]);
+        // (rows 0 and MAXR - 1 of the slice below and of R are never read)
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
-            wm[r] = wc[r];
+            if (r >= 1 && r + 1 < MAXR)
+                wm[r] = wc[r];
             wc[r] = wp[r];
-            wp[r] = wn[r];
-            rc[r] = rn[r];
         }
     }
     const double s = block_sum(ssd, red);

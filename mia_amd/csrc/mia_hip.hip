@@ -529,7 +529,9 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
     }
     if (n_z > 0) {
         const int nb = pl.s1z_blocks;
-        auto kern = pl.degree == 3 ? k_stage1z_3d<3, kS1zRows> : k_stage1z_3d<1, kS1zRows>;
+        const bool thr = pl.thr != 0.0f;
+        auto kern = pl.degree == 3 ? (thr ? k_stage1z_3d<3, kS1zRows, true> : k_stage1z_3d<3, kS1zRows, false>)
+                                   : (thr ? k_stage1z_3d<1, kS1zRows, true> : k_stage1z_3d<1, kS1zRows, false>);
         launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1z_prefix, nb, pl.tw, pl.ts, pl.cells,
                (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd + si.zoff, pl.cnt + si.zoff);
         LAUNCH_CHECK("stage1");
@@ -660,13 +662,16 @@ int set_stage1_lds(int dim, int max_nx, int max_cx, int max_ncx)
                                 {(const void *)k_stage1_3d<3, 1>, (const void *)k_stage1_3d<1, 1>}};
         const void *fn2[2][2] = {{(const void *)k_stage1_2d<3, 0>, (const void *)k_stage1_2d<1, 0>},
                                  {(const void *)k_stage1_2d<3, 1>, (const void *)k_stage1_2d<1, 1>}};
-        const void *fnz[2] = {(const void *)k_stage1z_3d<3, kS1zRows>, (const void *)k_stage1z_3d<1, kS1zRows>};
+        const void *fnz[4] = {(const void *)k_stage1z_3d<3, kS1zRows, false>,
+                              (const void *)k_stage1z_3d<1, kS1zRows, false>,
+                              (const void *)k_stage1z_3d<3, kS1zRows, true>,
+                              (const void *)k_stage1z_3d<1, kS1zRows, true>};
         if (lds > 64 * 1024) {
             for (int k = 0; k < 2; ++k)
                 HIP_TRY(hipFuncSetAttribute(dim == 3 ? fn[exact][k] : fn2[exact][k],
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             if (dim == 3 && !exact)
-                for (int k = 0; k < 2; ++k)
+                for (int k = 0; k < 4; ++k)
                     HIP_TRY(hipFuncSetAttribute(fnz[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
     }
