@@ -329,11 +329,11 @@ def registration_3d(threads):
     N(0, 1.5) clipped to 5 voxels); three levels (1/4, 1/2, full), cost
     `image:cost=[ssd]` plus the divcurl penalty weight 0.01, spline rate 8,
     L-BFGS with 200 evaluations per minimisation.  Per level: GPU wall time,
-    evaluations, cost; and the CPU port's time for one cost + gradient
-    evaluation at that level's size, times the level's evaluations (an
-    estimate of the CPU path's level time without the penalty)."""
+    evaluations, cost.  CPU baseline: the same registration run by the CPU
+    port (FP32 MIA data flow with its pfor threading, the same L-BFGS, divcurl
+    with the reference's pfor over z) on `threads` host threads, timed whole."""
     import mia_amd
-    from mia_amd import level, series, synth
+    from mia_amd import series, synth
     shape = (160, 192, 192)
     fix = synth.smoothed_noise(shape, 4.0, 21)
     cs = tuple(mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1])
@@ -351,31 +351,35 @@ def registration_3d(threads):
     c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
     log("registration 3D: %.2f s, final cost %.6g (identity %.6g)" % (wall, final, c_id))
     L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
-    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
     fp = np.ctypeslib.ndpointer(np.float32, flags="C")
-    L.mcp_cost_grad_3d.argtypes = [fp, fp] + [ctypes.c_int] * 6 + [dp, ctypes.c_int, dp, dp]
-    L.mcp_cost_grad_3d.restype = ctypes.c_double
+    L.mcp_register_3d.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.mcp_register_3d.restype = ctypes.c_double
+    lsec = np.zeros(3)
+    lev = np.zeros(3, dtype=np.int32)
+    lcost = np.zeros(3)
+    cfin = ctypes.c_double()
+    ccs = np.zeros(3, dtype=np.int32)
+    cpu_sec = L.mcp_register_3d(np.ascontiguousarray(mov, dtype=np.float32), np.ascontiguousarray(fix),
+                                shape[2], shape[1], shape[0], 8.0, 3, 200, 0.01, threads, lsec.ctypes.data,
+                                lev.ctypes.data, lcost.ctypes.data, ctypes.byref(cfin), ccs.ctypes.data)
+    log("registration 3D, CPU port (%d threads): %.2f s, final cost %.6g" % (threads, cpu_sec, cfin.value))
     levels = []
-    for st in stats:
-        sz = st["size"]
-        lshape = (sz[2], sz[1], sz[0])
-        la = np.ascontiguousarray(level.scale_image(a, lshape) if lshape != a.shape else a)
-        lb = np.ascontiguousarray(level.scale_image(b, lshape) if lshape != b.shape else b)
-        gcs = st["coeff_size"]
-        q = np.zeros(3 * gcs[0] * gcs[1] * gcs[2])
-        g = np.zeros_like(q)
-        tm = np.zeros(7)
-        L.mcp_cost_grad_3d(la, lb, sz[0], sz[1], sz[2], gcs[0], gcs[1], gcs[2], q, threads, g, tm)
-        t1 = time.perf_counter()
-        L.mcp_cost_grad_3d(la, lb, sz[0], sz[1], sz[2], gcs[0], gcs[1], gcs[2], q, threads, g, tm)
-        t_eval = time.perf_counter() - t1
-        levels.append({"size": list(sz), "coeff_size": list(gcs), "evaluations": st["evaluations"],
-                       "gpu_seconds": round(st["seconds"], 4), "cost": st["cost"],
-                       "cpu_port_seconds_per_evaluation": round(t_eval, 5),
-                       "cpu_port_level_seconds_estimate": round(t_eval * st["evaluations"], 3)})
+    for k, st in enumerate(stats):
+        levels.append({"size": list(st["size"]), "coeff_size": list(st["coeff_size"]),
+                       "evaluations": st["evaluations"], "gpu_seconds": round(st["seconds"], 4), "cost": st["cost"],
+                       "gpu_ms_per_evaluation": round(1e3 * st["seconds"] / max(st["evaluations"], 1), 4),
+                       "cpu_port_seconds": round(float(lsec[k]), 3), "cpu_port_evaluations": int(lev[k]),
+                       "cpu_port_cost": float(lcost[k])})
     return {"workload": "3D 192x192x160, 3 levels, spline rate 8 + divcurl 0.01, ssd, lbfgs maxiter=200",
             "seconds": round(wall, 3), "final_cost": final, "identity_cost": c_id,
-            "coeff_size": list(grid), "cpu_threads": threads, "levels": levels}
+            "coeff_size": list(grid), "levels": levels,
+            "cpu_port": {"seconds": round(cpu_sec, 3), "threads": threads, "kind": "port",
+                         "final_cost": cfin.value, "coeff_size": [int(v) for v in ccs],
+                         "sample": "the whole registration, FP32 port of MIA's 3D data flow and pfor threading, "
+                                   "same L-BFGS"},
+            "speedup_vs_cpu_port": round(cpu_sec / wall, 1)}
 
 
 def copy_calibration(mib, reps):

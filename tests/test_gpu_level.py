@@ -171,3 +171,41 @@ def test_nonrigid_register_device_matches_host_3d():
         assert a["cost"] == pytest.approx(b["cost"], rel=1e-9)
     assert fd == pytest.approx(fh, rel=1e-9)
     assert np.abs(pd - ph).max() <= 1e-7 * max(1.0, np.abs(ph).max())
+
+
+def test_nonrigid_register_matches_cpu_port():
+    """The device driver against the CPU port's multiresolution registration
+    (the bench's CPU baseline: FP32 MIA data flow, the same L-BFGS, divcurl
+    evaluated in FP64 on the host) on the same two-level problem: the same
+    level sizes, coefficient grids and evaluation counts, and final costs equal
+    to FP32 evaluation rounding."""
+    import ctypes
+    import mia_amd
+    from mia_amd import series, synth
+    shape = (40, 44, 48)
+    fix = synth.smoothed_noise(shape, 3.0, 31)
+    cs = tuple(mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1])
+    q = synth.random_spline_params(cs, 1.5, 3.0, 32)
+    mov = synth.spline_warp(fix, cs, q)
+    tr = "spline:rate=8,penalty=[divcurl:weight=0.01]"
+    pd, gd, sd, fd = series.nonrigid_register(mov, fix, tr, "image:cost=[ssd]", "lbfgs:maxiter=8", levels=2)
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    L.mcp_register_3d.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.mcp_register_3d.restype = ctypes.c_double
+    lsec = np.zeros(2)
+    lev = np.zeros(2, dtype=np.int32)
+    lcost = np.zeros(2)
+    cf = ctypes.c_double()
+    ccs = np.zeros(3, dtype=np.int32)
+    L.mcp_register_3d(np.ascontiguousarray(mov, dtype=np.float32), np.ascontiguousarray(fix), shape[2], shape[1],
+                      shape[0], 8.0, 2, 8, 0.01, 4, lsec.ctypes.data, lev.ctypes.data, lcost.ctypes.data, ctypes.byref(cf),
+                      ccs.ctypes.data)
+    print("device", [s["evaluations"] for s in sd], sd[-1]["cost"], "cpu port", list(lev), cf.value)
+    assert tuple(int(v) for v in ccs) == tuple(gd)
+    assert [s["evaluations"] for s in sd] == [int(v) for v in lev]
+    for s, c in zip(sd, lcost):
+        assert s["cost"] == pytest.approx(c, rel=1e-5)
+    assert sd[-1]["cost"] == pytest.approx(cf.value, rel=1e-5)
