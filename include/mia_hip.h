@@ -84,9 +84,11 @@ int mia_hip_set_images_device(mia_hip_ctx *ctx, int inst, const void *d_moving, 
 /* Which reference-gradient stage-1 layout instance `inst` runs with after
  * mia_hip_setup: 1 = per (z knot cell, y knot band) with the slices' W in
  * registers (3D instances with x rows up to 256 voxels and y bands of up to
- * 9 rows), 0 = per (slice, y band) (2D, exact mode, larger rows).  The
- * choice depends only on the instance's own sizes, so an instance computes
- * the same in any batch.  Counterpart of the reference's translate path
+ * 9 rows, where the tuned or forced layout asks for it:
+ * MIA_HIP_TUNE_STAGE1, mia_hip_set_stage1_path), 0 = per (slice, y band)
+ * (2D, exact mode, larger rows, or chosen).  The choice depends only on the
+ * instance's own sizes, so an instance computes the same in any batch.
+ * Counterpart of the reference's translate path
  * (mia/3d/transform/spline.cc:731-791); introspection only. */
 int mia_hip_get_stage1_path(mia_hip_ctx *ctx, int inst, int *path);
 
@@ -169,36 +171,65 @@ int mia_hip_profile_enable(mia_hip_ctx *ctx, int enable);
 int mia_hip_profile_read(mia_hip_ctx *ctx, double *ms, int *launches, int max_stages, int *n_stages);
 const char *mia_hip_stage_name(int stage);
 
-/* ---- 3D warp tile variants and their autotuner ------------------------- */
+/* ---- tile variants, stage-1 layouts and their autotuner ---------------- */
+
+/* Kinds of tuned choices, each made per (interpolation degree, size bucket
+ * ceil(log2(voxels or pixels))) from a table keyed by the GPU architecture. */
+#define MIA_HIP_TUNE_WARP3D 0 /* 3D warp tile variant (mia_hip_warp_variants) */
+#define MIA_HIP_TUNE_WARP2D 1 /* 2D warp tile variant (mia_hip_warp2d_variants) */
+#define MIA_HIP_TUNE_STAGE1 2 /* 3D reference-gradient stage-1 layout: 0 per (slice, y band) with LDS
+                                 accumulators, 1 per (z knot cell, y band) with the band in registers
+                                 (only where x size <= 256 and a band has at most 9 rows) */
+
+/* The GPU architecture of `device` ("gfx950"; the name the tuning table is
+ * keyed by) into buf.  No counterpart in the reference. */
+int mia_hip_device_arch(int device, char *buf, int len);
 
 /* The tile shapes (x, y, z voxels per block) of the 3D warp kernel's
  * variants; tile_dims receives 3 ints per variant (up to max_variants). */
 int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants);
+/* The tile shapes (x, y pixels per block) of the 2D warp kernel's variants;
+ * tile_dims receives 2 ints per variant (up to max_variants). */
+int mia_hip_warp2d_variants(int *tile_dims, int max_variants, int *n_variants);
 
-/* Force tile variant `variant` (0..n-1) for the 3D instances of the next
- * mia_hip_setup on this context, or -1 (default) for the tuned choice per
- * size bucket.  Results do not depend on the variant beyond FP32 rounding of
- * the displacement contraction; every instance always uses the same variant
- * for a given size, so batched and per-instance runs stay bitwise equal. */
+/* Force tile variant `variant` (0..n-1) for the instances of the next
+ * mia_hip_setup on this context (3D or 2D variants by the setup's dim), or
+ * -1 (default) for the tuned choice per size bucket.  Results do not depend
+ * on the variant beyond FP32 rounding of the displacement contraction;
+ * every instance always uses the same variant for a given size, so batched
+ * and per-instance runs stay bitwise equal. */
 int mia_hip_set_warp_variant(mia_hip_ctx *ctx, int variant);
 /* The variant instance `inst` runs with after mia_hip_setup. */
 int mia_hip_get_warp_variant(mia_hip_ctx *ctx, int inst, int *variant);
+/* Force the stage-1 layout of 3D reference gradients for the next
+ * mia_hip_setup: 0 per slice, 1 per z knot cell where the instance allows
+ * it, -1 (default) the tuned choice per size bucket.  The layouts sum the
+ * gradient in different orders (equal to FP rounding); the choice depends
+ * on the instance's size only, so batches stay bitwise equal to single
+ * instances.  mia_hip_get_stage1_path reports the layout taken. */
+int mia_hip_set_stage1_path(mia_hip_ctx *ctx, int path);
 
-/* Time every variant on synthetic 3D instances of 2^b voxels (cubes), b in
- * [b_min, b_max], spline rate 6 with displacements N(0, 1.5) clipped to 4
- * voxels, for interpolation degree 3, 1 or both (0); choose the fastest per
- * (degree, b), write the table as JSON keyed by the device's gfx
- * architecture to json_path (may be NULL) and use it from then on.  us_out
- * (may be NULL): mean microseconds per warp launch, n_degrees x n_buckets x
- * n_variants. */
+/* Time every choice of `kind` (MIA_HIP_TUNE_*) on synthetic instances of 2^b
+ * voxels or pixels (cubes / squares), b in [b_min, b_max]: 3D warp at spline
+ * rate 6 (b in [12, 26]), 2D warp at rate 8 (b in [10, 24]), stage-1 layout
+ * at rate 8 (b in [12, 24]: x sizes up to 256); displacements N(0, 1.5)
+ * clipped to 4 voxels; for interpolation degree 3, 1 or both (0).  The
+ * fastest choice per (degree, b) replaces the entry of that kind, degree
+ * and bucket in the table json_path (may be NULL: nothing written; the
+ * file's other entries are kept when it holds a table of this
+ * architecture), which is used from then on.  us_out (may be NULL): mean microseconds per launch (stage 1 plus
+ * reduce for the layout), n_degrees x n_buckets x n_choices. */
+int mia_hip_autotune(int device, int kind, int degree, int b_min, int b_max, const char *json_path,
+                     double *us_out);
+/* mia_hip_autotune(device, MIA_HIP_TUNE_WARP3D, ...). */
 int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const char *json_path, double *us_out);
 
-/* (Re)load the tuning table: json_path, or NULL for $MIA_HIP_WARP_TUNING,
- * else <library dir>/../tuning/warp3d_<arch>.json.  A table for another
- * architecture is ignored (variant 0 everywhere). */
+/* (Re)load the tuning table: json_path, or NULL for $MIA_HIP_TUNING, else
+ * <library dir>/../tuning/<arch>.json.  A table for another architecture is
+ * ignored (every kind's default: 3D and 2D variant 0, z-cell stage 1). */
 int mia_hip_load_warp_tuning(int device, const char *json_path);
 /* The file the table came from (empty if none) into buf; returns the number
- * of (degree, bucket) entries. */
+ * of (kind, degree, bucket) entries. */
 int mia_hip_warp_tuning_source(char *buf, int len);
 
 /* ---- per-level work of the multiresolution driver ---------------------- */

@@ -54,7 +54,12 @@ SIGNATURES = {
     "mia_hip_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_profile_read": (ctypes.c_int, [_vp, _dbl_p, _c_int_p, ctypes.c_int, _c_int_p]),
     "mia_hip_stage_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "mia_hip_device_arch": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "mia_hip_warp_variants": (ctypes.c_int, [_c_int_p, ctypes.c_int, _c_int_p]),
+    "mia_hip_warp2d_variants": (ctypes.c_int, [_c_int_p, ctypes.c_int, _c_int_p]),
+    "mia_hip_set_stage1_path": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mia_hip_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_char_p, _vp]),
     "mia_hip_set_warp_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_get_warp_variant": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p]),
     "mia_hip_autotune_warp": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -1,11 +1,16 @@
-"""Autotuner of the 3D warp kernel's tile shapes (include/mia_hip.h).
+"""Autotuner of the 3D and 2D warp tiles and the 3D stage-1 layout
+(include/mia_hip.h, mia_hip_autotune).
 
-    python -m mia_amd.autotune [--out PATH] [--degree 0|1|3] [--buckets 15 24]
+    python -m mia_amd.autotune [--out PATH] [--kinds warp3d warp2d stage1_3d]
+                               [--degree 0|1|3] [--buckets3d 15 24]
+                               [--buckets2d 12 18] [--buckets-stage1 15 24]
 
-times every tile variant on synthetic instances of 2^b voxels per size
-bucket b and interpolation degree, writes the fastest per bucket as JSON
-keyed by the GPU architecture (default mia_amd/tuning/warp3d_<arch>.json,
-where the library looks for it) and prints the table.
+times every choice of each kind on synthetic instances of 2^b voxels (3D
+cubes) or pixels (batches of 16 2D squares) per size bucket b and
+interpolation degree, writes the fastest per bucket into one JSON table keyed
+by the GPU architecture (default mia_amd/tuning/<arch>.json, where the library
+looks for it) and prints it.  The architecture comes from the library
+(mia_hip_device_arch).
 """
 import argparse
 import ctypes
@@ -20,8 +25,12 @@ from ._lib import check
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
+KINDS = ("warp3d", "warp2d", "stage1_3d")
+STAGE1_LAYOUTS = ("slice", "zcell")
+
 
 def variants():
+    """3D warp tile shapes (x, y, z) by variant id."""
     L = _lib.load()
     dims = (ctypes.c_int * 48)()
     n = ctypes.c_int()
@@ -29,22 +38,44 @@ def variants():
     return [tuple(dims[3 * v:3 * v + 3]) for v in range(n.value)]
 
 
-def device_arch(device=0):
-    import torch
-    return torch.cuda.get_device_properties(device).gcnArchName.split(":")[0]
-
-
-def run(out_path=None, degree=0, b_min=15, b_max=24, device=0):
-    """Tune and return (table dict, microseconds array [deg][bucket][variant])."""
+def variants_2d():
+    """2D warp tile shapes (x, y) by variant id."""
     L = _lib.load()
-    nv = len(variants())
+    dims = (ctypes.c_int * 32)()
+    n = ctypes.c_int()
+    check(L.mia_hip_warp2d_variants(dims, 16, ctypes.byref(n)))
+    return [tuple(dims[2 * v:2 * v + 2]) for v in range(n.value)]
+
+
+def choices(kind):
+    return {"warp3d": variants, "warp2d": variants_2d, "stage1_3d": lambda: list(STAGE1_LAYOUTS)}[kind]()
+
+
+def device_arch(device=0):
+    """The GPU architecture name the tuning table is keyed by ("gfx950")."""
+    buf = ctypes.create_string_buffer(64)
+    check(_lib.load().mia_hip_device_arch(int(device), buf, 64))
+    return buf.value.decode()
+
+
+def default_path(device=0):
+    return os.path.join(_HERE, "tuning", "%s.json" % device_arch(device))
+
+
+def run(out_path=None, degree=0, b_min=15, b_max=24, device=0, kind="warp3d"):
+    """Tune one kind and return (table dict or None, microseconds array
+    [degree][bucket][choice]).  Entries already in out_path (same
+    architecture) for other kinds, degrees or buckets are kept."""
+    L = _lib.load()
+    k = KINDS.index(kind)
+    nc = len(choices(kind))
     nd = 2 if degree == 0 else 1
     nb = b_max - b_min + 1
-    us = np.zeros(nd * nb * nv)
+    us = np.zeros(nd * nb * nc)
     path = out_path.encode() if out_path else None
-    check(L.mia_hip_autotune_warp(int(device), int(degree), int(b_min), int(b_max), path, us.ctypes.data))
+    check(L.mia_hip_autotune(int(device), k, int(degree), int(b_min), int(b_max), path, us.ctypes.data))
     table = json.load(open(out_path)) if out_path else None
-    return table, us.reshape(nd, nb, nv)
+    return table, us.reshape(nd, nb, nc)
 
 
 def tuning_source():
@@ -56,18 +87,24 @@ def tuning_source():
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=None)
+    ap.add_argument("--kinds", nargs="+", default=list(KINDS), choices=KINDS)
     ap.add_argument("--degree", type=int, default=0, choices=(0, 1, 3))
-    ap.add_argument("--buckets", type=int, nargs=2, default=(15, 24))
+    ap.add_argument("--buckets3d", type=int, nargs=2, default=(15, 24))
+    ap.add_argument("--buckets2d", type=int, nargs=2, default=(12, 18))
+    ap.add_argument("--buckets-stage1", type=int, nargs=2, default=(15, 24))
     ap.add_argument("--device", type=int, default=0)
     a = ap.parse_args(argv)
-    out = a.out or os.path.join(_HERE, "tuning", "warp3d_%s.json" % device_arch(a.device))
+    out = a.out or default_path(a.device)
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
-    table, us = run(out, a.degree, a.buckets[0], a.buckets[1], a.device)
-    v = variants()
-    print("variants (tile x, y, z):", v)
+    buckets = {"warp3d": a.buckets3d, "warp2d": a.buckets2d, "stage1_3d": a.buckets_stage1}
+    table = None
+    for kind in a.kinds:
+        table, _ = run(out, a.degree, buckets[kind][0], buckets[kind][1], a.device, kind)
+        print("tuned", kind, flush=True)
     for row in table["table"]:
-        print("degree %d  bucket 2^%d (%s^3)  best %s  us %s" % (row["degree"], row["bucket"], row["size"][0],
-                                                               v[row["variant"]], row["us"]))
+        kind = row.get("kernel", "warp3d")
+        print("%-9s degree %d  bucket 2^%d %s  best %s  us %s" % (kind, row["degree"], row["bucket"], row.get("size"),
+                                                                  choices(kind)[row["variant"]], row["us"]))
     print("written", out)
     return 0
 

@@ -47,8 +47,14 @@ class HipContext:
 
     # -- set-up ---------------------------------------------------------------
     def set_warp_variant(self, variant):
-        """Force a 3D warp tile variant for the next setup (-1: tuned)."""
+        """Force a warp tile variant (3D or 2D by the setup's dim) for the next
+        setup (-1: tuned)."""
         check(self._L.mia_hip_set_warp_variant(self._h, int(variant)))
+
+    def set_stage1_path(self, path):
+        """Force the 3D reference-gradient stage-1 layout for the next setup:
+        0 per slice, 1 per z knot cell where allowed, -1 tuned."""
+        check(self._L.mia_hip_set_stage1_path(self._h, int(path)))
 
     def warp_variant(self, inst):
         v = ctypes.c_int()

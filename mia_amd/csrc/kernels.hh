@@ -82,9 +82,13 @@ __host__ __device__ inline void w3_tile_dims(int v, int &tx, int &ty, int &tz)
     default: tx = W3T0::TX; ty = W3T0::TY; tz = W3T0::TZ; break;
     }
 }
-// 2D warp tile and LDS budgets
-constexpr int kW2TX = 64, kW2TY = 32;
-constexpr int kW2Win = (kW2TX + 4) * (kW2TY + 4);   // float2, largest coefficient window (rate 1)
+// 2D warp tile variants (x, y pixels per block of kBlock threads; a thread
+// takes one x column and kBlock / x rows apart) and LDS budgets
+constexpr int kW2Variants = 4;
+__host__ __device__ constexpr int w2_tile_x(int v) { return v == 2 ? 32 : v == 3 ? 128 : 64; }
+__host__ __device__ constexpr int w2_tile_y(int v) { return v == 1 || v == 3 ? 16 : 32; }
+// float2, largest coefficient window over the variants (rate 1)
+constexpr int kW2Win = (128 + 4) * (16 + 4);
 constexpr int kW2Region = 6144;                     // floats
 
 // Per-instance description, identical on host and device.
@@ -1230,9 +1234,9 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// 2D warp.  Block = 64 x 32 pixels, thread = 8 pixels of one x column
-// (rows y0 + ly + 4k).  One coefficient window per tile.
-template <int DEG>
+// 2D warp.  Block = TX x TY pixels, thread = TY * TX / kBlock pixels of one
+// x column (rows y0 + ly + (kBlock / TX) k).  One coefficient window per tile.
+template <int DEG, int TX, int TY>
 __global__ void __launch_bounds__(kBlock)
     k_warp2d(const InstDesc *__restrict__ inst, const int *__restrict__ ids, int n_inst,
              const int *__restrict__ prefix, int boff, int nblocks,
@@ -1240,7 +1244,9 @@ __global__ void __launch_bounds__(kBlock)
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
     constexpr int NS = DEG == 3 ? 4 : 2, LO = DEG == 3 ? 1 : 0;
-    constexpr int RPT = kW2TY / (kBlock / kW2TX);   // rows per thread (8)
+    static_assert(TX <= kBlock && kBlock % TX == 0 && TY % (kBlock / TX) == 0, "tile shape");
+    static_assert((TX + 4) * (TY + 4) <= kW2Win && TY * (TX + 4) <= kW2Region / 2, "tile LDS budget");
+    constexpr int RPT = TY / (kBlock / TX);   // rows per thread
     // dynamic LDS: the image box (kW2Region floats), then the coefficient
     // window, sized by the host for the launch's largest window (about 1 KB
     // at rate 8, so that six blocks fit a CU)
@@ -1248,25 +1254,25 @@ __global__ void __launch_bounds__(kBlock)
     float *region = smem;
     float2 *win = (float2 *)(smem + kW2Region);
     __shared__ float red[4][kBlock / 64];
-    __shared__ float4 wy_s[kW2TY];
-    __shared__ int sy_s[kW2TY];
+    __shared__ float4 wy_s[TY];
+    __shared__ int sy_s[TY];
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
     const int gi = find_inst(prefix, n_inst, b);
     const int i = ids[gi];
     const InstDesc d = inst[i];
     const int lb = b - prefix[gi];
-    const int tiles_x = (d.nx + kW2TX - 1) / kW2TX;
-    const int x0 = (lb % tiles_x) * kW2TX, y0 = (lb / tiles_x) * kW2TY;
-    const int x = x0 + (threadIdx.x & (kW2TX - 1));
-    const int ly = threadIdx.x / kW2TX;
+    const int tiles_x = (d.nx + TX - 1) / TX;
+    const int x0 = (lb % tiles_x) * TX, y0 = (lb / tiles_x) * TY;
+    const int x = x0 + (threadIdx.x & (TX - 1));
+    const int ly = threadIdx.x / TX;
     const bool xvalid = x < d.nx;
     const int xc = min(x, d.nx - 1);
     const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
-    const int wxn = ts[d.tab_x + min(x0 + kW2TX, d.nx) - 1] + 4 - kx0;
-    const int wyn = ts[d.tab_y + min(y0 + kW2TY, d.ny) - 1] + 4 - ky0;
+    const int wxn = ts[d.tab_x + min(x0 + TX, d.nx) - 1] + 4 - kx0;
+    const int wyn = ts[d.tab_y + min(y0 + TY, d.ny) - 1] + 4 - ky0;
     const double *P = params + d.par_off;
     // the tile's y tables (rows clamped to the image) once into LDS
-    if (threadIdx.x < kW2TY) {
+    if (threadIdx.x < TY) {
         const int yy = min(y0 + (int)threadIdx.x, d.ny - 1);
         wy_s[threadIdx.x] = tw[d.tab_y + yy];
         sy_s[threadIdx.x] = ts[d.tab_y + yy] - ky0;
@@ -1281,7 +1287,7 @@ __global__ void __launch_bounds__(kBlock)
     float2 *vy = (float2 *)region;
     {
         const float iwx = 1.0f / (float)wxn;
-        for (int e = threadIdx.x; e < kW2TY * wxn; e += kBlock) {
+        for (int e = threadIdx.x; e < TY * wxn; e += kBlock) {
             const int yl = small_div(e, iwx), kx = e - yl * wxn;
             const float4 wyv = wy_s[yl];
             const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
@@ -1304,7 +1310,7 @@ __global__ void __launch_bounds__(kBlock)
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-        const int yl = ly + k * (kBlock / kW2TX);
+        const int yl = ly + k * (kBlock / TX);
         const int y = y0 + yl;
         const float2 *wr = vy + yl * wxn + sx;
         float ux = 0.0f, uy = 0.0f;
@@ -1357,7 +1363,7 @@ __global__ void __launch_bounds__(kBlock)
     float *Wi = W + d.img_off;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-        const int y = y0 + ly + k * (kBlock / kW2TX);
+        const int y = y0 + ly + k * (kBlock / TX);
         if (y < d.ny) {
             float v;
             if (use_lds) {
@@ -1735,8 +1741,11 @@ __global__ void __launch_bounds__(kBlock, 4)
     const __amdgpu_buffer_rsrc_t rwi = raw_rsrc(W + d.img_off), rri = raw_rsrc(R + d.img_off);
     const unsigned vo = (unsigned)min(x, nx - 1) * 4u;
     // the column beyond the wave for its edge lanes (x - 1 for lane 0, x + 1
-    // for lane 63, clamped; the others read their own column)
-    const unsigned veo = lane == 0 ? (unsigned)max(x - 1, 0) * 4u
+    // for lane 63; the others read their own column), clamped to the row on
+    // both sides: lane 0 of a wave that starts beyond the row (nx <= 64 w)
+    // would otherwise read past the row, and on the image's last row past
+    // the image
+    const unsigned veo = lane == 0 ? (unsigned)min(max(x - 1, 0), nx - 1) * 4u
                                    : lane == 63 ? (unsigned)min(x + 1, nx - 1) * 4u : vo;
     // byte offset of window row r within a slice (rows clamped; the same for
     // every slice of the block, so these stay in scalar registers) and of

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mia_hip.h"
@@ -143,10 +144,11 @@ struct mia_hip_ctx {
     std::vector<WarpGroup> wgroups;
     std::vector<int> inst_variant, inst_group, inst_slot;
     int force_variant = -1; // -1: tuned per size bucket
+    int force_s1 = -1;      // stage-1 layout of 3D reference gradients: -1 tuned, 0 per slice, 1 z cell
     std::vector<char> images_set;
     long long n_img = 0, n_par = 0, n_part = 0, max_vox = 0;
     int max_nx = 0, max_cx = 0, max_ncx = 0, n_zcell = 0;
-    int w2_win = 0; // 2D warp: largest coefficient window of a 64 x 32 tile (float2)
+    int w2_win = 0; // 2D warp: largest coefficient window of a tile (float2)
 
     InstDesc *d_inst = nullptr;
     int *d_wgroups = nullptr, *d_s1_prefix = nullptr, *d_red_prefix = nullptr, *d_s1z_prefix = nullptr,
@@ -232,20 +234,29 @@ void free_buffers(mia_hip_ctx *c)
     c->graph = nullptr;
 }
 
-// Warp tile variants chosen per size bucket (ceil(log2(voxels))) and
-// interpolation degree, from a JSON table keyed by the GPU architecture,
-// written by mia_hip_autotune_warp.  Source: $MIA_HIP_WARP_TUNING, else
-// <library dir>/../tuning/warp3d_<arch>.json; without a table, variant 0.
-struct WarpTuning {
+// Tuned choices per (kind, interpolation degree, size bucket = ceil(log2(voxels
+// or pixels))): the 3D warp tile, the 2D warp tile and the stage-1 layout of
+// 3D reference gradients (0 per slice, 1 per z knot cell).  One JSON table
+// keyed by the GPU architecture, written by mia_hip_autotune.  Source:
+// $MIA_HIP_TUNING, else <library dir>/../tuning/<arch>.json; without an
+// entry for a kind and degree, the kind's default.
+constexpr int kTuneKinds = 3; // MIA_HIP_TUNE_WARP3D, _WARP2D, _STAGE1
+const char *const kTuneKindNames[kTuneKinds] = {"warp3d", "warp2d", "stage1_3d"};
+constexpr int kTuneChoices[kTuneKinds] = {kW3Variants, kW2Variants, 2};
+constexpr int kTuneDefault[kTuneKinds] = {0, 0, 1};
+constexpr int kMaxTuneChoices = 4;
+static_assert(kW3Variants <= kMaxTuneChoices && kW2Variants <= kMaxTuneChoices, "variant tables");
+
+struct Tuning {
     std::mutex m;
     bool loaded = false;
     std::string arch, source;
-    std::map<std::pair<int, int>, int> table; // (degree, bucket) -> variant
+    std::map<std::tuple<int, int, int>, int> table; // (kind, degree, bucket) -> choice
 };
 
-WarpTuning &warp_tuning()
+Tuning &tuning()
 {
-    static WarpTuning t;
+    static Tuning t;
     return t;
 }
 
@@ -260,51 +271,101 @@ std::string device_arch(int device)
 
 std::string default_tuning_path(const std::string &arch)
 {
-    if (const char *e = std::getenv("MIA_HIP_WARP_TUNING"))
+    if (const char *e = std::getenv("MIA_HIP_TUNING"))
         return e;
     Dl_info info;
     if (dladdr((const void *)&default_tuning_path, &info) && info.dli_fname) {
         std::string lib(info.dli_fname);
         const size_t k = lib.rfind('/');
         const std::string dir = k == std::string::npos ? "." : lib.substr(0, k);
-        return dir + "/../tuning/warp3d_" + arch + ".json";
+        return dir + "/../tuning/" + arch + ".json";
     }
     return "";
 }
 
-// parse the table written by write_tuning (entries
-// {"degree": D, "bucket": B, "variant": V, ...}); false if unreadable
-bool parse_tuning(const std::string &path, std::string &arch, std::map<std::pair<int, int>, int> &table)
+bool read_text(const std::string &path, std::string &text)
 {
     std::FILE *f = std::fopen(path.c_str(), "r");
     if (!f)
         return false;
-    std::string text;
+    text.clear();
     char buf[4096];
     size_t n;
     while ((n = std::fread(buf, 1, sizeof buf, f)) > 0)
         text.append(buf, n);
     std::fclose(f);
-    const size_t a = text.find("\"arch\"");
-    if (a == std::string::npos)
+    return true;
+}
+
+// the string value of "key" in text[b, e), or "" without one
+std::string json_string(const std::string &text, size_t b, size_t e, const char *key)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    const size_t a = text.find(k, b);
+    if (a == std::string::npos || a >= e)
+        return "";
+    const size_t q1 = text.find('"', text.find(':', a + k.size()) + 1);
+    const size_t q2 = q1 == std::string::npos ? q1 : text.find('"', q1 + 1);
+    if (q2 == std::string::npos || q2 >= e)
+        return "";
+    return text.substr(q1 + 1, q2 - q1 - 1);
+}
+
+// the integer value of "key" in text[b, e); false without one
+bool json_int(const std::string &text, size_t b, size_t e, const char *key, int &v)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    const size_t a = text.find(k, b);
+    if (a == std::string::npos || a >= e)
         return false;
-    const size_t q1 = text.find('"', text.find(':', a) + 1), q2 = text.find('"', q1 + 1);
-    if (q1 == std::string::npos || q2 == std::string::npos)
+    const size_t c = text.find(':', a + k.size());
+    if (c == std::string::npos || c >= e)
         return false;
-    arch = text.substr(q1 + 1, q2 - q1 - 1);
+    char *end = nullptr;
+    const long x = std::strtol(text.c_str() + c + 1, &end, 10);
+    if (end == text.c_str() + c + 1)
+        return false;
+    v = (int)x;
+    return true;
+}
+
+int tune_kind(const std::string &name)
+{
+    if (name.empty())
+        return 0; // entries without "kernel" are 3D warp tiles
+    for (int k = 0; k < kTuneKinds; ++k)
+        if (name == kTuneKindNames[k])
+            return k;
+    return -1;
+}
+
+// the entries of a table written by mia_hip_autotune: objects with "bucket",
+// "degree", "variant" and (optionally) "kernel"; false if unreadable
+bool parse_tuning(const std::string &path, std::string &arch, std::map<std::tuple<int, int, int>, int> &table)
+{
+    std::string text;
+    if (!read_text(path, text))
+        return false;
+    arch = json_string(text, 0, text.size(), "arch");
+    if (arch.empty())
+        return false;
     table.clear();
     size_t pos = 0;
-    while ((pos = text.find("{\"degree\":", pos)) != std::string::npos) {
-        int d = 0, b = 0, v = 0;
-        if (std::sscanf(text.c_str() + pos, "{\"degree\": %d, \"bucket\": %d, \"variant\": %d", &d, &b, &v) == 3 &&
-            v >= 0 && v < kW3Variants)
-            table[{d, b}] = v;
+    while ((pos = text.find("\"bucket\"", pos)) != std::string::npos) {
+        const size_t b = text.rfind('{', pos), e = text.find('}', pos);
         ++pos;
+        if (b == std::string::npos || e == std::string::npos)
+            continue;
+        const int kind = tune_kind(json_string(text, b, e, "kernel"));
+        int d = 0, bk = 0, v = 0;
+        if (kind >= 0 && json_int(text, b, e, "degree", d) && json_int(text, b, e, "bucket", bk) &&
+            json_int(text, b, e, "variant", v) && v >= 0 && v < kTuneChoices[kind])
+            table[std::make_tuple(kind, d, bk)] = v;
     }
     return true;
 }
 
-void load_tuning_locked(WarpTuning &t, int device, const char *path)
+void load_tuning_locked(Tuning &t, int device, const char *path)
 {
     t.loaded = true;
     t.table.clear();
@@ -312,7 +373,7 @@ void load_tuning_locked(WarpTuning &t, int device, const char *path)
     const std::string arch = device_arch(device);
     const std::string p = path ? std::string(path) : default_tuning_path(arch);
     std::string file_arch;
-    std::map<std::pair<int, int>, int> table;
+    std::map<std::tuple<int, int, int>, int> table;
     if (!p.empty() && parse_tuning(p, file_arch, table) && file_arch == arch) {
         t.table = table;
         t.source = p;
@@ -320,22 +381,49 @@ void load_tuning_locked(WarpTuning &t, int device, const char *path)
     t.arch = arch;
 }
 
-int tuned_variant(int device, int degree, long long nvox)
+int size_bucket(long long n)
 {
-    WarpTuning &t = warp_tuning();
+    int bucket = 0;
+    while ((1LL << bucket) < n)
+        ++bucket;
+    return bucket;
+}
+
+// the tuned choice of `kind` for an instance of n voxels (pixels): the entry
+// of the nearest bucket of that kind and degree, else the kind's default
+int tuned_choice(int device, int kind, int degree, long long n)
+{
+    Tuning &t = tuning();
     std::lock_guard<std::mutex> lock(t.m);
     if (!t.loaded)
         load_tuning_locked(t, device, nullptr);
-    int bucket = 0;
-    while ((1LL << bucket) < nvox)
-        ++bucket;
-    int best = 0, dist = 1 << 30;
+    const int bucket = size_bucket(n);
+    int best = kTuneDefault[kind], dist = 1 << 30;
     for (const auto &kv : t.table)
-        if (kv.first.first == degree && std::abs(kv.first.second - bucket) < dist) {
-            dist = std::abs(kv.first.second - bucket);
+        if (std::get<0>(kv.first) == kind && std::get<1>(kv.first) == degree &&
+            std::abs(std::get<2>(kv.first) - bucket) < dist) {
+            dist = std::abs(std::get<2>(kv.first) - bucket);
             best = kv.second;
         }
     return best;
+}
+
+// whether an instance may run the z-cell stage 1 (one x column per thread,
+// the band's rows in registers; its reduce indexes the gradient with 32-bit
+// arithmetic)
+bool zcell_eligible(int dim, int nx, int band_rows, long long ncoef)
+{
+    return dim == 3 && nx <= kBlock && band_rows + 2 <= kS1zRows && 3LL * ncoef < (1LL << 31);
+}
+
+// the coefficient window of a 2D warp tile: knots spanned by the tiles of
+// one axis (t pixels each), from the axis' start indices
+template <typename StartOf> int tile_span(int n, int t, StartOf start)
+{
+    int m = 0;
+    for (int p0 = 0; p0 < n; p0 += t)
+        m = std::max(m, start(std::min(p0 + t, n) - 1) + 4 - start(p0));
+    return m;
 }
 
 hipEvent_t get_event(mia_hip_ctx *c)
@@ -405,7 +493,7 @@ struct EvalPlan {
     int max_nx = 0, max_cx = 0, max_ncx = 0, w2_win = 0;
     int s1_blocks = 0, red_blocks = 0, s1z_blocks = 0, redz_blocks = 0; // prefix totals
     int n_warp = 0;
-    WarpLaunch warp[kW3Variants];
+    WarpLaunch warp[kMaxTuneChoices];
     const InstDesc *inst = nullptr;
     const int *s1_prefix = nullptr, *red_prefix = nullptr, *s1z_prefix = nullptr, *redz_prefix = nullptr;
     const float *M = nullptr, *R = nullptr;
@@ -465,6 +553,17 @@ EvalPlan plan_of(const mia_hip_ctx *c)
     return p;
 }
 
+// the 2D warp kernel of a degree and tile variant
+decltype(&k_warp2d<3, 64, 32>) warp2d_kernel(int degree, int v)
+{
+    switch (v) {
+    case 1: return degree == 3 ? k_warp2d<3, w2_tile_x(1), w2_tile_y(1)> : k_warp2d<1, w2_tile_x(1), w2_tile_y(1)>;
+    case 2: return degree == 3 ? k_warp2d<3, w2_tile_x(2), w2_tile_y(2)> : k_warp2d<1, w2_tile_x(2), w2_tile_y(2)>;
+    case 3: return degree == 3 ? k_warp2d<3, w2_tile_x(3), w2_tile_y(3)> : k_warp2d<1, w2_tile_x(3), w2_tile_y(3)>;
+    default: return degree == 3 ? k_warp2d<3, w2_tile_x(0), w2_tile_y(0)> : k_warp2d<1, w2_tile_x(0), w2_tile_y(0)>;
+    }
+}
+
 // Warp the instances of warp launch g, blocks [boff, boff + nb) of its tile
 // prefix.
 int launch_warp(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const double *d_params, int g, int boff,
@@ -480,7 +579,7 @@ int launch_warp(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const doub
         launch(s, prof, ST_WARP, kern, dim3(nb), 0, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw, pl.ts,
                pl.M, pl.clamp, pl.W);
     } else {
-        auto kern = pl.degree == 3 ? k_warp2d<3> : k_warp2d<1>;
+        auto kern = warp2d_kernel(pl.degree, G.variant);
         const size_t lds = (size_t)kW2Region * sizeof(float) + (size_t)pl.w2_win * sizeof(float2);
         launch(s, prof, ST_WARP, kern, dim3(nb), lds, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw,
                pl.ts, pl.M, pl.clamp, pl.W);
@@ -783,6 +882,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->images_set.assign(n_inst, 0);
     c->penalty.assign(n_inst, nullptr);
     c->penalty_weight.assign(n_inst, 0.0);
+    c->inst_variant.assign(n_inst, 0);
     std::vector<float> tw;
     std::vector<int> ts, cells;
     long long img = 0, par = 0, part = 0, maxv = 0;
@@ -825,12 +925,14 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         int band_rows = 0;
         for (int k = 0; k < d.ncy; ++k)
             band_rows = std::max(band_rows, ay.cells[k + 1] - ay.cells[k]);
-        // reference-gradient stage 1 per (z knot cell, y band): one x column
-        // per thread and the band's rows in registers (decided per instance,
-        // so that an instance computes the same in any batch)
-        // (its reduce indexes the gradient with 32-bit arithmetic)
-        d.zcell = dim == 3 && d.nx <= kBlock && band_rows + 2 <= kS1zRows &&
-                          3LL * d.cx * d.cy * d.cz < (1LL << 31)
+        // reference-gradient stage 1 per (z knot cell, y band) where the
+        // instance allows it and the tuned layout (or the forced one) asks
+        // for it: decided per instance from its sizes, so that an instance
+        // computes the same in any batch
+        const long long nvox = (long long)d.nx * d.ny * d.nz;
+        const long long ncoef = (long long)d.cx * d.cy * d.cz;
+        d.zcell = zcell_eligible(dim, d.nx, band_rows, ncoef) &&
+                          (c->force_s1 >= 0 ? c->force_s1 : tuned_choice(c->device, MIA_HIP_TUNE_STAGE1, degree, nvox)) == 1
                       ? 1
                       : 0;
         n_zcell += d.zcell;
@@ -844,8 +946,6 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
             *coff[a] = (int)cells.size();
             cells.insert(cells.end(), axes[a]->cells.begin(), axes[a]->cells.end());
         }
-        const long long nvox = (long long)d.nx * d.ny * d.nz;
-        const long long ncoef = (long long)d.cx * d.cy * d.cz;
         d.img_off = img;
         img += align64(nvox);
         d.par_off = par;
@@ -858,14 +958,12 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         maxv = std::max(maxv, nvox);
         max_nx = std::max(max_nx, d.nx);
         if (dim == 2) {
-            // knots spanned by the tiles of k_warp2d (the window it loads)
-            auto span = [](const AxisTable &a, int n, int t) {
-                int m = 0;
-                for (int p0 = 0; p0 < n; p0 += t)
-                    m = std::max(m, a.start[std::min(p0 + t, n) - 1] + 4 - a.start[p0]);
-                return m;
-            };
-            w2_win = std::max(w2_win, span(ax, d.nx, kW2TX) * span(ay, d.ny, kW2TY));
+            // the 2D warp tile and the coefficient window it loads
+            const int v = c->force_variant >= 0 ? c->force_variant : tuned_choice(c->device, MIA_HIP_TUNE_WARP2D, degree, nvox);
+            const int wx = tile_span(d.nx, w2_tile_x(v), [&](int p) { return ax.start[p]; });
+            const int wy = tile_span(d.ny, w2_tile_y(v), [&](int p) { return ay.start[p]; });
+            w2_win = std::max(w2_win, wx * wy);
+            c->inst_variant[i] = v;
         }
         max_cx = std::max(max_cx, d.cx);
         max_ncx = std::max(max_ncx, d.ncx);
@@ -877,22 +975,20 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         c->s1z_prefix[i + 1] = c->s1z_prefix[i] + d.ncz * d.ncy;
         c->redz_prefix[i + 1] = c->redz_prefix[i] + (int)((dim * ncoef + kBlock - 1) / kBlock);
     }
-    // warp groups: 3D instances by tile variant (tuned per size bucket
-    // unless forced), 2D in one group
-    c->inst_variant.assign(n_inst, 0);
+    // warp groups: instances by tile variant (tuned per size bucket unless
+    // forced)
     c->inst_group.assign(n_inst, 0);
     c->inst_slot.assign(n_inst, 0);
     c->wgroups.clear();
     {
-        int gidx[kW3Variants];
-        for (int v = 0; v < kW3Variants; ++v)
+        int gidx[kMaxTuneChoices];
+        for (int v = 0; v < kMaxTuneChoices; ++v)
             gidx[v] = -1;
         for (int i = 0; i < n_inst; ++i) {
             const InstDesc &d = c->inst[i];
             const long long nvox = (long long)d.nx * d.ny * d.nz;
-            const int v = dim == 3 ? (c->force_variant >= 0 ? c->force_variant
-                                                           : tuned_variant(c->device, degree, nvox))
-                                   : 0;
+            const int v = dim == 3 ? (c->force_variant >= 0 ? c->force_variant : tuned_choice(c->device, MIA_HIP_TUNE_WARP3D, degree, nvox))
+                                   : c->inst_variant[i];
             c->inst_variant[i] = v;
             if (gidx[v] < 0) {
                 gidx[v] = (int)c->wgroups.size();
@@ -907,7 +1003,8 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                 w3_tile_dims(v, tx, ty, tz);
                 tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty) * ((d.nz + tz - 1) / tz);
             } else {
-                tiles = ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
+                const int tx = w2_tile_x(v), ty = w2_tile_y(v);
+                tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty);
             }
             c->inst_group[i] = gidx[v];
             c->inst_slot[i] = (int)G.ids.size();
@@ -1073,7 +1170,8 @@ int mia_hip_get_stage1_path(mia_hip_ctx *c, int inst, int *path)
         return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
     if (inst < 0 || inst >= c->n_inst)
         return fail(MIA_HIP_ERR_ARG, "instance index out of range");
-    *path = c->inst[inst].zcell;
+    // the exact gradient always runs per slice
+    *path = c->inst[inst].zcell && c->grad_mode != MIA_HIP_GRAD_EXACT ? 1 : 0;
     return MIA_HIP_OK;
 }
 
@@ -1429,8 +1527,20 @@ const char *mia_hip_stage_name(int stage)
 } // extern "C"
 
 // ---------------------------------------------------------------------------
-// warp tile variants and their autotuner
+// tile variants, stage-1 layouts and their autotuner
 // ---------------------------------------------------------------------------
+
+int mia_hip_device_arch(int device, char *buf, int len)
+{
+    if (!buf || len <= 0)
+        return fail(MIA_HIP_ERR_ARG, "null buffer");
+    const std::string a = device_arch(device);
+    if (a.empty())
+        return fail(MIA_HIP_ERR_DEVICE, "cannot query device " + std::to_string(device));
+    std::strncpy(buf, a.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+    return MIA_HIP_OK;
+}
 
 int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants)
 {
@@ -1441,12 +1551,24 @@ int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants)
     return MIA_HIP_OK;
 }
 
+int mia_hip_warp2d_variants(int *tile_dims, int max_variants, int *n_variants)
+{
+    if (n_variants)
+        *n_variants = kW2Variants;
+    for (int v = 0; v < kW2Variants && v < max_variants && tile_dims; ++v) {
+        tile_dims[2 * v] = w2_tile_x(v);
+        tile_dims[2 * v + 1] = w2_tile_y(v);
+    }
+    return MIA_HIP_OK;
+}
+
 int mia_hip_set_warp_variant(mia_hip_ctx *ctx, int variant)
 {
     if (!ctx)
         return fail(MIA_HIP_ERR_ARG, "null context");
-    if (variant < -1 || variant >= kW3Variants)
-        return fail(MIA_HIP_ERR_ARG, "warp variant must be -1 (tuned) or 0.." + std::to_string(kW3Variants - 1));
+    const int nv = std::min(kW3Variants, kW2Variants);
+    if (variant < -1 || variant >= nv)
+        return fail(MIA_HIP_ERR_ARG, "warp variant must be -1 (tuned) or 0.." + std::to_string(nv - 1));
     ctx->force_variant = variant;
     return MIA_HIP_OK;
 }
@@ -1463,19 +1585,29 @@ int mia_hip_get_warp_variant(mia_hip_ctx *ctx, int inst, int *variant)
     return MIA_HIP_OK;
 }
 
+int mia_hip_set_stage1_path(mia_hip_ctx *ctx, int path)
+{
+    if (!ctx)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (path < -1 || path > 1)
+        return fail(MIA_HIP_ERR_ARG, "stage-1 layout must be -1 (tuned), 0 (per slice) or 1 (per z knot cell)");
+    ctx->force_s1 = path;
+    return MIA_HIP_OK;
+}
+
 int mia_hip_load_warp_tuning(int device, const char *json_path)
 {
-    WarpTuning &t = warp_tuning();
+    Tuning &t = tuning();
     std::lock_guard<std::mutex> lock(t.m);
     load_tuning_locked(t, device, json_path);
     if (json_path && t.source.empty())
-        return fail(MIA_HIP_ERR_ARG, std::string("no warp tuning for this GPU in ") + json_path);
+        return fail(MIA_HIP_ERR_ARG, std::string("no tuning table for this GPU in ") + json_path);
     return MIA_HIP_OK;
 }
 
 int mia_hip_warp_tuning_source(char *buf, int len)
 {
-    WarpTuning &t = warp_tuning();
+    Tuning &t = tuning();
     std::lock_guard<std::mutex> lock(t.m);
     if (buf && len > 0) {
         std::strncpy(buf, t.source.c_str(), (size_t)len - 1);
@@ -1501,21 +1633,92 @@ std::vector<double> tuning_params(size_t n, double sd, double clip, unsigned lon
     }
     return p;
 }
+
+// the synthetic workload of one tuning measurement
+struct TuneCase {
+    int dim, n_inst, n; // n: side of the cube / square
+    float rate;
+};
+
+TuneCase tune_case(int kind, int bucket)
+{
+    if (kind == MIA_HIP_TUNE_WARP2D)
+        return {2, 16, (int)std::llround(std::pow(2.0, bucket / 2.0)), 8.0f};
+    return {3, 1, (int)std::llround(std::pow(2.0, bucket / 3.0)), kind == MIA_HIP_TUNE_WARP3D ? 6.0f : 8.0f};
+}
+
+// mean microseconds per evaluation of the kind's stages with choice v forced
+int time_choice(int device, int kind, int degree, const TuneCase &tc, int bucket, int v, double &us)
+{
+    uint32_t sz[3], cs[3];
+    std::vector<uint32_t> sizes, csizes;
+    const int c1 = mia_hip_spline_csize((uint32_t)tc.n, tc.rate, 3);
+    for (int a = 0; a < 3; ++a) {
+        sz[a] = (uint32_t)tc.n;
+        cs[a] = (uint32_t)c1;
+    }
+    for (int k = 0; k < tc.n_inst; ++k) {
+        sizes.insert(sizes.end(), sz, sz + 3);
+        csizes.insert(csizes.end(), cs, cs + 3);
+    }
+    const size_t np = (size_t)tc.n_inst * tc.dim * c1 * c1 * (tc.dim == 3 ? c1 : 1);
+    const std::vector<double> p = tuning_params(np, 1.5, 4.0, 1000 + bucket);
+    std::vector<float> img((size_t)tc.n * tc.n * (tc.dim == 3 ? tc.n : 1), 0.0f);
+    std::vector<double> g(np), cost(tc.n_inst);
+    mia_hip_ctx *ctx = nullptr;
+    int st = mia_hip_ctx_create(device, &ctx);
+    if (st)
+        return st;
+    if (kind == MIA_HIP_TUNE_STAGE1)
+        st = mia_hip_set_stage1_path(ctx, v);
+    else
+        st = mia_hip_set_warp_variant(ctx, v);
+    if (!st)
+        st = mia_hip_setup(ctx, tc.dim, tc.n_inst, sizes.data(), csizes.data(), degree);
+    for (int k = 0; k < tc.n_inst && !st; ++k)
+        st = mia_hip_set_images(ctx, k, img.data(), img.data());
+    for (int k = 0; k < 2 && !st; ++k)
+        st = mia_hip_cost_grad(ctx, p.data(), g.data(), cost.data());
+    if (!st)
+        st = mia_hip_profile_enable(ctx, 1);
+    const int reps = 5;
+    for (int k = 0; k < reps && !st; ++k)
+        st = mia_hip_cost_grad(ctx, p.data(), g.data(), cost.data());
+    double ms[ST_COUNT];
+    int nl[ST_COUNT], ns = 0;
+    if (!st)
+        st = mia_hip_profile_read(ctx, ms, nl, ST_COUNT, &ns);
+    mia_hip_ctx_destroy(ctx);
+    if (st)
+        return st;
+    if (kind == MIA_HIP_TUNE_STAGE1)
+        us = 1e3 * (ms[ST_STAGE1] + ms[ST_REDUCE]) / reps;
+    else
+        us = nl[ST_WARP] ? 1e3 * ms[ST_WARP] / nl[ST_WARP] : 1e300;
+    return MIA_HIP_OK;
+}
 } // namespace
 
-int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const char *json_path, double *us_out)
+int mia_hip_autotune(int device, int kind, int degree, int b_min, int b_max, const char *json_path, double *us_out)
 {
+    if (kind < 0 || kind >= kTuneKinds)
+        return fail(MIA_HIP_ERR_ARG, "autotune: unknown kind " + std::to_string(kind));
     if (degree != 0 && degree != 1 && degree != 3)
         return fail(MIA_HIP_ERR_ARG, "autotune: degree must be 1, 3 or 0 (both)");
-    if (b_min < 12 || b_max > 26 || b_min > b_max)
-        return fail(MIA_HIP_ERR_ARG, "autotune: size buckets must satisfy 12 <= b_min <= b_max <= 26");
+    const int lo[kTuneKinds] = {12, 10, 12}, hi[kTuneKinds] = {26, 24, 24};
+    if (b_min < lo[kind] || b_max > hi[kind] || b_min > b_max)
+        return fail(MIA_HIP_ERR_ARG, std::string("autotune: size buckets of ") + kTuneKindNames[kind] +
+                                         " must satisfy " + std::to_string(lo[kind]) + " <= b_min <= b_max <= " +
+                                         std::to_string(hi[kind]));
     const std::string arch = device_arch(device);
     if (arch.empty())
         return fail(MIA_HIP_ERR_DEVICE, "autotune: cannot query device " + std::to_string(device));
+    const int nc = kTuneChoices[kind];
     const int degs[2] = {3, 1};
     struct Row {
-        int degree, bucket, n, variant;
-        double us[kW3Variants];
+        int degree, bucket, variant;
+        TuneCase tc;
+        double us[kMaxTuneChoices];
     };
     std::vector<Row> rows;
     int k_out = 0;
@@ -1524,72 +1727,80 @@ int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const ch
         if (degree != 0 && degree != deg)
             continue;
         for (int b = b_min; b <= b_max; ++b) {
-            Row r{deg, b, (int)std::llround(std::pow(2.0, b / 3.0)), 0, {0}};
-            const uint32_t sz[3] = {(uint32_t)r.n, (uint32_t)r.n, (uint32_t)r.n};
-            const int c1 = mia_hip_spline_csize(sz[0], 6.0f, 3);
-            const uint32_t cs[3] = {(uint32_t)c1, (uint32_t)c1, (uint32_t)c1};
-            const std::vector<double> p = tuning_params(3 * (size_t)c1 * c1 * c1, 1.5, 4.0, 1000 + b);
-            std::vector<float> img((size_t)r.n * r.n * r.n, 0.0f);
-            std::vector<double> g(p.size());
+            Row r{deg, b, 0, tune_case(kind, b), {0}};
             double best = 1e300;
-            for (int v = 0; v < kW3Variants; ++v) {
-                mia_hip_ctx *ctx = nullptr;
-                int st = mia_hip_ctx_create(device, &ctx);
+            for (int v = 0; v < nc; ++v) {
+                const int st = time_choice(device, kind, deg, r.tc, b, v, r.us[v]);
                 if (st)
                     return st;
-                double cost = 0.0, ms[ST_COUNT];
-                int nl[ST_COUNT], ns = 0;
-                if ((st = mia_hip_set_warp_variant(ctx, v)) || (st = mia_hip_setup(ctx, 3, 1, sz, cs, deg)) ||
-                    (st = mia_hip_set_images(ctx, 0, img.data(), img.data())) ||
-                    (st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost)) ||
-                    (st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost)) ||
-                    (st = mia_hip_profile_enable(ctx, 1))) {
-                    mia_hip_ctx_destroy(ctx);
-                    return st;
-                }
-                for (int k = 0; k < 5 && !st; ++k)
-                    st = mia_hip_cost_grad(ctx, p.data(), g.data(), &cost);
-                if (!st)
-                    st = mia_hip_profile_read(ctx, ms, nl, ST_COUNT, &ns);
-                mia_hip_ctx_destroy(ctx);
-                if (st)
-                    return st;
-                r.us[v] = nl[ST_WARP] ? 1e3 * ms[ST_WARP] / nl[ST_WARP] : 1e300;
                 if (r.us[v] < best) {
                     best = r.us[v];
                     r.variant = v;
                 }
                 if (us_out)
-                    us_out[k_out * kW3Variants + v] = r.us[v];
+                    us_out[k_out * nc + v] = r.us[v];
             }
             ++k_out;
             rows.push_back(r);
         }
     }
-    if (json_path) {
-        std::FILE *f = std::fopen(json_path, "w");
-        if (!f)
-            return fail(MIA_HIP_ERR_ARG, std::string("autotune: cannot write ") + json_path);
-        std::fprintf(f, "{\"arch\": \"%s\", \"kernel\": \"warp3d\", \"rate\": 6, \"variants\": [", arch.c_str());
-        for (int v = 0; v < kW3Variants; ++v) {
-            int tx, ty, tz;
-            w3_tile_dims(v, tx, ty, tz);
-            std::fprintf(f, "%s[%d, %d, %d]", v ? ", " : "", tx, ty, tz);
+    if (!json_path)
+        return MIA_HIP_OK;
+    // the entries an existing table of this architecture holds for other
+    // kinds, degrees or buckets
+    std::vector<std::string> keep;
+    {
+        std::string text, file_arch;
+        std::map<std::tuple<int, int, int>, int> old;
+        if (parse_tuning(json_path, file_arch, old) && file_arch == arch && read_text(json_path, text)) {
+            size_t pos = 0;
+            while ((pos = text.find("\"bucket\"", pos)) != std::string::npos) {
+                const size_t b = text.rfind('{', pos), e = text.find('}', pos);
+                ++pos;
+                if (b == std::string::npos || e == std::string::npos)
+                    continue;
+                int d = 0, bk = 0;
+                const bool retuned = tune_kind(json_string(text, b, e, "kernel")) == kind &&
+                                     json_int(text, b, e, "degree", d) && (degree == 0 || d == degree) &&
+                                     json_int(text, b, e, "bucket", bk) && bk >= b_min && bk <= b_max;
+                if (!retuned)
+                    keep.push_back(text.substr(b, e - b + 1));
+            }
         }
-        std::fprintf(f, "],\n \"table\": [\n");
-        for (size_t k = 0; k < rows.size(); ++k) {
-            const Row &r = rows[k];
-            std::fprintf(f, "  {\"degree\": %d, \"bucket\": %d, \"variant\": %d, \"size\": [%d, %d, %d], \"us\": [",
-                         r.degree, r.bucket, r.variant, r.n, r.n, r.n);
-            for (int v = 0; v < kW3Variants; ++v)
-                std::fprintf(f, "%s%.2f", v ? ", " : "", r.us[v]);
-            std::fprintf(f, "]}%s\n", k + 1 < rows.size() ? "," : "");
-        }
-        std::fprintf(f, " ]}\n");
-        std::fclose(f);
-        return mia_hip_load_warp_tuning(device, json_path);
     }
-    return MIA_HIP_OK;
+    std::FILE *f = std::fopen(json_path, "w");
+    if (!f)
+        return fail(MIA_HIP_ERR_ARG, std::string("autotune: cannot write ") + json_path);
+    std::fprintf(f, "{\"arch\": \"%s\",\n \"variants\": {\"warp3d\": [", arch.c_str());
+    for (int v = 0; v < kW3Variants; ++v) {
+        int tx, ty, tz;
+        w3_tile_dims(v, tx, ty, tz);
+        std::fprintf(f, "%s[%d, %d, %d]", v ? ", " : "", tx, ty, tz);
+    }
+    std::fprintf(f, "], \"warp2d\": [");
+    for (int v = 0; v < kW2Variants; ++v)
+        std::fprintf(f, "%s[%d, %d]", v ? ", " : "", w2_tile_x(v), w2_tile_y(v));
+    std::fprintf(f, "], \"stage1_3d\": [\"slice\", \"zcell\"]},\n \"table\": [\n");
+    size_t left = keep.size() + rows.size();
+    for (const std::string &e : keep)
+        std::fprintf(f, "  %s%s\n", e.c_str(), --left ? "," : "");
+    for (const Row &r : rows) {
+        std::fprintf(f, "  {\"kernel\": \"%s\", \"degree\": %d, \"bucket\": %d, \"variant\": %d, \"instances\": %d, "
+                        "\"size\": [%d, %d%s], \"rate\": %g, \"us\": [",
+                     kTuneKindNames[kind], r.degree, r.bucket, r.variant, r.tc.n_inst, r.tc.n, r.tc.n,
+                     r.tc.dim == 3 ? (", " + std::to_string(r.tc.n)).c_str() : "", (double)r.tc.rate);
+        for (int v = 0; v < nc; ++v)
+            std::fprintf(f, "%s%.2f", v ? ", " : "", r.us[v]);
+        std::fprintf(f, "]}%s\n", --left ? "," : "");
+    }
+    std::fprintf(f, " ]}\n");
+    std::fclose(f);
+    return mia_hip_load_warp_tuning(device, json_path);
+}
+
+int mia_hip_autotune_warp(int device, int degree, int b_min, int b_max, const char *json_path, double *us_out)
+{
+    return mia_hip_autotune(device, MIA_HIP_TUNE_WARP3D, degree, b_min, b_max, json_path, us_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1688,7 +1899,8 @@ int mia_hip_cost_grad_noalloc(int device, int dim, const uint32_t *size, const u
     d.ncx = ax.nbands;
     d.ncy = ay.nbands;
     d.ncz = az.nbands;
-    d.zcell = dim == 3 && d.nx <= kBlock && ay.band_rows + 2 <= kS1zRows && 3LL * d.cx * d.cy * d.cz < (1LL << 31)
+    d.zcell = zcell_eligible(dim, d.nx, ay.band_rows, (long long)d.cx * d.cy * d.cz) &&
+                      tuned_choice(device, MIA_HIP_TUNE_STAGE1, degree, nvox) == 1
                   ? 1
                   : 0;
     d.img_off = d.par_off = d.part_off = 0;
@@ -1706,24 +1918,20 @@ int mia_hip_cost_grad_noalloc(int device, int dim, const uint32_t *size, const u
     nc.red = (int)((dim * ncoef + (dim == 3 ? 64 : kBlock) - 1) / (dim == 3 ? 64 : kBlock));
     nc.s1z = d.ncz * d.ncy;
     nc.redz = (int)((dim * ncoef + kBlock - 1) / kBlock);
-    const int variant = dim == 3 ? tuned_variant(device, degree, nvox) : 0;
+    const int variant = tuned_choice(device, dim == 3 ? MIA_HIP_TUNE_WARP3D : MIA_HIP_TUNE_WARP2D, degree, nvox);
     int w2_win = 0;
     if (dim == 3) {
         int tx, ty, tz;
         w3_tile_dims(variant, tx, ty, tz);
         nc.tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty) * ((d.nz + tz - 1) / tz);
     } else {
-        nc.tiles = ((d.nx + kW2TX - 1) / kW2TX) * ((d.ny + kW2TY - 1) / kW2TY);
-        // knots spanned by the tiles of k_warp2d (the window it loads)
-        auto span = [](int n, int cs, int t) {
-            float w[4];
-            int m = 0;
-            for (int p0 = 0; p0 < n; p0 += t)
-                m = std::max(m, spline_axis_entry(std::min(p0 + t, n) - 1, n, cs, w) + 4 -
-                                    spline_axis_entry(p0, n, cs, w));
-            return m;
-        };
-        w2_win = span(d.nx, d.cx, kW2TX) * span(d.ny, d.cy, kW2TY);
+        const int tx = w2_tile_x(variant), ty = w2_tile_y(variant);
+        nc.tiles = ((d.nx + tx - 1) / tx) * ((d.ny + ty - 1) / ty);
+        // the coefficient window of a tile (from the same start indices as
+        // the context's axis tables)
+        float w[4];
+        w2_win = tile_span(d.nx, tx, [&](int p) { return spline_axis_entry(p, d.nx, d.cx, w); }) *
+                 tile_span(d.ny, ty, [&](int p) { return spline_axis_entry(p, d.ny, d.cy, w); });
         if (w2_win > kW2Win)
             return fail(MIA_HIP_ERR_ARG, "2D coefficient window exceeds its bound (spline rate below 1?)");
     }

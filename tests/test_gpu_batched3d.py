@@ -62,6 +62,8 @@ def test_mixed_stage1_paths_bitwise():
         data.append((mov, fix, cs, synth.random_spline_params(cs, 1.5, 3.0, 700 + k)))
     with mia_amd.HipContext(0) as c:
         c.setup(3, shapes, [d[2] for d in data], 3)
+        print("stage-1 paths", [c.stage1_path(i) for i in range(len(shapes))], flush=True)
+        assert c.stage1_path(0) == 0 and c.stage1_path(1) == 0
         for i, d in enumerate(data):
             c.set_images(i, d[0], d[1])
         cb, gb = c.cost_grad(np.concatenate([d[3] for d in data]))

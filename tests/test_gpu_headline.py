@@ -46,10 +46,11 @@ def headline():
     return mov, fix, cs, p
 
 
-def _gpu(mov, fix, cs, p, mode, with_warp=False):
+def _gpu(mov, fix, cs, p, mode, with_warp=False, s1path=-1):
     import mia_amd
     shape = mov.shape
     with mia_amd.HipContext(0) as c:
+        c.set_stage1_path(s1path)
         c.setup(3, [shape], [cs], 3)
         c.set_images(0, mov, fix)
         c.set_cost(grad_mode=mode)
@@ -59,14 +60,15 @@ def _gpu(mov, fix, cs, p, mode, with_warp=False):
     return costs[0], g, W, path
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_headline_vs_oracle(headline, mode):
+@pytest.mark.parametrize("mode,s1path", [(0, 1), (0, 0), (1, -1)])
+def test_headline_vs_oracle(headline, mode, s1path):
     mov, fix, cs, p = headline
-    print("256^3 rate 8 cubic, gradient mode", mode, flush=True)
-    cost, g, W, path = _gpu(mov, fix, cs, p, mode, with_warp=True)
-    # the reference gradient takes the z-cell stage 1 (every lane of the
-    # four waves live, x = 255 on lane 63 of wave 3)
-    assert path == 1
+    print("256^3 rate 8 cubic, gradient mode", mode, "stage-1 layout", s1path, flush=True)
+    cost, g, W, path = _gpu(mov, fix, cs, p, mode, with_warp=True, s1path=s1path)
+    # the reference gradient in either stage-1 layout (z cell: every lane of
+    # the four waves live, x = 255 on lane 63 of wave 3); the exact gradient
+    # always per slice
+    assert path == (s1path if mode == 0 else 0)
     c_ref, g_ref, W_ref = O.cost_grad(mov, fix, cs, p, grad_mode=mode, want_warped=True)
     e_c = abs(cost - c_ref) / abs(c_ref)
     e_g = _rel_inf(g, g_ref)
@@ -105,6 +107,7 @@ def test_zcell_wide_rows(shape, rate, degree):
     cs = tuple(O.spline_csize(n, rate, 3) for n in shape[::-1])
     p = synth.random_spline_params(cs, 1.5, 4.0, 13)
     with mia_amd.HipContext(0) as c:
+        c.set_stage1_path(1)
         c.setup(3, [shape], [cs], degree)
         c.set_images(0, mov, fix)
         assert c.stage1_path(0) == 1

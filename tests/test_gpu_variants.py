@@ -1,8 +1,9 @@
-"""GPU tests of the 3D warp tile variants and their autotuner: every variant
-against the FP64 oracle (both interpolation degrees, including tiles whose
-tap box exceeds the LDS pitches), batches whose instances run different
-variants against one context per instance (bitwise), and the autotuner's
-table."""
+"""GPU tests of the tuned choices and their autotuner: every 3D and 2D warp
+tile variant and both 3D stage-1 layouts against the FP64 oracle (both
+interpolation degrees, including tiles whose tap box exceeds the LDS pitches
+and images that the tiles do not divide), batches whose instances run
+different choices against one context per instance (bitwise), and the
+autotuner's table for each kind."""
 import json
 
 import numpy as np
@@ -21,7 +22,7 @@ def _inputs(shape, rate, seed, sd=1.5, clip=3.0):
     from mia_amd import synth
     fix = synth.smoothed_noise(shape, 2.0, seed)
     mov = synth.smoothed_noise(shape, 2.0, seed + 100)
-    cs = tuple(O.spline_csize(n, rate, 3) for n in shape[::-1])
+    cs = tuple(O.spline_csize(n, rate, len(shape)) for n in shape[::-1])
     p = synth.random_spline_params(cs, sd, clip, seed + 200)
     return mov, fix, cs, p
 
@@ -101,5 +102,124 @@ def test_autotune_and_mixed_variant_batch(tmp_path):
             off += g1.size
             c_ref, g_ref = O.cost_grad(d[0], d[1], d[2], d[3])
             assert c1s[0] == pytest.approx(c_ref, rel=COST_RTOL)
+    finally:
+        L.mia_hip_load_warp_tuning(0, None)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape,rate,degree", [((97, 131), 8.0, 3), ((50, 230), 6.0, 1), ((224, 96), 8.0, 3)])
+def test_warp2d_variant_matches_oracle(variant, shape, rate, degree):
+    import mia_amd
+    mov, fix, cs, p = _inputs(shape, rate, 21, 1.5, 4.0)
+    with mia_amd.HipContext(0) as c:
+        c.set_warp_variant(variant)
+        c.setup(2, [shape], [cs], degree)
+        assert c.warp_variant(0) == variant
+        c.set_images(0, mov, fix)
+        costs, g = c.cost_grad(p)
+        w = c.warp(0, p)
+    with mia_amd.HipContext(0) as c0:
+        c0.set_warp_variant(0)
+        c0.setup(2, [shape], [cs], degree)
+        c0.set_images(0, mov, fix)
+        w0 = c0.warp(0, p)
+    c_ref, g_ref, w_ref = O.cost_grad(mov, fix, cs, p, degree=degree, want_warped=True)[:3]
+    print("variant", variant, "warped max diff to variant 0", float(np.abs(w - w0).max()))
+    assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
+    assert np.abs(g - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max()
+    assert np.abs(w - w_ref).max() <= WARP_ATOL
+
+
+@pytest.mark.parametrize("shape,rate,degree", [((44, 52, 61), 8.0, 3), ((44, 52, 61), 8.0, 1),
+                                               ((32, 32, 32), 6.0, 3)])
+def test_stage1_layouts_match_oracle(shape, rate, degree):
+    """Both 3D stage-1 layouts of the reference gradient on instances that
+    allow the z-cell layout: each against the oracle, and against each other
+    to FP32 summation-order rounding.  32^3 alone in its context: three of
+    the block's four waves lie beyond the row, and the image ends where the
+    allocation does (edge-lane loads clamped to the row)."""
+    import mia_amd
+    mov, fix, cs, p = _inputs(shape, rate, 41, 1.5, 4.0)
+    res = {}
+    for path in (0, 1):
+        with mia_amd.HipContext(0) as c:
+            c.set_stage1_path(path)
+            c.setup(3, [shape], [cs], degree)
+            assert c.stage1_path(0) == path
+            c.set_images(0, mov, fix)
+            res[path] = c.cost_grad(p)
+    c_ref, g_ref = O.cost_grad(mov, fix, cs, p, degree=degree)[:2]
+    for path, (costs, g) in res.items():
+        assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
+        assert np.abs(g - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max()
+    gs, gz = res[0][1], res[1][1]
+    print("layouts: grad max diff", float(np.abs(gs - gz).max()), "of", float(np.abs(g_ref).max()))
+    assert np.abs(gs - gz).max() <= 1e-6 * np.abs(g_ref).max()
+
+
+def test_stage1_path_errors():
+    import mia_amd
+    with mia_amd.HipContext(0) as c:
+        for bad in (-2, 2):
+            with pytest.raises(ValueError):
+                c.set_stage1_path(bad)
+
+
+def test_autotune_2d_and_stage1_tables(tmp_path):
+    """Tune the 2D warp tile and the stage-1 layout into one table (each
+    tuning keeps the other kind's entries), check the library reports the
+    architecture it is keyed by, then load tables that map size buckets to
+    different 2D variants and stage-1 layouts: batches with one instance per
+    bucket match one context per instance bitwise."""
+    import mia_amd
+    from mia_amd import _lib, autotune
+    L = _lib.load()
+    out = str(tmp_path / "tuned.json")
+    try:
+        t2, us2 = autotune.run(out, degree=3, b_min=12, b_max=13, kind="warp2d")
+        assert us2.shape == (1, 2, len(autotune.variants_2d())) and np.all(us2 > 0)
+        t3, us3 = autotune.run(out, degree=3, b_min=15, b_max=15, kind="stage1_3d")
+        assert us3.shape == (1, 1, 2) and np.all(us3 > 0)
+        assert t3["arch"] == autotune.device_arch(0) and t3["arch"].startswith("gfx")
+        kinds = [r["kernel"] for r in t3["table"]]
+        assert kinds.count("warp2d") == 2 and kinds.count("stage1_3d") == 1
+        for r in t3["table"]:
+            assert r["variant"] == int(np.argmin(r["us"]))
+        src, n = autotune.tuning_source()
+        assert src == out and n == 3
+
+        mixed = {"arch": t3["arch"], "table": [
+            {"kernel": "warp2d", "degree": 3, "bucket": b, "variant": v} for b, v in ((13, 1), (14, 2), (15, 3), (16, 0))
+        ] + [{"kernel": "stage1_3d", "degree": 3, "bucket": b, "variant": v} for b, v in ((15, 0), (16, 1))]}
+        path = str(tmp_path / "mixed.json")
+        with open(path, "w") as f:
+            json.dump(mixed, f)
+        _lib.check(L.mia_hip_load_warp_tuning(0, path.encode()))
+        cases = {2: [(70, 101), (110, 130), (181, 170), (250, 251)],   # 2^13 .. 2^16 pixels
+                 3: [(30, 33, 31), (40, 41, 39)]}                       # 2^15, 2^16 voxels
+        for dim, shapes in cases.items():
+            rate = 8.0
+            data = [_inputs(s, rate, 60 + k, 1.5, 4.0) for k, s in enumerate(shapes)]
+            with mia_amd.HipContext(0) as c:
+                c.setup(dim, shapes, [d[2] for d in data], 3)
+                if dim == 2:
+                    assert [c.warp_variant(i) for i in range(4)] == [1, 2, 3, 0]
+                else:
+                    assert [c.stage1_path(i) for i in range(2)] == [0, 1]
+                for i, d in enumerate(data):
+                    c.set_images(i, d[0], d[1])
+                cb, gb = c.cost_grad(np.concatenate([d[3] for d in data]))
+            off = 0
+            for i, (s, d) in enumerate(zip(shapes, data)):
+                with mia_amd.HipContext(0) as c1:
+                    c1.setup(dim, [s], [d[2]], 3)
+                    c1.set_images(0, d[0], d[1])
+                    c1s, g1 = c1.cost_grad(d[3])
+                assert c1s[0] == cb[i]
+                assert np.array_equal(g1, gb[off:off + g1.size])
+                off += g1.size
+                c_ref, g_ref = O.cost_grad(d[0], d[1], d[2], d[3])[:2]
+                assert c1s[0] == pytest.approx(c_ref, rel=COST_RTOL)
+                assert np.abs(g1 - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max()
     finally:
         L.mia_hip_load_warp_tuning(0, None)
