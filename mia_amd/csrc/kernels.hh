@@ -2105,16 +2105,40 @@ __global__ void __launch_bounds__(kBlock)
             const int za = zb + (len * g4) / 4, zc = zb + (len * (g4 + 1)) / 4;
             const float *twf = (const float *)tw;
             const long long row = 12LL * d.cx;
-            for (int z = za; z < zc; ++z) {
-                const double wz = twf[4 * (d.tab_z + z) + (kz - ts[d.tab_z + z])];
-                double s = 0.0;
+            // kU slices at a time: their weights and partial sums are loaded
+            // before any is added (a slice at a time waits out the memory
+            // latency once per slice), then added in the same z and t order
+            constexpr int kU = 8;
+            for (int z0 = za; z0 < zc; z0 += kU) {
+                float v[kU][4], wzf[kU];
+                int tsz[kU];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int band = ky - t;
-                    if (band >= 0 && band < d.ncy)
-                        s += P[d.part_off + ((long long)z * d.ncy + band) * row + (t * d.cx + kx) * 3 + comp];
+                for (int u = 0; u < kU; ++u)
+                    tsz[u] = ts[d.tab_z + min(z0 + u, zc - 1)];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int z = min(z0 + u, zc - 1);
+                    wzf[u] = twf[4 * (d.tab_z + z) + (kz - tsz[u])];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        // unconditional loads from a clamped band (a load
+                        // under a branch is waited for at the branch's end)
+                        const int band = min(max(ky - t, 0), d.ncy - 1);
+                        v[u][t] = P[d.part_off + ((long long)z * d.ncy + band) * row + (t * d.cx + kx) * 3 + comp];
+                    }
                 }
-                g += wz * s;
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (z0 + u < zc) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int band = ky - t;
+                            if (band >= 0 && band < d.ncy)
+                                s += v[u][t];
+                        }
+                        g += (double)wzf[u] * s;
+                    }
             }
         }
         part[g4][lane] = g;
@@ -2138,14 +2162,16 @@ __global__ void __launch_bounds__(kBlock)
             kx = (int)(e % d.cx);
             ky = (int)(e / d.cx);
         }
-        // the four partial sums in flight first (a missing one reads as 0),
-        // then the sum in the fixed t order
+        // the four partial sums in flight first (unconditional loads from a
+        // clamped band, a missing one then reads as 0), then the sum in the
+        // fixed t order
         float v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int band = ky - t;
-            v[t] = band >= 0 && band < d.ncy ? P[d.part_off + (long long)band * 8 * d.cx + (t * d.cx + kx) * 2 + comp]
-                                             : 0.0f;
+            const int bc = min(max(band, 0), d.ncy - 1);
+            const float q = P[d.part_off + (long long)bc * 8 * d.cx + (t * d.cx + kx) * 2 + comp];
+            v[t] = band >= 0 && band < d.ncy ? q : 0.0f;
         }
         double g = 0.0;
 #pragma unroll
@@ -2197,17 +2223,21 @@ __global__ void __launch_bounds__(kBlock)
     const int ky = (int)(row - (unsigned)kz * (unsigned)d.cy);
     const float *Pi = P + d.part_off + 3 * kx + comp;
     const long long cell_stride = 48LL * d.cx;
-    // all 16 partial sums in flight first (a missing one reads as 0), then
-    // the sum in the fixed (k, t) order
+    // all 16 partial sums in flight first (unconditional loads from a
+    // clamped cell and band, a missing one then reads as 0), then the sum in
+    // the fixed (k, t) order
     float v[16];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int cz = kz - k;
+        const int czc = min(max(cz, 0), d.ncz - 1);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int band = ky - t;
+            const int bc = min(max(band, 0), d.ncy - 1);
             const bool in = cz >= 0 && cz < d.ncz && band >= 0 && band < d.ncy;
-            v[4 * k + t] = in ? Pi[((long long)cz * d.ncy + band) * cell_stride + (4 * k + t) * 3LL * d.cx] : 0.0f;
+            const float q = Pi[((long long)czc * d.ncy + bc) * cell_stride + (4 * k + t) * 3LL * d.cx];
+            v[4 * k + t] = in ? q : 0.0f;
         }
     }
     double g = 0.0;
