@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "cost+gradient Mvoxel/s (3D 256³) + batched 2D regs/sec at equal SSD"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernels_cost_grad_3d_256cube_rate8_traffic.md")
+COUNTERS_PROFILE = os.path.join(ROOT, "profiles", "kernels_cost_grad_3d_256cube_rate8_counters.md")
 
 
 def log(*a):
@@ -116,6 +117,25 @@ def profiled_traffic(stage, size, rate, grad_mode):
         return None
     k = js["kernels"].get(stage)
     return k["traffic_bytes"] if k else None
+
+
+def profiled_counters(stage, size, rate, grad_mode):
+    """VALU / LDS figures of `stage` from the committed SQ-counter summary
+    (VALU floor = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz), its share
+    of the kernel time, LDS bank-conflict cycles per active LDS cycle), when
+    that summary covers this workload; None otherwise."""
+    try:
+        text = open(COUNTERS_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None
+    js = json.loads(m.group(1))
+    w = js["workload"]
+    if w["size"] != size or abs(w["rate"] - rate) > 1e-9 or w["grad_mode"] != grad_mode:
+        return None
+    return js["kernels"].get(stage)
 
 
 def cpu_baseline(mov, fix, cs, params, threads):
@@ -350,36 +370,39 @@ def registration_3d(threads):
     a, b = series.normalize_pair(mov, fix)
     c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
     log("registration 3D: %.2f s, final cost %.6g (identity %.6g)" % (wall, final, c_id))
-    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
-    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
-    L.mcp_register_3d.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
-                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-    L.mcp_register_3d.restype = ctypes.c_double
     lsec = np.zeros(3)
     lev = np.zeros(3, dtype=np.int32)
     lcost = np.zeros(3)
-    cfin = ctypes.c_double()
-    ccs = np.zeros(3, dtype=np.int32)
-    cpu_sec = L.mcp_register_3d(np.ascontiguousarray(mov, dtype=np.float32), np.ascontiguousarray(fix),
-                                shape[2], shape[1], shape[0], 8.0, 3, 200, 0.01, threads, lsec.ctypes.data,
-                                lev.ctypes.data, lcost.ctypes.data, ctypes.byref(cfin), ccs.ctypes.data)
-    log("registration 3D, CPU port (%d threads): %.2f s, final cost %.6g" % (threads, cpu_sec, cfin.value))
+    cpu_sec = None
+    if threads:
+        L = ctypes.CDLL(os.path.join(ROOT, "oracle", "lib", "libmia_cpu_port.so"))
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+        L.mcp_register_3d.argtypes = [fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.mcp_register_3d.restype = ctypes.c_double
+        cfin = ctypes.c_double()
+        ccs = np.zeros(3, dtype=np.int32)
+        cpu_sec = L.mcp_register_3d(np.ascontiguousarray(mov, dtype=np.float32), np.ascontiguousarray(fix),
+                                    shape[2], shape[1], shape[0], 8.0, 3, 200, 0.01, threads, lsec.ctypes.data,
+                                    lev.ctypes.data, lcost.ctypes.data, ctypes.byref(cfin), ccs.ctypes.data)
+        log("registration 3D, CPU port (%d threads): %.2f s, final cost %.6g" % (threads, cpu_sec, cfin.value))
     levels = []
     for k, st in enumerate(stats):
         levels.append({"size": list(st["size"]), "coeff_size": list(st["coeff_size"]),
                        "evaluations": st["evaluations"], "gpu_seconds": round(st["seconds"], 4), "cost": st["cost"],
                        "gpu_ms_per_evaluation": round(1e3 * st["seconds"] / max(st["evaluations"], 1), 4),
-                       "cpu_port_seconds": round(float(lsec[k]), 3), "cpu_port_evaluations": int(lev[k]),
-                       "cpu_port_cost": float(lcost[k])})
+                       "cpu_port_seconds": round(float(lsec[k]), 3) if threads else None,
+                       "cpu_port_evaluations": int(lev[k]) if threads else None,
+                       "cpu_port_cost": float(lcost[k]) if threads else None})
     return {"workload": "3D 192x192x160, 3 levels, spline rate 8 + divcurl 0.01, ssd, lbfgs maxiter=200",
             "seconds": round(wall, 3), "final_cost": final, "identity_cost": c_id,
             "coeff_size": list(grid), "levels": levels,
             "cpu_port": {"seconds": round(cpu_sec, 3), "threads": threads, "kind": "port",
                          "final_cost": cfin.value, "coeff_size": [int(v) for v in ccs],
                          "sample": "the whole registration, FP32 port of MIA's 3D data flow and pfor threading, "
-                                   "same L-BFGS"},
-            "speedup_vs_cpu_port": round(cpu_sec / wall, 1)}
+                                   "same L-BFGS"} if threads else None,
+            "speedup_vs_cpu_port": round(cpu_sec / wall, 1) if threads else None}
 
 
 def copy_calibration(mib, reps):
@@ -522,8 +545,13 @@ def main():
                 "alg_bytes": kd["alg_bytes"],
                 "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                 # what bounds the 3D warp instead: LDS gathers and VALU issue (SQ counters)
-                "limiter": "lds+valu" if dom == "warp" else None,
-                "counters_source": "profiles/kernels_cost_grad_3d_256cube_rate8_counters.md"}
+                "limiter": "lds+valu" if dom == "warp" else None}
+    cnt = profiled_counters(dom, n, args.rate, args.grad_mode)
+    if cnt:
+        roofline.update({"valu_issue_frac": cnt["valu_issue_frac"], "valu_floor_us": cnt["valu_floor_us"],
+                         "valu_per_voxel": cnt["valu_per_voxel"],
+                         "lds_conflict_per_active_cycle": cnt["lds_conflict_per_active_cycle"],
+                         "counters_source": os.path.relpath(COUNTERS_PROFILE, ROOT)})
     tsum = sum(v["avg_us"] for v in kernels.values())
     log("kernels: %s" % json.dumps(kernels))
 
@@ -553,7 +581,7 @@ def main():
     reg = reg2 = None
     if not args.no_registration:
         reg2 = registration_2d()
-        reg = registration_3d(args.cpu_threads or min(16, os.cpu_count() or 1))
+        reg = registration_3d(0 if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1)))
 
     out = {
         "metric": METRIC,
