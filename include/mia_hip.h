@@ -188,12 +188,15 @@ int mia_hip_device_arch(int device, char *buf, int len);
 /* The tile shapes (x, y, z voxels per block) of the 3D warp kernel's
  * variants; tile_dims receives 3 ints per variant (up to max_variants). */
 int mia_hip_warp_variants(int *tile_dims, int max_variants, int *n_variants);
-/* The tile shapes (x, y pixels per block) of the 2D warp kernel's variants;
- * tile_dims receives 2 ints per variant (up to max_variants). */
+/* The tile shapes of the 2D warp kernel's variants: tile_dims receives 3
+ * ints per variant (up to max_variants): x, y pixels per block and 1 when the
+ * tile's taps go through an LDS box, 0 when each pixel gathers them from the
+ * image. */
 int mia_hip_warp2d_variants(int *tile_dims, int max_variants, int *n_variants);
 
 /* Force tile variant `variant` (0..n-1) for the instances of the next
- * mia_hip_setup on this context (3D or 2D variants by the setup's dim), or
+ * mia_hip_setup on this context (3D or 2D variants by the setup's dim; a
+ * number beyond the dim's variants makes that setup fail), or
  * -1 (default) for the tuned choice per size bucket.  Results do not depend
  * on the variant beyond FP32 rounding of the displacement contraction;
  * every instance always uses the same variant for a given size, so batched

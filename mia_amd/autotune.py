@@ -39,12 +39,12 @@ def variants():
 
 
 def variants_2d():
-    """2D warp tile shapes (x, y) by variant id."""
+    """2D warp tile shapes (x, y, "box" or "direct") by variant id."""
     L = _lib.load()
-    dims = (ctypes.c_int * 32)()
+    dims = (ctypes.c_int * 48)()
     n = ctypes.c_int()
     check(L.mia_hip_warp2d_variants(dims, 16, ctypes.byref(n)))
-    return [tuple(dims[2 * v:2 * v + 2]) for v in range(n.value)]
+    return [(dims[3 * v], dims[3 * v + 1], "box" if dims[3 * v + 2] else "direct") for v in range(n.value)]
 
 
 def choices(kind):

@@ -83,13 +83,22 @@ __host__ __device__ inline void w3_tile_dims(int v, int &tx, int &ty, int &tz)
     }
 }
 // 2D warp tile variants (x, y pixels per block of kBlock threads; a thread
-// takes one x column and kBlock / x rows apart) and LDS budgets
-constexpr int kW2Variants = 4;
-__host__ __device__ constexpr int w2_tile_x(int v) { return v == 2 ? 32 : v == 3 ? 128 : 64; }
-__host__ __device__ constexpr int w2_tile_y(int v) { return v == 1 || v == 3 ? 16 : 32; }
+// takes one x column and kBlock / x rows apart; the taps from an LDS box of
+// the tile, or gathered straight from the image: variants 4 and 5) and LDS
+// budgets
+constexpr int kW2Variants = 6;
+__host__ __device__ constexpr int w2_tile_x(int v) { return v == 2 || v == 5 ? 32 : v == 3 ? 128 : 64; }
+__host__ __device__ constexpr int w2_tile_y(int v) { return v == 1 || v == 3 || v == 4 ? 16 : 32; }
+__host__ __device__ constexpr bool w2_box(int v) { return v < 4; }
 // float2, largest coefficient window over the variants (rate 1)
 constexpr int kW2Win = (128 + 4) * (16 + 4);
-constexpr int kW2Region = 6144;                     // floats
+constexpr int kW2Region = 6144;                     // floats: the tap box
+// floats ahead of the coefficient window in the dynamic LDS: the tap box, or
+// without one the y-contracted displacement of the tile (TY x (TX + 4) float2)
+__host__ __device__ constexpr int w2_region(int v)
+{
+    return w2_box(v) ? kW2Region : w2_tile_y(v) * (w2_tile_x(v) + 4) * 2;
+}
 
 // Per-instance description, identical on host and device.
 struct InstDesc {
@@ -1236,7 +1245,11 @@ __global__ void __launch_bounds__(kBlock)
 
 // 2D warp.  Block = TX x TY pixels, thread = TY * TX / kBlock pixels of one
 // x column (rows y0 + ly + (kBlock / TX) k).  One coefficient window per tile.
-template <int DEG, int TX, int TY>
+// BOX: the tile's taps in an LDS box sized from the extremes of its sample
+// positions; otherwise each pixel gathers its taps from the image (small
+// images stay in L2), with no box extent, fill or barrier.  Both contract in
+// the same order, so every variant computes the same image.
+template <int DEG, int TX, int TY, bool BOX>
 __global__ void __launch_bounds__(kBlock)
     k_warp2d(const InstDesc *__restrict__ inst, const int *__restrict__ ids, int n_inst,
              const int *__restrict__ prefix, int boff, int nblocks,
@@ -1247,12 +1260,14 @@ __global__ void __launch_bounds__(kBlock)
     static_assert(TX <= kBlock && kBlock % TX == 0 && TY % (kBlock / TX) == 0, "tile shape");
     static_assert((TX + 4) * (TY + 4) <= kW2Win && TY * (TX + 4) <= kW2Region / 2, "tile LDS budget");
     constexpr int RPT = TY / (kBlock / TX);   // rows per thread
-    // dynamic LDS: the image box (kW2Region floats), then the coefficient
+    // dynamic LDS: the image box (kW2Region floats; without a box only the
+    // y-contracted displacement, TY x (TX + 4) float2), then the coefficient
     // window, sized by the host for the launch's largest window (about 1 KB
     // at rate 8, so that six blocks fit a CU)
+    constexpr int REG = BOX ? kW2Region : TY * (TX + 4) * 2;
     extern __shared__ float smem[];
     float *region = smem;
-    float2 *win = (float2 *)(smem + kW2Region);
+    float2 *win = (float2 *)(smem + REG);
     __shared__ float red[4][kBlock / 64];
     __shared__ float4 wy_s[TY];
     __shared__ int sy_s[TY];
@@ -1329,33 +1344,38 @@ __global__ void __launch_bounds__(kBlock)
             mxy = fmaxf(mxy, py[k]);
         }
     }
-    mnx = wave_min(mnx);
-    mny = wave_min(mny);
-    mxx = wave_max(mxx);
-    mxy = wave_max(mxy);
-    const int wid = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][wid] = mnx;
-        red[1][wid] = mny;
-        red[2][wid] = mxx;
-        red[3][wid] = mxy;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kBlock / 64; ++k) {
-        mnx = fminf(mnx, red[0][k]);
-        mny = fminf(mny, red[1][k]);
-        mxx = fmaxf(mxx, red[2][k]);
-        mxy = fmaxf(mxy, red[3][k]);
-    }
-    const int rx0 = fastfloor16(mnx) - LO, ry0 = fastfloor16(mny) - LO;
-    const int RX = fastfloor16(mxx) - LO + NS - rx0;
-    const int RY = fastfloor16(mxy) - LO + NS - ry0;
-    const bool use_lds = (long long)RX * RY <= kW2Region;
+    int rx0 = 0, ry0 = 0, RX = 0;
+    bool use_lds = false;
     const float *Mi = M + d.img_off;
-    if (use_lds) {
-        fill_box2(region, Mi, d.nx, d.ny, rx0, ry0, RX, RY);
+    if constexpr (BOX) {
+        mnx = wave_min(mnx);
+        mny = wave_min(mny);
+        mxx = wave_max(mxx);
+        mxy = wave_max(mxy);
+        const int wid = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[0][wid] = mnx;
+            red[1][wid] = mny;
+            red[2][wid] = mxx;
+            red[3][wid] = mxy;
+        }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBlock / 64; ++k) {
+            mnx = fminf(mnx, red[0][k]);
+            mny = fminf(mny, red[1][k]);
+            mxx = fmaxf(mxx, red[2][k]);
+            mxy = fmaxf(mxy, red[3][k]);
+        }
+        rx0 = fastfloor16(mnx) - LO;
+        ry0 = fastfloor16(mny) - LO;
+        RX = fastfloor16(mxx) - LO + NS - rx0;
+        const int RY = fastfloor16(mxy) - LO + NS - ry0;
+        use_lds = (long long)RX * RY <= kW2Region;
+        if (use_lds) {
+            fill_box2(region, Mi, d.nx, d.ny, rx0, ry0, RX, RY);
+            __syncthreads();
+        }
     }
     if (!xvalid)
         return;

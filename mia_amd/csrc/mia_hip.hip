@@ -244,7 +244,7 @@ constexpr int kTuneKinds = 3; // MIA_HIP_TUNE_WARP3D, _WARP2D, _STAGE1
 const char *const kTuneKindNames[kTuneKinds] = {"warp3d", "warp2d", "stage1_3d"};
 constexpr int kTuneChoices[kTuneKinds] = {kW3Variants, kW2Variants, 2};
 constexpr int kTuneDefault[kTuneKinds] = {0, 0, 1};
-constexpr int kMaxTuneChoices = 4;
+constexpr int kMaxTuneChoices = 6;
 static_assert(kW3Variants <= kMaxTuneChoices && kW2Variants <= kMaxTuneChoices, "variant tables");
 
 struct Tuning {
@@ -554,13 +554,22 @@ EvalPlan plan_of(const mia_hip_ctx *c)
 }
 
 // the 2D warp kernel of a degree and tile variant
-decltype(&k_warp2d<3, 64, 32>) warp2d_kernel(int degree, int v)
+template <int V> constexpr auto warp2d_v(int degree)
 {
+    return degree == 3 ? k_warp2d<3, w2_tile_x(V), w2_tile_y(V), w2_box(V)>
+                       : k_warp2d<1, w2_tile_x(V), w2_tile_y(V), w2_box(V)>;
+}
+
+decltype(&k_warp2d<3, 64, 32, true>) warp2d_kernel(int degree, int v)
+{
+    static_assert(kW2Variants == 6, "one case per variant");
     switch (v) {
-    case 1: return degree == 3 ? k_warp2d<3, w2_tile_x(1), w2_tile_y(1)> : k_warp2d<1, w2_tile_x(1), w2_tile_y(1)>;
-    case 2: return degree == 3 ? k_warp2d<3, w2_tile_x(2), w2_tile_y(2)> : k_warp2d<1, w2_tile_x(2), w2_tile_y(2)>;
-    case 3: return degree == 3 ? k_warp2d<3, w2_tile_x(3), w2_tile_y(3)> : k_warp2d<1, w2_tile_x(3), w2_tile_y(3)>;
-    default: return degree == 3 ? k_warp2d<3, w2_tile_x(0), w2_tile_y(0)> : k_warp2d<1, w2_tile_x(0), w2_tile_y(0)>;
+    case 1: return warp2d_v<1>(degree);
+    case 2: return warp2d_v<2>(degree);
+    case 3: return warp2d_v<3>(degree);
+    case 4: return warp2d_v<4>(degree);
+    case 5: return warp2d_v<5>(degree);
+    default: return warp2d_v<0>(degree);
     }
 }
 
@@ -580,7 +589,7 @@ int launch_warp(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const doub
                pl.M, pl.clamp, pl.W);
     } else {
         auto kern = warp2d_kernel(pl.degree, G.variant);
-        const size_t lds = (size_t)kW2Region * sizeof(float) + (size_t)pl.w2_win * sizeof(float2);
+        const size_t lds = (size_t)w2_region(G.variant) * sizeof(float) + (size_t)pl.w2_win * sizeof(float2);
         launch(s, prof, ST_WARP, kern, dim3(nb), lds, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw,
                pl.ts, pl.M, pl.clamp, pl.W);
     }
@@ -868,6 +877,9 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         return fail(MIA_HIP_ERR_ARG, "need at least one instance with sizes and coefficient sizes");
     if (degree != 1 && degree != 3)
         return fail(MIA_HIP_ERR_ARG, "image interpolation kernel must be bspline:d=1 or bspline:d=3");
+    if (c->force_variant >= (dim == 3 ? kW3Variants : kW2Variants))
+        return fail(MIA_HIP_ERR_ARG, "forced warp variant " + std::to_string(c->force_variant) + " does not exist in " +
+                                         std::to_string(dim) + "D");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     free_buffers(c);
@@ -1556,8 +1568,9 @@ int mia_hip_warp2d_variants(int *tile_dims, int max_variants, int *n_variants)
     if (n_variants)
         *n_variants = kW2Variants;
     for (int v = 0; v < kW2Variants && v < max_variants && tile_dims; ++v) {
-        tile_dims[2 * v] = w2_tile_x(v);
-        tile_dims[2 * v + 1] = w2_tile_y(v);
+        tile_dims[3 * v] = w2_tile_x(v);
+        tile_dims[3 * v + 1] = w2_tile_y(v);
+        tile_dims[3 * v + 2] = w2_box(v) ? 1 : 0;
     }
     return MIA_HIP_OK;
 }
@@ -1566,9 +1579,10 @@ int mia_hip_set_warp_variant(mia_hip_ctx *ctx, int variant)
 {
     if (!ctx)
         return fail(MIA_HIP_ERR_ARG, "null context");
-    const int nv = std::min(kW3Variants, kW2Variants);
+    const int nv = std::max(kW3Variants, kW2Variants);
     if (variant < -1 || variant >= nv)
-        return fail(MIA_HIP_ERR_ARG, "warp variant must be -1 (tuned) or 0.." + std::to_string(nv - 1));
+        return fail(MIA_HIP_ERR_ARG, "warp variant must be -1 (tuned) or 0.." + std::to_string(nv - 1) +
+                                         " (3D: 0.." + std::to_string(kW3Variants - 1) + ")");
     ctx->force_variant = variant;
     return MIA_HIP_OK;
 }
@@ -1779,7 +1793,7 @@ int mia_hip_autotune(int device, int kind, int degree, int b_min, int b_max, con
     }
     std::fprintf(f, "], \"warp2d\": [");
     for (int v = 0; v < kW2Variants; ++v)
-        std::fprintf(f, "%s[%d, %d]", v ? ", " : "", w2_tile_x(v), w2_tile_y(v));
+        std::fprintf(f, "%s[%d, %d, \"%s\"]", v ? ", " : "", w2_tile_x(v), w2_tile_y(v), w2_box(v) ? "box" : "direct");
     std::fprintf(f, "], \"stage1_3d\": [\"slice\", \"zcell\"]},\n \"table\": [\n");
     size_t left = keep.size() + rows.size();
     for (const std::string &e : keep)

@@ -53,12 +53,22 @@ def test_variant_matches_oracle(variant, shape, rate, degree, sd, clip):
 
 
 def test_variant_errors():
+    """Out-of-range variants fail: beyond every dim's variants at once, beyond
+    the 3D variants (but a 2D one) at the 3D set-up."""
     import mia_amd
+    from mia_amd import autotune
+    n2 = len(autotune.variants_2d())
+    assert n2 > _n_variants()
     with mia_amd.HipContext(0) as c:
         with pytest.raises(ValueError):
-            c.set_warp_variant(_n_variants())
+            c.set_warp_variant(n2)
         with pytest.raises(ValueError):
             c.set_warp_variant(-2)
+        c.set_warp_variant(_n_variants())
+        with pytest.raises(ValueError):
+            c.setup(3, [(20, 20, 20)], [(6, 6, 6)], 3)
+        c.setup(2, [(40, 40)], [(8, 8)], 3)
+        assert c.warp_variant(0) == _n_variants()
 
 
 def test_autotune_and_mixed_variant_batch(tmp_path):
@@ -106,7 +116,7 @@ def test_autotune_and_mixed_variant_batch(tmp_path):
         L.mia_hip_load_warp_tuning(0, None)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("shape,rate,degree", [((97, 131), 8.0, 3), ((50, 230), 6.0, 1), ((224, 96), 8.0, 3)])
 def test_warp2d_variant_matches_oracle(variant, shape, rate, degree):
     import mia_amd
@@ -125,6 +135,8 @@ def test_warp2d_variant_matches_oracle(variant, shape, rate, degree):
         w0 = c0.warp(0, p)
     c_ref, g_ref, w_ref = O.cost_grad(mov, fix, cs, p, degree=degree, want_warped=True)[:3]
     print("variant", variant, "warped max diff to variant 0", float(np.abs(w - w0).max()))
+    # every tile shape and tap source contracts in the same order: the same image
+    assert np.array_equal(w, w0)
     assert costs[0] == pytest.approx(c_ref, rel=COST_RTOL)
     assert np.abs(g - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max()
     assert np.abs(w - w_ref).max() <= WARP_ATOL
