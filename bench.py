@@ -391,6 +391,8 @@ def registration_3d(threads):
     for k, st in enumerate(stats):
         levels.append({"size": list(st["size"]), "coeff_size": list(st["coeff_size"]),
                        "evaluations": st["evaluations"], "gpu_seconds": round(st["seconds"], 4), "cost": st["cost"],
+                       "gpu_minimize_seconds": round(st["minimize_seconds"], 4),
+                       "gpu_loop_seconds": round(st["loop_seconds"], 4),
                        "gpu_ms_per_evaluation": round(1e3 * st["seconds"] / max(st["evaluations"], 1), 4),
                        "cpu_port_seconds": round(float(lsec[k]), 3) if threads else None,
                        "cpu_port_evaluations": int(lev[k]) if threads else None,

@@ -66,6 +66,9 @@ typedef struct {
     int evaluations;        /* cost + gradient evaluations at the level */
     double cost;            /* best cost (costs + penalty) at the level */
     double seconds;         /* wall time of the level */
+    double minimize_seconds; /* of seconds: in the minimisations (device optimizer set-up and graph capture
+                                included) */
+    double loop_seconds;    /* of those: the device optimizer's iteration loop (0 with a host minimizer) */
 } mia_host_level_stats;
 
 /* Register src to ref (dim 2 or 3, size x,y[,z], float images) as the

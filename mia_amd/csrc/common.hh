@@ -15,11 +15,11 @@ int divcurl_grid(const mia_hip_divcurl *p, uint32_t cs[3]);
 
 // Enqueue on stream s: the penalty of the double parameters `params` (one
 // instance's, rounded to float as the transformation stores them), its
-// weighted value added to *cost and -weight times its gradient added to
-// grad.  Uses the penalty object's partial-sum buffer: evaluations of one
-// penalty object must not overlap.
-int divcurl_enqueue_add(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *grad,
-                        double *cost);
+// unweighted value into *value and -weight times its gradient into wgrad
+// (each product rounded on its own).  Uses the penalty object's partial-sum
+// buffer: evaluations of one penalty object must not overlap.
+int divcurl_enqueue(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *wgrad,
+                    double *value);
 } // namespace mia_hip_rt
 
 #define HIP_TRY(expr)                                                                                   \

@@ -151,6 +151,28 @@ struct SsdIndex {
     int zoff, zmode;
 };
 
+// Spline penalties evaluated beside the image cost (on another stream of the
+// same evaluation): per instance whether it has one (on[i]), its weight, the
+// unweighted value of its quadratic form and, per parameter, its weighted
+// gradient -w dP/dc.  The finalize and the reduce add them to the image
+// cost's value and gradient as the reference's full cost adds a penalty's:
+// each product rounded, then the sum (mia/3d/splinetransformpenalty.cc:52-71).
+// A null `on` means no penalties.
+struct PenaltyIn {
+    const int *on;
+    const double *w, *value, *grad;
+};
+
+// grad = sc * g (+ the instance's penalty gradient element e)
+__device__ __forceinline__ double grad_with_penalty(double sc, double g, const PenaltyIn &pen, int i, long long e)
+{
+#pragma clang fp contract(off)
+    double v = sc * g;
+    if (pen.on && pen.on[i])
+        v = v + pen.grad[e];
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -2022,7 +2044,7 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
                                                   const double *__restrict__ ssd_part,
                                                   const unsigned *__restrict__ cnt_part, int norm, float thr,
                                                   double weight, double *__restrict__ costs,
-                                                  double *__restrict__ gscale)
+                                                  double *__restrict__ gscale, const PenaltyIn &pen)
 {
     __shared__ double red[kBlock / 64];
     __shared__ unsigned redu[kBlock / 64];
@@ -2037,9 +2059,14 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
         unsigned n[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
+            // unconditional loads from a clamped index (a load under a
+            // branch is waited for at the branch's end)
             const int k = k0 + u * kBlock;
-            a[u] = k < end ? ssd_part[k] : 0.0;
-            n[u] = k < end ? cnt_part[k] : 0u;
+            const int kc = min(k, end - 1);
+            const double av = ssd_part[kc];
+            const unsigned nv = cnt_part[kc];
+            a[u] = k < end ? av : 0.0;
+            n[u] = k < end ? nv : 0u;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -2062,7 +2089,13 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
             sc = 0.0;
             cost = (double)FLT_MAX;
         }
-        costs[i] = weight * cost;
+        {
+#pragma clang fp contract(off)
+            double f = weight * cost;
+            if (pen.on && pen.on[i])
+                f = f + pen.w[i] * pen.value[i];
+            costs[i] = f;
+        }
         if (gscale)
             gscale[i] = -sc * weight;
     }
@@ -2071,10 +2104,10 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
 __global__ void __launch_bounds__(kBlock)
     k_finalize(const InstDesc *__restrict__ inst, SsdIndex si, const double *__restrict__ ssd_part,
                const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
-               double *__restrict__ gscale)
+               double *__restrict__ gscale, PenaltyIn pen)
 {
     const int i = blockIdx.x;
-    finalize_instance(inst[i], i, si, ssd_part, cnt_part, norm, thr, weight, costs, gscale);
+    finalize_instance(inst[i], i, si, ssd_part, cnt_part, norm, thr, weight, costs, gscale, pen);
 }
 
 // Per coefficient component: gradient = scale * sum_z wz(z) sum_t
@@ -2089,11 +2122,11 @@ __global__ void __launch_bounds__(kBlock)
              const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const int *__restrict__ cells, SsdIndex si, int nfin, const double *__restrict__ ssd_part,
              const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
-             const double *__restrict__ gscale, double *__restrict__ grad)
+             const double *__restrict__ gscale, double *__restrict__ grad, PenaltyIn pen)
 {
     int b = blockIdx.x;
     if (b < nfin) {
-        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
+        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr, pen);
         return;
     }
     b -= nfin;
@@ -2164,7 +2197,8 @@ __global__ void __launch_bounds__(kBlock)
         part[g4][lane] = g;
         __syncthreads();
         if (g4 == 0 && e3 < 3 * ncoef)
-            grad[d.par_off + e3] = sc * (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+            grad[d.par_off + e3] = grad_with_penalty(
+                sc, ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane], pen, i, d.par_off + e3);
         return;
     } else {
         const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
@@ -2197,7 +2231,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int t = 0; t < 4; ++t)
             g += v[t];
-        grad[d.par_off + e3] = sc * g;
+        grad[d.par_off + e3] = grad_with_penalty(sc, g, pen, i, d.par_off + e3);
     }
 }
 
@@ -2212,11 +2246,12 @@ __global__ void __launch_bounds__(kBlock)
     k_reducez_3d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
                  const float *__restrict__ P, SsdIndex si, int nfin, const double *__restrict__ ssd_part,
                  const unsigned *__restrict__ cnt_part, int norm, float thr, double weight,
-                 double *__restrict__ costs, const double *__restrict__ gscale, double *__restrict__ grad)
+                 double *__restrict__ costs, const double *__restrict__ gscale, double *__restrict__ grad,
+                 PenaltyIn pen)
 {
     int b = blockIdx.x;
     if (b < nfin) {
-        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr);
+        finalize_instance(inst[b], b, si, ssd_part, cnt_part, norm, thr, weight, costs, nullptr, pen);
         return;
     }
     b -= nfin;
@@ -2264,7 +2299,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int q = 0; q < 16; ++q)
         g += (double)v[q];
-    grad[d.par_off + e3] = sc * g;
+    grad[d.par_off + e3] = grad_with_penalty(sc, g, pen, i, d.par_off + e3);
 }
 
 // ---------------------------------------------------------------------------

@@ -211,10 +211,11 @@ constexpr int kDcCoef = kLB / kDcPlanes;
 // bounds, ppmatrix.cc:330-345): thread (i, q) takes the z plane
 // zj = zi - 4 + q, the eight planes' sums are added in ascending order,
 // and the block's coefficients' row sums in a fixed order into part[block].
-// ADD: grad[3i + k] += -w * g_k (the penalty's weighted gradient with the
-// reference's sign, splinetransformpenalty.cc:60-71, added to a gradient
-// already there); otherwise grad = g (unweighted, may be NULL).
-template <typename T, bool ADD>
+// SCALED: grad[3i + k] = -w * g_k (the penalty's weighted gradient with the
+// reference's sign, splinetransformpenalty.cc:60-71, the product rounded on
+// its own, to be added to the image cost's gradient); otherwise grad = g
+// (unweighted, may be NULL).
+template <typename T, bool SCALED>
 __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, const PPCell *__restrict__ cells,
                                                  int nx, int ny, int nz, double w, double *__restrict__ grad,
                                                  double *__restrict__ part)
@@ -283,13 +284,10 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, con
             for (int pl = 1; pl < kDcPlanes; ++pl)
                 t[k] += acc[cl][pl][k];
         }
-        if (ADD) {
-            // the product rounded before the sum, as the host adds the
-            // penalty's gradient vector (no fused multiply-add)
-#pragma clang fp contract(off)
-            grad[3 * i] = grad[3 * i] + -w * t[6];
-            grad[3 * i + 1] = grad[3 * i + 1] + -w * t[7];
-            grad[3 * i + 2] = grad[3 * i + 2] + -w * t[8];
+        if (SCALED) {
+            grad[3 * i] = -w * t[6];
+            grad[3 * i + 1] = -w * t[7];
+            grad[3 * i + 2] = -w * t[8];
         } else if (grad) {
             grad[3 * i] = t[6];
             grad[3 * i + 1] = t[7];
@@ -310,11 +308,8 @@ __global__ void __launch_bounds__(kLB) k_divcurl(const T *__restrict__ coef, con
     }
 }
 
-// Sum of the block partials in a fixed order: *out = sum, or with ADD
-// *out = *out + w * sum (the weighted penalty added to a cost already there).
-template <bool ADD>
-__global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ part, int n, double w,
-                                                   double *__restrict__ out)
+// Sum of the block partials in a fixed order into *out.
+__global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ part, int n, double *__restrict__ out)
 {
     __shared__ double red[kLB / 64];
     double s = 0.0;
@@ -329,12 +324,7 @@ __global__ void __launch_bounds__(kLB) k_sum_parts(const double *__restrict__ pa
         double t = 0.0;
         for (int wv = 0; wv < kLB / 64; ++wv)
             t += red[wv];
-        if (ADD) {
-#pragma clang fp contract(off)
-            *out = *out + w * t;
-        } else {
-            *out = t;
-        }
+        *out = t;
     }
 }
 
@@ -646,8 +636,8 @@ extern "C" int mia_hip_divcurl_evaluate(mia_hip_divcurl *p, const float *coeffs,
                        (const float *)p->d_coef, (const PPCell *)p->d_cells, (int)p->cs[0], (int)p->cs[1],
                        (int)p->cs[2], 0.0, gradient ? p->d_grad : nullptr, p->d_part);
     LAUNCH_CHECK("divcurl");
-    hipLaunchKernelGGL(k_sum_parts<false>, dim3(1), dim3(kLB), 0, p->stream, (const double *)p->d_part, p->nblocks,
-                       0.0, p->d_value);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kLB), 0, p->stream, (const double *)p->d_part, p->nblocks,
+                       p->d_value);
     LAUNCH_CHECK("divcurl sum");
     HIP_TRY(hipMemcpyAsync(value, p->d_value, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     if (gradient)
@@ -666,15 +656,14 @@ int divcurl_grid(const mia_hip_divcurl *p, uint32_t cs[3])
     return MIA_HIP_OK;
 }
 
-int divcurl_enqueue_add(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *grad,
-                        double *cost)
+int divcurl_enqueue(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *wgrad,
+                    double *value)
 {
     hipLaunchKernelGGL((k_divcurl<double, true>), dim3(p->nblocks), dim3(kLB), 0, s, params,
-                       (const PPCell *)p->d_cells, (int)p->cs[0], (int)p->cs[1], (int)p->cs[2], weight, grad,
+                       (const PPCell *)p->d_cells, (int)p->cs[0], (int)p->cs[1], (int)p->cs[2], weight, wgrad,
                        p->d_part);
     LAUNCH_CHECK("divcurl");
-    hipLaunchKernelGGL(k_sum_parts<true>, dim3(1), dim3(kLB), 0, s, (const double *)p->d_part, p->nblocks, weight,
-                       cost);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kLB), 0, s, (const double *)p->d_part, p->nblocks, value);
     LAUNCH_CHECK("divcurl sum");
     return MIA_HIP_OK;
 }

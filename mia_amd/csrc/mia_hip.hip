@@ -176,6 +176,13 @@ struct mia_hip_ctx {
     // spline penalties evaluated with every evaluation, per instance (none: null)
     std::vector<mia_hip_divcurl *> penalty;
     std::vector<double> penalty_weight;
+    // their device inputs of the finalize and reduce (allocated with the
+    // first attached penalty) and the second stream they are evaluated on,
+    // forked from and joined back into the context stream in every evaluation
+    int *d_pen_on = nullptr;
+    double *d_pen_w = nullptr, *d_pen_value = nullptr, *d_pen_grad = nullptr;
+    hipStream_t pen_stream = nullptr;
+    hipEvent_t pen_fork = nullptr, pen_join = nullptr;
 
     bool profile = false;
     struct Ev {
@@ -197,7 +204,8 @@ void free_buffers(mia_hip_ctx *c)
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
                     c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
-                    c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
+                    c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2, c->d_pen_on, c->d_pen_w,
+                    c->d_pen_value, c->d_pen_grad};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -229,6 +237,8 @@ void free_buffers(mia_hip_ctx *c)
     c->d_ts = c->d_cells = nullptr;
     c->d_ssd = c->d_gscale = c->d_params = c->d_grad = c->d_costs = c->d_scratch = nullptr;
     c->d_cnt = nullptr;
+    c->d_pen_on = nullptr;
+    c->d_pen_w = c->d_pen_value = c->d_pen_grad = nullptr;
     c->h_params = c->h_grad = c->h_costs = nullptr;
     c->gexec = nullptr;
     c->graph = nullptr;
@@ -503,6 +513,10 @@ struct EvalPlan {
     const float2 *clamp = nullptr;
     double *ssd = nullptr, *gscale = nullptr;
     unsigned *cnt = nullptr;
+    // penalties added by the finalize and reduce, and the event their
+    // evaluation on another stream records (none: null)
+    PenaltyIn pen{nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t pen_join = nullptr;
 };
 
 EvalPlan plan_of(const mia_hip_ctx *c)
@@ -645,9 +659,13 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
         LAUNCH_CHECK("stage1");
     }
     const bool fused = pl.thr == 0.0f;
+    // the penalties' value and gradient, evaluated on another stream, are
+    // added by the finalize and the reduce
+    if (pl.pen_join)
+        HIP_TRY(hipStreamWaitEvent(s, pl.pen_join, 0));
     if (!fused) {
         launch(s, prof, ST_FINALIZE, k_finalize, dim3(n), 0, pl.inst, si, (const double *)pl.ssd,
-               (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs, pl.gscale);
+               (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs, pl.gscale, pl.pen);
         LAUNCH_CHECK("finalize");
     }
     // with a fused finalize, the first reduce launch does it for every instance
@@ -661,7 +679,7 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
             kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
         launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.red_prefix, (const float *)pl.P, pl.tw, pl.ts,
                pl.cells, si, nfin, (const double *)pl.ssd, (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight,
-               d_costs, (const double *)pl.gscale, d_grad);
+               d_costs, (const double *)pl.gscale, d_grad, pl.pen);
         LAUNCH_CHECK("reduce");
         nfin = 0;
     }
@@ -670,7 +688,7 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
         auto kern = fused ? k_reducez_3d<true> : k_reducez_3d<false>;
         launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.redz_prefix, (const float *)pl.P, si, nfin,
                (const double *)pl.ssd, (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs,
-               (const double *)pl.gscale, d_grad);
+               (const double *)pl.gscale, d_grad, pl.pen);
         LAUNCH_CHECK("reduce");
     }
     return MIA_HIP_OK;
@@ -678,18 +696,30 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
 
 int enqueue_eval(mia_hip_ctx *c, const double *d_params, double *d_grad, double *d_costs, bool timed)
 {
-    int st = enqueue_plan(plan_of(c), c->stream, timed ? c : nullptr, d_params, d_grad, d_costs);
-    if (st)
-        return st;
-    // attached penalties, after the image cost has written cost and gradient
-    for (int i = 0; i < (int)c->penalty.size(); ++i)
-        if (c->penalty[i]) {
-            const InstDesc &d = c->inst[i];
-            if ((st = mia_hip_rt::divcurl_enqueue_add(c->penalty[i], c->stream, d_params + d.par_off,
-                                                      c->penalty_weight[i], d_grad + d.par_off, d_costs + i)))
-                return st;
-        }
-    return MIA_HIP_OK;
+    EvalPlan pl = plan_of(c);
+    bool any = false;
+    for (auto *p : c->penalty)
+        any = any || p;
+    if (any) {
+        // attached penalties on the second stream, beside the image cost:
+        // forked after whatever wrote the parameters, joined before the
+        // finalize and reduce that add them to the cost and gradient
+        HIP_TRY(hipEventRecord(c->pen_fork, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->pen_stream, c->pen_fork, 0));
+        for (int i = 0; i < (int)c->penalty.size(); ++i)
+            if (c->penalty[i]) {
+                const InstDesc &d = c->inst[i];
+                const int st = mia_hip_rt::divcurl_enqueue(c->penalty[i], c->pen_stream, d_params + d.par_off,
+                                                           c->penalty_weight[i], c->d_pen_grad + d.par_off,
+                                                           c->d_pen_value + i);
+                if (st)
+                    return st;
+            }
+        HIP_TRY(hipEventRecord(c->pen_join, c->pen_stream));
+        pl.pen = PenaltyIn{c->d_pen_on, c->d_pen_w, c->d_pen_value, c->d_pen_grad};
+        pl.pen_join = c->pen_join;
+    }
+    return enqueue_plan(pl, c->stream, timed ? c : nullptr, d_params, d_grad, d_costs);
 }
 
 // A captured graph holds the launches of the settings it was captured with.
@@ -862,6 +892,12 @@ int mia_hip_ctx_destroy(mia_hip_ctx *c)
     }
     for (auto e : c->pool)
         (void)hipEventDestroy(e);
+    if (c->pen_stream) {
+        (void)hipStreamSynchronize(c->pen_stream);
+        (void)hipStreamDestroy(c->pen_stream);
+        (void)hipEventDestroy(c->pen_fork);
+        (void)hipEventDestroy(c->pen_join);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MIA_HIP_OK;
@@ -1168,8 +1204,28 @@ int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double wei
         if (!std::isfinite(weight))
             return fail(MIA_HIP_ERR_ARG, "set_penalty: weight must be finite");
     }
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (p && !c->d_pen_on) {
+        int st;
+        if ((st = dev_alloc(&c->d_pen_on, c->n_inst)) || (st = dev_alloc(&c->d_pen_w, c->n_inst)) ||
+            (st = dev_alloc(&c->d_pen_value, c->n_inst)) || (st = dev_alloc(&c->d_pen_grad, c->n_par)))
+            return st;
+    }
+    if (p && !c->pen_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->pen_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->pen_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->pen_join, hipEventDisableTiming));
+    }
     c->penalty[inst] = p;
     c->penalty_weight[inst] = p ? weight : 0.0;
+    if (c->d_pen_on) {
+        std::vector<int> on(c->n_inst);
+        for (int i = 0; i < c->n_inst; ++i)
+            on[i] = c->penalty[i] ? 1 : 0;
+        HIP_TRY(hipMemcpy(c->d_pen_on, on.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->d_pen_w, c->penalty_weight.data(), sizeof(double) * c->n_inst, hipMemcpyHostToDevice));
+    }
     drop_graph(c);
     return MIA_HIP_OK;
 }

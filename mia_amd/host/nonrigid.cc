@@ -947,10 +947,11 @@ bool CNonrigidRegister::minimize_on_device(CSplineTransformation &t, std::vector
     } detach{ctx};
     hip_check(mia_hip_set_penalty(ctx, 0, pen, pen_w), "nonrigid register: penalty");
     std::vector<double> xo(x.size());
-    double cost = 0.0;
+    double cost = 0.0, loop = 0.0;
     int evals = 0;
-    hip_check(mia_hip_optimize(ctx, x.data(), maxeval, memory, step, 10, xo.data(), &cost, &evals, nullptr),
+    hip_check(mia_hip_optimize(ctx, x.data(), maxeval, memory, step, 10, xo.data(), &cost, &evals, &loop),
               "nonrigid register: device L-BFGS");
+    st.loop_seconds += loop;
     if (evals > 0 && std::isfinite(cost))
         x = xo;
     st.evaluations += evals;
@@ -992,14 +993,18 @@ PSplineTransformation CNonrigidRegister::run(const CImage &src_in, const CImage 
         st.size = local;
         CNonrigRegGradientProblem gp(m_costs, *t);
         std::vector<double> x = t->get_parameters();
+        auto m0 = std::chrono::steady_clock::now();
         if (!minimize_on_device(*t, x, st, st.cost))
             st.cost = minimize(gp, x, st);
+        st.minimize_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - m0).count();
         t->set_parameters(x);
         if (t->refine()) {
             gp.reset_counters();
             x = t->get_parameters();
+            m0 = std::chrono::steady_clock::now();
             if (!minimize_on_device(*t, x, st, st.cost))
                 st.cost = minimize(gp, x, st);
+            st.minimize_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - m0).count();
             t->set_parameters(x);
         }
         st.coeff_size = t->get_coeff_size();

@@ -343,6 +343,9 @@ public:
         Size3 size, coeff_size;
         int evaluations = 0;
         double cost = 0.0, seconds = 0.0;
+        // of `seconds`: in the minimisations (with the device optimizer's
+        // set-up and graph capture), and in its iteration loop alone
+        double minimize_seconds = 0.0, loop_seconds = 0.0;
     };
     CNonrigidRegister(CFullCostList &costs, const std::string &minimizer, const CSplineTransformCreator &creator,
                       int mg_levels, int idx = -1);

@@ -66,6 +66,8 @@ extern "C" int mia_host_nonrigid_register(int device, int dim, const uint32_t *s
                 stats[l].evaluations = st[l].evaluations;
                 stats[l].cost = st[l].cost;
                 stats[l].seconds = st[l].seconds;
+                stats[l].minimize_seconds = st[l].minimize_seconds;
+                stats[l].loop_seconds = st[l].loop_seconds;
             }
         }
         if (final_cost) {

@@ -52,7 +52,8 @@ SIGNATURES = {
 
 class LevelStats(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32 * 3), ("coeff_size", ctypes.c_uint32 * 3), ("evaluations", ctypes.c_int),
-                ("cost", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("cost", ctypes.c_double), ("seconds", ctypes.c_double), ("minimize_seconds", ctypes.c_double),
+                ("loop_seconds", ctypes.c_double)]
 
 _lib = None
 
@@ -186,7 +187,8 @@ def nonrigid_register(src, ref, transform="spline:rate=8", cost="image:cost=[ssd
                                         cost.encode(), minimizer.encode(), int(levels), params.ctypes.data, cap,
                                         ctypes.byref(n), cs, ctypes.cast(stats, _vp), int(levels), ctypes.byref(fc)))
     st = [{"size": tuple(s.size), "coeff_size": tuple(s.coeff_size), "evaluations": s.evaluations,
-           "cost": s.cost, "seconds": s.seconds} for s in stats[:levels]]
+           "cost": s.cost, "seconds": s.seconds, "minimize_seconds": s.minimize_seconds,
+           "loop_seconds": s.loop_seconds} for s in stats[:levels]]
     return params[:n.value].copy(), tuple(cs), st, fc.value
 
 
