@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -24,6 +25,39 @@ using mia_hip_rt::fail;
 namespace {
 
 constexpr int kLB = 256;
+
+// Per-device state of the entry points that hold no context, kept until the
+// process ends: one stream (each entry point waits for its own work on it
+// before it returns) and a scratch allocation for the resampling that grows
+// to the largest call so far (used under the mutex).
+struct DeviceCache {
+    std::mutex m;
+    hipStream_t stream = nullptr;
+    void *scratch = nullptr;
+    size_t cap = 0;
+};
+constexpr int kMaxCacheDevices = 64;
+
+DeviceCache *device_cache(int device)
+{
+    static DeviceCache caches[kMaxCacheDevices];
+    return device >= 0 && device < kMaxCacheDevices ? &caches[device] : nullptr;
+}
+
+// the device's shared stream, created on first use (call with the mutex held
+// and the device current)
+hipError_t cache_stream(DeviceCache &c, hipStream_t *s)
+{
+    if (!c.stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            c.stream = nullptr;
+            return e;
+        }
+    }
+    *s = c.stream;
+    return hipSuccess;
+}
 
 __device__ __forceinline__ int mirror_line(int i, int n)
 {
@@ -479,29 +513,32 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
             vmax = std::max(vmax, v[0] * v[1] * v[2]);
         }
     }
+    // the device's shared stream and scratch (input, output, two double
+    // volumes, the line coefficients and the binomial taps, 256-byte aligned)
+    DeviceCache *dc = device_cache(device);
+    if (!dc)
+        return fail(MIA_HIP_ERR_ARG, "scale_image: device index out of range");
+    std::lock_guard<std::mutex> lock(dc->m);
     hipStream_t s = nullptr;
-    HIP_TRY(hipStreamCreate(&s));
-    std::unique_ptr<std::remove_pointer<hipStream_t>::type, hipError_t (*)(hipStream_t)> sg(s, hipStreamDestroy);
-    float *d_in = nullptr, *d_out = nullptr;
-    double *d_a = nullptr, *d_b = nullptr, *d_scr = nullptr, *d_g = nullptr;
-    auto cleanup = [&]() {
-        (void)hipFree(d_in);
-        (void)hipFree(d_out);
-        (void)hipFree(d_a);
-        (void)hipFree(d_b);
-        (void)hipFree(d_scr);
-        (void)hipFree(d_g);
-    };
-    struct Guard {
-        decltype(cleanup) &f;
-        ~Guard() { f(); }
-    } guard{cleanup};
-    HIP_TRY(hipMalloc(&d_in, vin * sizeof(float)));
-    HIP_TRY(hipMalloc(&d_out, vout * sizeof(float)));
-    HIP_TRY(hipMalloc(&d_a, vmax * sizeof(double)));
-    HIP_TRY(hipMalloc(&d_b, vmax * sizeof(double)));
-    HIP_TRY(hipMalloc(&d_scr, vmax * sizeof(double)));
-    HIP_TRY(hipMalloc(&d_g, 256 * sizeof(double)));
+    HIP_TRY(cache_stream(*dc, &s));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_in = al(vin * sizeof(float)), b_out = al(vout * sizeof(float)), b_v = al(vmax * sizeof(double));
+    const size_t need = b_in + b_out + 3 * b_v + al(256 * sizeof(double));
+    if (dc->cap < need) {
+        if (dc->scratch)
+            HIP_TRY(hipFree(dc->scratch));
+        dc->scratch = nullptr;
+        dc->cap = 0;
+        HIP_TRY(hipMalloc(&dc->scratch, need));
+        dc->cap = need;
+    }
+    char *base = (char *)dc->scratch;
+    float *d_in = (float *)base;
+    float *d_out = (float *)(base + b_in);
+    double *d_a = (double *)(base + b_in + b_out);
+    double *d_b = (double *)(base + b_in + b_out + b_v);
+    double *d_scr = (double *)(base + b_in + b_out + 2 * b_v);
+    double *d_g = (double *)(base + b_in + b_out + 3 * b_v);
     HIP_TRY(hipMemcpyAsync(d_in, in, vin * sizeof(float), hipMemcpyHostToDevice, s));
     int cur[3] = {n_in[0], n_in[1], n_in[2]};
     const float *src_f = d_in;
@@ -586,8 +623,7 @@ extern "C" int mia_hip_divcurl_destroy(mia_hip_divcurl *p)
     (void)hipFree(p->d_grad);
     (void)hipFree(p->d_part);
     (void)hipFree(p->d_value);
-    if (p->stream)
-        (void)hipStreamDestroy(p->stream);
+    // (p->stream is the device's shared stream)
     delete p;
     return MIA_HIP_OK;
 }
@@ -613,7 +649,13 @@ extern "C" int mia_hip_divcurl_create(int device, const uint32_t *csize, const f
     p->nblocks = (int)((p->n + kDcCoef - 1) / kDcCoef);
     make_pp_cells(csize, range, wdiv, wcurl, p->cells);
     HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipStreamCreate(&p->stream));
+    {
+        DeviceCache *dc = device_cache(device);
+        if (!dc)
+            return fail(MIA_HIP_ERR_ARG, "divcurl_create: device index out of range");
+        std::lock_guard<std::mutex> lock(dc->m);
+        HIP_TRY(cache_stream(*dc, &p->stream));
+    }
     HIP_TRY(hipMalloc(&p->d_cells, p->cells.size() * sizeof(PPCell)));
     HIP_TRY(hipMalloc(&p->d_coef, 3 * p->n * sizeof(float)));
     HIP_TRY(hipMalloc(&p->d_grad, 3 * p->n * sizeof(double)));

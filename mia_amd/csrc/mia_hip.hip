@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <cmath>
 #include <cstdio>
@@ -184,6 +185,13 @@ struct mia_hip_ctx {
     hipStream_t pen_stream = nullptr;
     hipEvent_t pen_fork = nullptr, pen_join = nullptr;
 
+    // device blocks this context allocated (bytes), and those of them free
+    // for reuse: a set-up re-allocates from the blocks the previous set-up
+    // released, and returns the rest to HIP (hipMalloc and hipFree cost
+    // tens to hundreds of microseconds each)
+    std::unordered_map<void *, size_t> blocks;
+    std::multimap<size_t, void *> idle;
+
     bool profile = false;
     struct Ev {
         int stage;
@@ -197,6 +205,28 @@ struct mia_hip_ctx {
 
 namespace {
 
+// Return a block of this context to its idle pool.
+void pool_release(mia_hip_ctx *c, void *p)
+{
+    if (!p)
+        return;
+    auto it = c->blocks.find(p);
+    if (it == c->blocks.end())
+        (void)hipFree(p);
+    else
+        c->idle.emplace(it->second, p);
+}
+
+// hipFree the idle blocks.
+void pool_trim(mia_hip_ctx *c)
+{
+    for (auto &kv : c->idle) {
+        c->blocks.erase(kv.second);
+        (void)hipFree(kv.second);
+    }
+    c->idle.clear();
+}
+
 void free_buffers(mia_hip_ctx *c)
 {
     void *ptrs[] = {c->d_inst, c->d_wgroups, c->d_s1_prefix, c->d_red_prefix, c->d_s1z_prefix, c->d_redz_prefix,
@@ -206,9 +236,11 @@ void free_buffers(mia_hip_ctx *c)
                     c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
                     c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2, c->d_pen_on, c->d_pen_w,
                     c->d_pen_value, c->d_pen_grad};
+    // blocks the previous set-up released and this one did not reuse go
+    // back to HIP; this set-up's join the pool (the stream is idle here)
+    pool_trim(c);
     for (void *p : ptrs)
-        if (p)
-            (void)hipFree(p);
+        pool_release(c, p);
     if (c->h_params)
         (void)hipHostFree(c->h_params);
     if (c->h_grad)
@@ -466,12 +498,30 @@ void launch(hipStream_t s, mia_hip_ctx *prof, int stage, K kern, dim3 grid, size
         prof->pending.push_back({stage, a, b});
 }
 
+// A device block of `count` elements for context c: an idle block of the
+// context of at least that size and at most twice it (plus 1 MiB), else a new
+// one (after returning the idle blocks to HIP if it cannot be had).
 template <typename T>
-int dev_alloc(T **p, size_t count)
+int dev_alloc(mia_hip_ctx *c, T **p, size_t count)
 {
-    if (count == 0)
-        count = 1;
-    HIP_TRY(hipMalloc((void **)p, sizeof(T) * count));
+    const size_t bytes = ((std::max<size_t>(count, 1) * sizeof(T)) + 255) & ~(size_t)255;
+    auto it = c->idle.lower_bound(bytes);
+    if (it != c->idle.end() && it->first <= 2 * bytes + (1u << 20)) {
+        *p = (T *)it->second;
+        c->idle.erase(it);
+        return MIA_HIP_OK;
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess && !c->idle.empty()) {
+        (void)hipGetLastError();
+        pool_trim(c);
+        e = hipMalloc(&q, bytes);
+    }
+    if (e != hipSuccess)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->blocks[q] = bytes;
+    *p = (T *)q;
     return MIA_HIP_OK;
 }
 
@@ -886,6 +936,7 @@ int mia_hip_ctx_destroy(mia_hip_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_buffers(c);
+    pool_trim(c);
     for (auto &e : c->pending) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -1084,18 +1135,18 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
             return st;
     }
     int st;
-    if ((st = dev_alloc(&c->d_inst, n_inst)) || (st = dev_alloc(&c->d_wgroups, wg.size())) ||
-        (st = dev_alloc(&c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_red_prefix, n_inst + 1)) ||
-        (st = dev_alloc(&c->d_s1z_prefix, n_inst + 1)) || (st = dev_alloc(&c->d_redz_prefix, n_inst + 1)) ||
-        (st = dev_alloc(&c->d_M, img)) || (st = dev_alloc(&c->d_R, img)) || (st = dev_alloc(&c->d_W, img)) ||
-        (st = dev_alloc(&c->d_P, part)) || (st = dev_alloc(&c->d_clamp, n_inst)) || (st = dev_alloc(&c->d_mm, 1024)) ||
-        (st = dev_alloc(&c->d_tw, tw.size() / 4)) ||
-        (st = dev_alloc(&c->d_ts, ts.size())) || (st = dev_alloc(&c->d_cells, cells.size())) ||
-        (st = dev_alloc(&c->d_ssd, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
-        (st = dev_alloc(&c->d_cnt, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
-        (st = dev_alloc(&c->d_gscale, n_inst)) || (st = dev_alloc(&c->d_params, par)) ||
-        (st = dev_alloc(&c->d_grad, par)) || (st = dev_alloc(&c->d_costs, n_inst)) ||
-        (st = dev_alloc(&c->d_scratch, maxv))) {
+    if ((st = dev_alloc(c, &c->d_inst, n_inst)) || (st = dev_alloc(c, &c->d_wgroups, wg.size())) ||
+        (st = dev_alloc(c, &c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(c, &c->d_red_prefix, n_inst + 1)) ||
+        (st = dev_alloc(c, &c->d_s1z_prefix, n_inst + 1)) || (st = dev_alloc(c, &c->d_redz_prefix, n_inst + 1)) ||
+        (st = dev_alloc(c, &c->d_M, img)) || (st = dev_alloc(c, &c->d_R, img)) || (st = dev_alloc(c, &c->d_W, img)) ||
+        (st = dev_alloc(c, &c->d_P, part)) || (st = dev_alloc(c, &c->d_clamp, n_inst)) || (st = dev_alloc(c, &c->d_mm, 1024)) ||
+        (st = dev_alloc(c, &c->d_tw, tw.size() / 4)) ||
+        (st = dev_alloc(c, &c->d_ts, ts.size())) || (st = dev_alloc(c, &c->d_cells, cells.size())) ||
+        (st = dev_alloc(c, &c->d_ssd, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
+        (st = dev_alloc(c, &c->d_cnt, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
+        (st = dev_alloc(c, &c->d_gscale, n_inst)) || (st = dev_alloc(c, &c->d_params, par)) ||
+        (st = dev_alloc(c, &c->d_grad, par)) || (st = dev_alloc(c, &c->d_costs, n_inst)) ||
+        (st = dev_alloc(c, &c->d_scratch, maxv))) {
         free_buffers(c);
         return st;
     }
@@ -1208,8 +1259,8 @@ int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double wei
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (p && !c->d_pen_on) {
         int st;
-        if ((st = dev_alloc(&c->d_pen_on, c->n_inst)) || (st = dev_alloc(&c->d_pen_w, c->n_inst)) ||
-            (st = dev_alloc(&c->d_pen_value, c->n_inst)) || (st = dev_alloc(&c->d_pen_grad, c->n_par)))
+        if ((st = dev_alloc(c, &c->d_pen_on, c->n_inst)) || (st = dev_alloc(c, &c->d_pen_w, c->n_inst)) ||
+            (st = dev_alloc(c, &c->d_pen_value, c->n_inst)) || (st = dev_alloc(c, &c->d_pen_grad, c->n_par)))
             return st;
     }
     if (p && !c->pen_stream) {
@@ -1413,21 +1464,20 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
         void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone,
                        c->d_ost, c->d_oSY, c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
         for (void *p : old)
-            if (p)
-                (void)hipFree(p);
+            pool_release(c, p);
         if (c->h_done)
             (void)hipHostFree(c->h_done);
         c->h_done = nullptr;
         c->opt_memory = 0;
         const size_t R = (size_t)memory + 1; // ring: memory pairs and the candidate's slot
-        if ((st = dev_alloc(&c->d_oX, c->n_par)) || (st = dev_alloc(&c->d_oG, c->n_par)) ||
-            (st = dev_alloc(&c->d_oD, c->n_par)) || (st = dev_alloc(&c->d_oBX, c->n_par)) ||
-            (st = dev_alloc(&c->d_oS, R * c->n_par)) || (st = dev_alloc(&c->d_oY, R * c->n_par)) ||
-            (st = dev_alloc(&c->d_obest, n)) || (st = dev_alloc(&c->d_odone, n)) || (st = dev_alloc(&c->d_ost, n)) ||
-            (st = dev_alloc(&c->d_oSY, R * R * n)) || (st = dev_alloc(&c->d_oYY, R * R * n)) ||
-            (st = dev_alloc(&c->d_oPART, (size_t)kPart * seg[n])) ||
-            (st = dev_alloc(&c->d_oPART2, (size_t)kPart * seg[n])) || (st = dev_alloc(&c->d_ost2, n)) ||
-            (st = dev_alloc(&c->d_oseg, n + 1)))
+        if ((st = dev_alloc(c, &c->d_oX, c->n_par)) || (st = dev_alloc(c, &c->d_oG, c->n_par)) ||
+            (st = dev_alloc(c, &c->d_oD, c->n_par)) || (st = dev_alloc(c, &c->d_oBX, c->n_par)) ||
+            (st = dev_alloc(c, &c->d_oS, R * c->n_par)) || (st = dev_alloc(c, &c->d_oY, R * c->n_par)) ||
+            (st = dev_alloc(c, &c->d_obest, n)) || (st = dev_alloc(c, &c->d_odone, n)) || (st = dev_alloc(c, &c->d_ost, n)) ||
+            (st = dev_alloc(c, &c->d_oSY, R * R * n)) || (st = dev_alloc(c, &c->d_oYY, R * R * n)) ||
+            (st = dev_alloc(c, &c->d_oPART, (size_t)kPart * seg[n])) ||
+            (st = dev_alloc(c, &c->d_oPART2, (size_t)kPart * seg[n])) || (st = dev_alloc(c, &c->d_ost2, n)) ||
+            (st = dev_alloc(c, &c->d_oseg, n + 1)))
             return st;
         HIP_TRY(hipHostMalloc((void **)&c->h_done, sizeof(int) * n, hipHostMallocDefault));
         c->opt_memory = memory;
