@@ -71,6 +71,15 @@ int mia_hip_ctx_destroy(mia_hip_ctx *ctx);
 int mia_hip_setup(mia_hip_ctx *ctx, int dim, int n_inst, const uint32_t *sizes,
                   const uint32_t *csizes, int degree);
 
+/* New coefficient grids csizes[3*i..] for the instances of the last
+ * mia_hip_setup, their image sizes, dimension and interpolation kernel
+ * unchanged: the same as mia_hip_setup with those sizes, except that the
+ * images already set (the prefiltered moving image, the fixed image, the
+ * clamp range) are kept and need not be set again.  For the refinement of a
+ * spline grid at one level (mia/3d/transform/spline.cc:360-397 followed by
+ * the next minimisation).  Penalties are detached as by mia_hip_setup. */
+int mia_hip_setup_grid(mia_hip_ctx *ctx, const uint32_t *csizes);
+
 /* Upload instance `inst`'s moving (source) and fixed (reference) images from
  * host memory, prefilter the moving image and record its clamp range.
  * Replaces C?DImageFullCost::do_reinit/set_reference + the interpolator

@@ -33,6 +33,7 @@ SIGNATURES = {
     "mia_hip_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "mia_hip_ctx_destroy": (ctypes.c_int, [_vp]),
     "mia_hip_setup": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u32_p, _u32_p, ctypes.c_int]),
+    "mia_hip_setup_grid": (ctypes.c_int, [_vp, _u32_p]),
     "mia_hip_set_images": (ctypes.c_int, [_vp, ctypes.c_int, _flt_p, _flt_p]),
     "mia_hip_set_images_device": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "mia_hip_get_stage1_path": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p]),

@@ -86,6 +86,25 @@ class HipContext:
             self._offsets.append((b.value, a.value))
         self.n_params_total = sum(n for _, n in self._offsets)
 
+    def setup_grid(self, csizes):
+        """New coefficient grids (x, y[, z] per instance) for the instances of
+        the last setup, keeping the images already set."""
+        if self.n_inst and len(csizes) != self.n_inst:
+            raise ValueError("expected %d coefficient grids" % self.n_inst)
+        cs = np.zeros(3 * max(self.n_inst, 1), dtype=np.uint32)
+        for i, c in enumerate(csizes[:self.n_inst]):
+            if len(c) != self.dim:
+                raise ValueError("instance %d: dimension mismatch" % i)
+            cs[3 * i:3 * i + 3] = list(c) + [1] * (3 - self.dim)
+        check(self._L.mia_hip_setup_grid(self._h, cs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        self.csizes = [tuple(c) for c in csizes]
+        self._offsets = []
+        for i in range(self.n_inst):
+            a, b = ctypes.c_size_t(), ctypes.c_size_t()
+            check(self._L.mia_hip_n_params(self._h, i, ctypes.byref(a), ctypes.byref(b)))
+            self._offsets.append((b.value, a.value))
+        self.n_params_total = sum(n for _, n in self._offsets)
+
     def stage1_path(self, inst):
         """1: reference-gradient stage 1 per (z knot cell, y band); 0: per (slice, y band)."""
         v = ctypes.c_int()

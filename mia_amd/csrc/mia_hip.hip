@@ -954,7 +954,34 @@ int mia_hip_ctx_destroy(mia_hip_ctx *c)
     return MIA_HIP_OK;
 }
 
+namespace {
+int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const uint32_t *csizes, int degree,
+               bool keep_images);
+}
+
 int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const uint32_t *csizes, int degree)
+{
+    return setup_impl(c, dim, n_inst, sizes, csizes, degree, false);
+}
+
+int mia_hip_setup_grid(mia_hip_ctx *c, const uint32_t *csizes)
+{
+    if (!c || !csizes)
+        return fail(MIA_HIP_ERR_ARG, "null argument");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    std::vector<uint32_t> sizes(3 * c->n_inst);
+    for (int i = 0; i < c->n_inst; ++i) {
+        sizes[3 * i] = (uint32_t)c->inst[i].nx;
+        sizes[3 * i + 1] = (uint32_t)c->inst[i].ny;
+        sizes[3 * i + 2] = (uint32_t)c->inst[i].nz;
+    }
+    return setup_impl(c, c->dim, c->n_inst, sizes.data(), csizes, c->degree, true);
+}
+
+namespace {
+int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const uint32_t *csizes, int degree,
+               bool keep_images)
 {
     if (!c)
         return fail(MIA_HIP_ERR_ARG, "null context");
@@ -969,6 +996,29 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
                                          std::to_string(dim) + "D");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // a new coefficient grid for the same images keeps the uploaded moving
+    // (prefiltered) and fixed images and their clamp ranges: the image
+    // buffers' layout depends on the image sizes only
+    float *kept_M = nullptr, *kept_R = nullptr;
+    float2 *kept_clamp = nullptr;
+    if (keep_images) {
+        kept_M = c->d_M;
+        kept_R = c->d_R;
+        kept_clamp = c->d_clamp;
+        c->d_M = c->d_R = nullptr;
+        c->d_clamp = nullptr;
+    }
+    struct KeptRelease { // released like the others if the set-up fails
+        mia_hip_ctx *c;
+        float *&M, *&R;
+        float2 *&cl;
+        ~KeptRelease()
+        {
+            pool_release(c, M);
+            pool_release(c, R);
+            pool_release(c, cl);
+        }
+    } kept_guard{c, kept_M, kept_R, kept_clamp};
     free_buffers(c);
     c->dim = dim;
     c->degree = degree;
@@ -978,7 +1028,8 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->red_prefix.assign(n_inst + 1, 0);
     c->s1z_prefix.assign(n_inst + 1, 0);
     c->redz_prefix.assign(n_inst + 1, 0);
-    c->images_set.assign(n_inst, 0);
+    if (!keep_images)
+        c->images_set.assign(n_inst, 0);
     c->penalty.assign(n_inst, nullptr);
     c->penalty_weight.assign(n_inst, 0.0);
     c->inst_variant.assign(n_inst, 0);
@@ -1134,12 +1185,20 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
         if (st)
             return st;
     }
+    if (keep_images) {
+        c->d_M = kept_M;
+        c->d_R = kept_R;
+        c->d_clamp = kept_clamp;
+        kept_M = kept_R = nullptr; // owned by the context again
+        kept_clamp = nullptr;
+    }
     int st;
     if ((st = dev_alloc(c, &c->d_inst, n_inst)) || (st = dev_alloc(c, &c->d_wgroups, wg.size())) ||
         (st = dev_alloc(c, &c->d_s1_prefix, n_inst + 1)) || (st = dev_alloc(c, &c->d_red_prefix, n_inst + 1)) ||
         (st = dev_alloc(c, &c->d_s1z_prefix, n_inst + 1)) || (st = dev_alloc(c, &c->d_redz_prefix, n_inst + 1)) ||
-        (st = dev_alloc(c, &c->d_M, img)) || (st = dev_alloc(c, &c->d_R, img)) || (st = dev_alloc(c, &c->d_W, img)) ||
-        (st = dev_alloc(c, &c->d_P, part)) || (st = dev_alloc(c, &c->d_clamp, n_inst)) || (st = dev_alloc(c, &c->d_mm, 1024)) ||
+        (st = keep_images ? 0 : dev_alloc(c, &c->d_M, img)) || (st = keep_images ? 0 : dev_alloc(c, &c->d_R, img)) ||
+        (st = dev_alloc(c, &c->d_W, img)) || (st = dev_alloc(c, &c->d_P, part)) ||
+        (st = keep_images ? 0 : dev_alloc(c, &c->d_clamp, n_inst)) || (st = dev_alloc(c, &c->d_mm, 1024)) ||
         (st = dev_alloc(c, &c->d_tw, tw.size() / 4)) ||
         (st = dev_alloc(c, &c->d_ts, ts.size())) || (st = dev_alloc(c, &c->d_cells, cells.size())) ||
         (st = dev_alloc(c, &c->d_ssd, c->s1_prefix[n_inst] + c->s1z_prefix[n_inst])) ||
@@ -1170,6 +1229,7 @@ int mia_hip_setup(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, co
     c->n_inst = n_inst;
     return MIA_HIP_OK;
 }
+} // namespace
 
 static int set_images_impl(mia_hip_ctx *c, int inst, const void *moving, const void *fixed, hipMemcpyKind kind)
 {

@@ -721,8 +721,8 @@ void CImageFullCost::reinit()
         throw std::invalid_argument(dim_prefix(m_dim) + "ImageFullCost: image dimension mismatch");
     if (m_src->size != m_ref->size)
         throw std::runtime_error(dim_prefix(m_dim) + "ImageFullCost only works with images of equal size");
-    m_src_scaled = *m_src;
-    m_ref_scaled = *m_ref;
+    m_src_scaled = m_src;
+    m_ref_scaled = m_ref;
     m_current = m_src->size;
     m_ctx_size = Size3{0, 0, 0};
 }
@@ -746,8 +746,8 @@ void CImageFullCost::set_size(Size3 size)
         size.z = 1;
     if (size == m_current)
         return;
-    m_src_scaled = size == m_src->size ? *m_src : scale_image(*m_src, size, m_device);
-    m_ref_scaled = size == m_ref->size ? *m_ref : scale_image(*m_ref, size, m_device);
+    m_src_scaled = size == m_src->size ? m_src : std::make_shared<CImage>(scale_image(*m_src, size, m_device));
+    m_ref_scaled = size == m_ref->size ? m_ref : std::make_shared<CImage>(scale_image(*m_ref, size, m_device));
     m_current = size;
     m_ctx_size = Size3{0, 0, 0};
 }
@@ -766,8 +766,14 @@ void CImageFullCost::prepare(const CSplineTransformation &t) const
     const uint32_t sz[3] = {m_current.x, m_current.y, m_current.z};
     const Size3 cs = t.get_coeff_size();
     const uint32_t c3[3] = {cs.x, cs.y, cs.z};
-    hip_check(mia_hip_setup(m_ctx, m_dim, 1, sz, c3, t.get_image_degree()), "image cost: setup");
-    hip_check(mia_hip_set_images(m_ctx, 0, m_src_scaled.data.data(), m_ref_scaled.data.data()), "image cost: images");
+    if (m_ctx_size == m_current && m_ctx_degree == t.get_image_degree()) {
+        // a refined grid at the same level: the images on the device stay
+        hip_check(mia_hip_setup_grid(m_ctx, c3), "image cost: grid");
+    } else {
+        hip_check(mia_hip_setup(m_ctx, m_dim, 1, sz, c3, t.get_image_degree()), "image cost: setup");
+        hip_check(mia_hip_set_images(m_ctx, 0, m_src_scaled->data.data(), m_ref_scaled->data.data()),
+                  "image cost: images");
+    }
     hip_check(mia_hip_set_cost(m_ctx, m_cost.norm ? 1 : 0, m_cost.autothresh, m_weight, m_grad_mode),
               "image cost: cost");
     m_ctx_size = m_current;

@@ -286,7 +286,8 @@ private:
     int m_grad_mode;
     int m_device;
     PImage m_src, m_ref;
-    CImage m_src_scaled, m_ref_scaled;
+    // the images at the current size (the full-size ones shared, not copied)
+    PImage m_src_scaled, m_ref_scaled;
     Size3 m_current;
     mutable mia_hip_ctx *m_ctx = nullptr;
     mutable Size3 m_ctx_size, m_ctx_csize;

@@ -209,3 +209,34 @@ def test_nonrigid_register_matches_cpu_port():
     for s, c in zip(sd, lcost):
         assert s["cost"] == pytest.approx(c, rel=1e-5)
     assert sd[-1]["cost"] == pytest.approx(cf.value, rel=1e-5)
+
+
+def test_setup_grid_keeps_images():
+    """mia_hip_setup_grid: new coefficient grids for the same images (as a
+    refinement does) give bitwise the cost and gradient of a context set up
+    afresh with those grids and the images set again; 3D and 2D, a batch of
+    two instances; the grid must be set up first."""
+    import mia_amd
+    from mia_amd import synth
+    for dim, shapes, rates in ((3, [(30, 36, 41), (25, 33, 28)], (9.0, 6.0)), (2, [(70, 90), (64, 50)], (12.0, 7.0))):
+        data = [(synth.smoothed_noise(s, 2.0, 80 + k), synth.smoothed_noise(s, 2.0, 90 + k)) for k, s in enumerate(shapes)]
+        cs0 = [tuple(mia_amd.spline_csize(n, rates[0], dim) for n in s[::-1]) for s in shapes]
+        cs1 = [tuple(mia_amd.spline_csize(n, rates[1], dim) for n in s[::-1]) for s in shapes]
+        p1 = np.concatenate([synth.random_spline_params(c, 1.0, 3.0, 95 + k) for k, c in enumerate(cs1)])
+        with mia_amd.HipContext(0) as c:
+            c.setup(dim, shapes, cs0, 3)
+            for i, (m, f) in enumerate(data):
+                c.set_images(i, m, f)
+            c.cost_grad(np.zeros(c.n_params_total))
+            c.setup_grid(cs1)
+            assert c.n_params_total == p1.size
+            ca, ga = c.cost_grad(p1)
+        with mia_amd.HipContext(0) as c:
+            c.setup(dim, shapes, cs1, 3)
+            for i, (m, f) in enumerate(data):
+                c.set_images(i, m, f)
+            cb, gb = c.cost_grad(p1)
+        assert np.array_equal(ca, cb) and np.array_equal(ga, gb), dim
+    with mia_amd.HipContext(0) as c:
+        with pytest.raises(RuntimeError):
+            c.setup_grid([(8, 8, 8)])
