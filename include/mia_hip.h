@@ -246,6 +246,24 @@ int mia_hip_warp_tuning_source(char *buf, int len);
 
 /* ---- per-level work of the multiresolution driver ---------------------- */
 
+/* An image kept in device memory (float, x fastest), so that the levels of a
+ * registration resample it and set it into a context without going through
+ * host memory.  No counterpart in the reference (its images are host data,
+ * mia/core/datapool.hh): the device-resident form of the images the full cost
+ * reads from the data pool (mia/3d/fullcost/image.cc:80-97,111-127). */
+typedef struct mia_hip_image mia_hip_image;
+/* Upload `data` (dim 2 or 3, sizes x, y[, z]) to device `device`; synchronous. */
+int mia_hip_image_create(int device, int dim, const uint32_t *size, const float *data, mia_hip_image **out);
+/* A new device image: `in` resampled to out_size with mia_hip_scale_image's
+ * arithmetic (bitwise the same values); synchronous. */
+int mia_hip_image_scale(const mia_hip_image *in, const uint32_t *out_size, mia_hip_image **out);
+/* Copy the image to host memory (its element count of floats). */
+int mia_hip_image_read(const mia_hip_image *img, float *out);
+int mia_hip_image_destroy(mia_hip_image *img);
+/* mia_hip_set_images_device from two device images of the instance's size
+ * (MIA_HIP_ERR_SIZE otherwise) on the context's device. */
+int mia_hip_set_images_image(mia_hip_ctx *ctx, int inst, const mia_hip_image *moving, const mia_hip_image *fixed);
+
 /* Resample a float image (dim 2 or 3, sizes x,y[,z]) to out_size with the
  * `scale` filter's semantics, interp=bspline:d=3: per axis that changes,
  * downscaling applies a binomial filter of half width round(1/scale), then

@@ -68,6 +68,11 @@ SIGNATURES = {
     "mia_hip_load_warp_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p]),
     "mia_hip_warp_tuning_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "mia_hip_scale_image": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32_p, _flt_p, _u32_p, _flt_p]),
+    "mia_hip_image_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32_p, _flt_p, ctypes.POINTER(_vp)]),
+    "mia_hip_image_scale": (ctypes.c_int, [_vp, _u32_p, ctypes.POINTER(_vp)]),
+    "mia_hip_image_read": (ctypes.c_int, [_vp, _flt_p]),
+    "mia_hip_image_destroy": (ctypes.c_int, [_vp]),
+    "mia_hip_set_images_image": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "mia_hip_divcurl_create": (ctypes.c_int, [ctypes.c_int, _u32_p, _flt_p, ctypes.c_double, ctypes.c_double,
                                               ctypes.POINTER(_vp)]),
     "mia_hip_divcurl_evaluate": (ctypes.c_int, [_vp, _flt_p, _dbl_p, _vp]),

@@ -111,6 +111,10 @@ class HipContext:
         check(self._L.mia_hip_get_stage1_path(self._h, int(inst), ctypes.byref(v)))
         return v.value
 
+    def set_images_image(self, inst, moving, fixed):
+        """Images from two level.DeviceImage objects of the instance's size."""
+        check(self._L.mia_hip_set_images_image(self._h, int(inst), moving.handle(), fixed.handle()))
+
     def set_images(self, inst, moving, fixed):
         moving = np.ascontiguousarray(moving, dtype=np.float32)
         fixed = np.ascontiguousarray(fixed, dtype=np.float32)

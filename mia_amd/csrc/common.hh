@@ -13,6 +13,10 @@ int fail(int code, const std::string &msg);
 // The divcurl penalty object's coefficient grid (x, y, z).
 int divcurl_grid(const mia_hip_divcurl *p, uint32_t cs[3]);
 
+// Instance `inst` of a set-up context: dimension, image size (x, y, z; z = 1
+// in 2D) and the context's device.
+int instance_geometry(const mia_hip_ctx *ctx, int inst, int *dim, uint32_t size[3], int *device);
+
 // Enqueue on stream s: the penalty of the double parameters `params` (one
 // instance's, rounded to float as the transformation stores them), its
 // unweighted value into *value and -weight times its gradient into wgrad

@@ -477,10 +477,14 @@ void make_pp_cells(const uint32_t cs[3], const float range[3], double wdiv, doub
 // C ABI
 // ---------------------------------------------------------------------------
 
-extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size, const float *in,
-                                   const uint32_t *out_size, float *out)
+namespace {
+// The resampling of one image on `device`: input from host memory (h_in) or
+// the device (d_in), output to host memory (h_out) or the device (d_out);
+// synchronous.  Validates the sizes as mia_hip_scale_image documents.
+int scale_impl(int device, int dim, const uint32_t *in_size, const float *h_in, const float *d_in_ext,
+               const uint32_t *out_size, float *h_out, float *d_out_ext)
 {
-    if ((dim != 2 && dim != 3) || !in_size || !out_size || !in || !out)
+    if ((dim != 2 && dim != 3) || !in_size || !out_size || !(h_in || d_in_ext) || !(h_out || d_out_ext))
         return fail(MIA_HIP_ERR_ARG, "scale_image: bad arguments");
     int n_in[3], n_out[3];
     for (int a = 0; a < 3; ++a) {
@@ -495,8 +499,24 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
     HIP_TRY(hipSetDevice(device));
     const size_t vin = (size_t)n_in[0] * n_in[1] * n_in[2];
     const size_t vout = (size_t)n_out[0] * n_out[1] * n_out[2];
-    if (vin == vout && n_in[0] == n_out[0] && n_in[1] == n_out[1] && n_in[2] == n_out[2]) {
-        std::copy(in, in + vin, out);
+    // the device's shared stream and scratch (input, output, two double
+    // volumes, the line coefficients and the binomial taps, 256-byte aligned)
+    DeviceCache *dc = device_cache(device);
+    if (!dc)
+        return fail(MIA_HIP_ERR_ARG, "scale_image: device index out of range");
+    std::lock_guard<std::mutex> lock(dc->m);
+    hipStream_t s = nullptr;
+    HIP_TRY(cache_stream(*dc, &s));
+    if (n_in[0] == n_out[0] && n_in[1] == n_out[1] && n_in[2] == n_out[2]) {
+        const size_t b = vin * sizeof(float);
+        if (h_in && h_out)
+            std::copy(h_in, h_in + vin, h_out);
+        else
+            HIP_TRY(hipMemcpyAsync(h_out ? (void *)h_out : (void *)d_out_ext, h_in ? (const void *)h_in : d_in_ext, b,
+                                   h_in ? hipMemcpyHostToDevice
+                                        : (h_out ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice),
+                                   s));
+        HIP_TRY(hipStreamSynchronize(s));
         return MIA_HIP_OK;
     }
     // passes over the axes that change size, x then y then z (scale.cc:44-101);
@@ -513,16 +533,9 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
             vmax = std::max(vmax, v[0] * v[1] * v[2]);
         }
     }
-    // the device's shared stream and scratch (input, output, two double
-    // volumes, the line coefficients and the binomial taps, 256-byte aligned)
-    DeviceCache *dc = device_cache(device);
-    if (!dc)
-        return fail(MIA_HIP_ERR_ARG, "scale_image: device index out of range");
-    std::lock_guard<std::mutex> lock(dc->m);
-    hipStream_t s = nullptr;
-    HIP_TRY(cache_stream(*dc, &s));
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t b_in = al(vin * sizeof(float)), b_out = al(vout * sizeof(float)), b_v = al(vmax * sizeof(double));
+    const size_t b_in = d_in_ext ? 0 : al(vin * sizeof(float)), b_out = d_out_ext ? 0 : al(vout * sizeof(float));
+    const size_t b_v = al(vmax * sizeof(double));
     const size_t need = b_in + b_out + 3 * b_v + al(256 * sizeof(double));
     if (dc->cap < need) {
         if (dc->scratch)
@@ -533,13 +546,14 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
         dc->cap = need;
     }
     char *base = (char *)dc->scratch;
-    float *d_in = (float *)base;
-    float *d_out = (float *)(base + b_in);
+    const float *d_in = d_in_ext ? d_in_ext : (const float *)base;
+    float *d_out = d_out_ext ? d_out_ext : (float *)(base + b_in);
     double *d_a = (double *)(base + b_in + b_out);
     double *d_b = (double *)(base + b_in + b_out + b_v);
     double *d_scr = (double *)(base + b_in + b_out + 2 * b_v);
     double *d_g = (double *)(base + b_in + b_out + 3 * b_v);
-    HIP_TRY(hipMemcpyAsync(d_in, in, vin * sizeof(float), hipMemcpyHostToDevice, s));
+    if (!d_in_ext)
+        HIP_TRY(hipMemcpyAsync((void *)d_in, h_in, vin * sizeof(float), hipMemcpyHostToDevice, s));
     int cur[3] = {n_in[0], n_in[1], n_in[2]};
     const float *src_f = d_in;
     const double *src_d = nullptr;
@@ -561,8 +575,10 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
             P.strategy = SC_DOWN;
             P.dx = 1.0 / sc;
             P.F = (int)std::floor(1.0 / sc + 0.5);
-            if (P.F < 1 || P.F > 127)
+            if (P.F < 1 || P.F > 127) {
+                (void)hipStreamSynchronize(s);
                 return fail(MIA_HIP_ERR_SIZE, "scale_image: downscale factor out of range");
+            }
             // binomial kernel of 2F+1 taps normalised by 2^(2F) (gauss.cc:25-55)
             const int K = 2 * P.F + 1;
             gauss.assign(K, 0.0);
@@ -574,6 +590,8 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
             for (double &g : gauss)
                 g /= norm;
             HIP_TRY(hipMemcpyAsync(d_g, gauss.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
+            // the taps are read by the pass below before `gauss` changes
+            HIP_TRY(hipStreamSynchronize(s));
         }
         // line grid of this pass
         long long nlines;
@@ -596,9 +614,133 @@ extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size,
         src_f = nullptr;
         src_d = dst_d;
     }
-    HIP_TRY(hipMemcpyAsync(out, d_out, vout * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (h_out)
+        HIP_TRY(hipMemcpyAsync(h_out, d_out, vout * sizeof(float), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return MIA_HIP_OK;
+}
+} // namespace
+
+extern "C" int mia_hip_scale_image(int device, int dim, const uint32_t *in_size, const float *in,
+                                   const uint32_t *out_size, float *out)
+{
+    if (!in || !out)
+        return fail(MIA_HIP_ERR_ARG, "scale_image: bad arguments");
+    return scale_impl(device, dim, in_size, in, nullptr, out_size, out, nullptr);
+}
+
+// An image held in device memory (float, x fastest).
+struct mia_hip_image {
+    int device = 0, dim = 0;
+    uint32_t size[3] = {1, 1, 1};
+    float *d = nullptr;
+    size_t count() const { return (size_t)size[0] * size[1] * size[2]; }
+};
+
+extern "C" int mia_hip_image_destroy(mia_hip_image *img)
+{
+    if (!img)
+        return MIA_HIP_OK;
+    (void)hipSetDevice(img->device);
+    if (img->d)
+        (void)hipFree(img->d);
+    delete img;
+    return MIA_HIP_OK;
+}
+
+namespace {
+int image_alloc(int device, int dim, const uint32_t *size, mia_hip_image **out)
+{
+    if (!out || !size || (dim != 2 && dim != 3))
+        return fail(MIA_HIP_ERR_ARG, "image: bad arguments");
+    *out = nullptr;
+    std::unique_ptr<mia_hip_image, int (*)(mia_hip_image *)> img(new mia_hip_image, mia_hip_image_destroy);
+    img->device = device;
+    img->dim = dim;
+    for (int a = 0; a < dim; ++a) {
+        if (size[a] < 1)
+            return fail(MIA_HIP_ERR_SIZE, "image: empty size");
+        img->size[a] = size[a];
+    }
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipMalloc(&img->d, img->count() * sizeof(float)));
+    *out = img.release();
+    return MIA_HIP_OK;
+}
+} // namespace
+
+extern "C" int mia_hip_image_create(int device, int dim, const uint32_t *size, const float *data, mia_hip_image **out)
+{
+    if (!data)
+        return fail(MIA_HIP_ERR_ARG, "image_create: null data");
+    mia_hip_image *img = nullptr;
+    int st = image_alloc(device, dim, size, &img);
+    if (st)
+        return st;
+    DeviceCache *dc = device_cache(device);
+    if (!dc) {
+        mia_hip_image_destroy(img);
+        return fail(MIA_HIP_ERR_ARG, "image_create: device index out of range");
+    }
+    {
+        std::lock_guard<std::mutex> lock(dc->m);
+        hipStream_t s = nullptr;
+        hipError_t e = cache_stream(*dc, &s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(img->d, data, img->count() * sizeof(float), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            mia_hip_image_destroy(img);
+            return fail(MIA_HIP_ERR_DEVICE, std::string("image_create: ") + hipGetErrorString(e));
+        }
+    }
+    *out = img;
+    return MIA_HIP_OK;
+}
+
+extern "C" int mia_hip_image_scale(const mia_hip_image *in, const uint32_t *out_size, mia_hip_image **out)
+{
+    if (!in || !out_size || !out)
+        return fail(MIA_HIP_ERR_ARG, "image_scale: null argument");
+    mia_hip_image *img = nullptr;
+    int st = image_alloc(in->device, in->dim, out_size, &img);
+    if (st)
+        return st;
+    if ((st = scale_impl(in->device, in->dim, in->size, nullptr, in->d, out_size, nullptr, img->d))) {
+        mia_hip_image_destroy(img);
+        return st;
+    }
+    *out = img;
+    return MIA_HIP_OK;
+}
+
+extern "C" int mia_hip_image_read(const mia_hip_image *img, float *out)
+{
+    if (!img || !out)
+        return fail(MIA_HIP_ERR_ARG, "image_read: null argument");
+    HIP_TRY(hipSetDevice(img->device));
+    HIP_TRY(hipMemcpy(out, img->d, img->count() * sizeof(float), hipMemcpyDeviceToHost));
+    return MIA_HIP_OK;
+}
+
+extern "C" int mia_hip_set_images_image(mia_hip_ctx *ctx, int inst, const mia_hip_image *moving,
+                                        const mia_hip_image *fixed)
+{
+    if (!ctx || !moving || !fixed)
+        return fail(MIA_HIP_ERR_ARG, "set_images_image: null argument");
+    uint32_t size[3];
+    int dim = 0, device = -1;
+    int st = mia_hip_rt::instance_geometry(ctx, inst, &dim, size, &device);
+    if (st)
+        return st;
+    for (const mia_hip_image *img : {moving, fixed}) {
+        if (img->device != device)
+            return fail(MIA_HIP_ERR_ARG, "set_images_image: image on another device");
+        if (img->dim != dim || img->size[0] != size[0] || img->size[1] != size[1] || img->size[2] != size[2])
+            return fail(MIA_HIP_ERR_SIZE, "set_images_image: image size differs from the instance's");
+    }
+    return mia_hip_set_images_device(ctx, inst, moving->d, fixed->d);
 }
 
 struct mia_hip_divcurl {

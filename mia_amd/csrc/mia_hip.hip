@@ -885,6 +885,25 @@ int collect_profile(mia_hip_ctx *c)
 
 } // namespace
 
+namespace mia_hip_rt {
+int instance_geometry(const mia_hip_ctx *c, int inst, int *dim, uint32_t size[3], int *device)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (c->n_inst == 0)
+        return fail(MIA_HIP_ERR_STATE, "mia_hip_setup has not been called");
+    if (inst < 0 || inst >= c->n_inst)
+        return fail(MIA_HIP_ERR_ARG, "instance index out of range");
+    const InstDesc &d = c->inst[inst];
+    *dim = c->dim;
+    size[0] = (uint32_t)d.nx;
+    size[1] = (uint32_t)d.ny;
+    size[2] = (uint32_t)d.nz;
+    *device = c->device;
+    return MIA_HIP_OK;
+}
+} // namespace mia_hip_rt
+
 extern "C" {
 
 const char *mia_hip_last_error(void)
@@ -1340,6 +1359,7 @@ int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double wei
     drop_graph(c);
     return MIA_HIP_OK;
 }
+
 
 int mia_hip_get_stage1_path(mia_hip_ctx *c, int inst, int *path)
 {

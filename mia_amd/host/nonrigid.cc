@@ -721,8 +721,10 @@ void CImageFullCost::reinit()
         throw std::invalid_argument(dim_prefix(m_dim) + "ImageFullCost: image dimension mismatch");
     if (m_src->size != m_ref->size)
         throw std::runtime_error(dim_prefix(m_dim) + "ImageFullCost only works with images of equal size");
-    m_src_scaled = m_src;
-    m_ref_scaled = m_ref;
+    m_dsrc.reset();
+    m_dref.reset();
+    m_dsrc_level.reset();
+    m_dref_level.reset();
     m_current = m_src->size;
     m_ctx_size = Size3{0, 0, 0};
 }
@@ -746,10 +748,39 @@ void CImageFullCost::set_size(Size3 size)
         size.z = 1;
     if (size == m_current)
         return;
-    m_src_scaled = size == m_src->size ? m_src : std::make_shared<CImage>(scale_image(*m_src, size, m_device));
-    m_ref_scaled = size == m_ref->size ? m_ref : std::make_shared<CImage>(scale_image(*m_ref, size, m_device));
+    m_dsrc_level.reset();
+    m_dref_level.reset();
     m_current = size;
     m_ctx_size = Size3{0, 0, 0};
+}
+
+void CImageFullCost::level_images() const
+{
+    if (m_dsrc_level && m_dref_level)
+        return;
+    auto upload = [&](const CImage &img) {
+        const uint32_t sz[3] = {img.size.x, img.size.y, img.size.z};
+        mia_hip_image *p = nullptr;
+        hip_check(mia_hip_image_create(m_device, img.dim, sz, img.data.data(), &p), "image cost: upload");
+        return PDeviceImage(p, mia_hip_image_destroy);
+    };
+    if (!m_dsrc || !m_dref) {
+        m_dsrc = upload(*m_src);
+        m_dref = upload(*m_ref);
+    }
+    if (m_current == m_src->size) {
+        m_dsrc_level = m_dsrc;
+        m_dref_level = m_dref;
+        return;
+    }
+    auto scale = [&](const PDeviceImage &full) {
+        const uint32_t sz[3] = {m_current.x, m_current.y, m_dim == 3 ? m_current.z : 1u};
+        mia_hip_image *p = nullptr;
+        hip_check(mia_hip_image_scale(full.get(), sz, &p), "image cost: scale");
+        return PDeviceImage(p, mia_hip_image_destroy);
+    };
+    m_dsrc_level = scale(m_dsrc);
+    m_dref_level = scale(m_dref);
 }
 
 void CImageFullCost::prepare(const CSplineTransformation &t) const
@@ -770,9 +801,9 @@ void CImageFullCost::prepare(const CSplineTransformation &t) const
         // a refined grid at the same level: the images on the device stay
         hip_check(mia_hip_setup_grid(m_ctx, c3), "image cost: grid");
     } else {
+        level_images();
         hip_check(mia_hip_setup(m_ctx, m_dim, 1, sz, c3, t.get_image_degree()), "image cost: setup");
-        hip_check(mia_hip_set_images(m_ctx, 0, m_src_scaled->data.data(), m_ref_scaled->data.data()),
-                  "image cost: images");
+        hip_check(mia_hip_set_images_image(m_ctx, 0, m_dsrc_level.get(), m_dref_level.get()), "image cost: images");
     }
     hip_check(mia_hip_set_cost(m_ctx, m_cost.norm ? 1 : 0, m_cost.autothresh, m_weight, m_grad_mode),
               "image cost: cost");

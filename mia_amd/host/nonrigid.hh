@@ -286,8 +286,12 @@ private:
     int m_grad_mode;
     int m_device;
     PImage m_src, m_ref;
-    // the images at the current size (the full-size ones shared, not copied)
-    PImage m_src_scaled, m_ref_scaled;
+    // the images on the device: full size (uploaded once per reinit) and at
+    // the current size (resampled there from the full-size ones, or shared
+    // with them), made when a context first needs them
+    using PDeviceImage = std::shared_ptr<mia_hip_image>;
+    void level_images() const;
+    mutable PDeviceImage m_dsrc, m_dref, m_dsrc_level, m_dref_level;
     Size3 m_current;
     mutable mia_hip_ctx *m_ctx = nullptr;
     mutable Size3 m_ctx_size, m_ctx_csize;

@@ -30,6 +30,54 @@ def scale_image(img, out_shape, device=0):
     return out
 
 
+class DeviceImage:
+    """A float image (numpy (z,y,x) or (y,x)) uploaded to the device
+    (mia_hip_image): resampled there and set into contexts without host
+    copies."""
+
+    def __init__(self, img=None, device=0, _handle=None, _shape=None):
+        self._L = _lib.load()
+        if _handle is not None:
+            self._h, self.shape = _handle, tuple(_shape)
+            return
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        if img.ndim not in (2, 3):
+            raise ValueError("DeviceImage: 2D or 3D image expected")
+        a, ap = _u32(list(img.shape[::-1]) + [1] * (3 - img.ndim))
+        h = ctypes.c_void_p()
+        check(self._L.mia_hip_image_create(int(device), img.ndim, ap, img.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           ctypes.byref(h)))
+        self._h, self.shape = h, img.shape
+
+    def scale(self, out_shape):
+        """A new device image resampled to out_shape (mia_hip_scale_image's values)."""
+        if len(out_shape) != len(self.shape):
+            raise ValueError("DeviceImage.scale: dimension mismatch")
+        b, bp = _u32(list(out_shape[::-1]) + [1] * (3 - len(out_shape)))
+        h = ctypes.c_void_p()
+        check(self._L.mia_hip_image_scale(self._h, bp, ctypes.byref(h)))
+        return DeviceImage(_handle=h, _shape=out_shape)
+
+    def read(self):
+        out = np.zeros(self.shape, dtype=np.float32)
+        check(self._L.mia_hip_image_read(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            self._L.mia_hip_image_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DivcurlPenalty:
     """The divcurl form c^T P c over a coefficient grid csize (x,y,z) for an
     image of extent rng (x,y,z) (mia/3d/ppmatrix.cc); evaluate() returns the

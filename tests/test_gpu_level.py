@@ -240,3 +240,34 @@ def test_setup_grid_keeps_images():
     with mia_amd.HipContext(0) as c:
         with pytest.raises(RuntimeError):
             c.setup_grid([(8, 8, 8)])
+
+
+def test_device_images_match_host_path():
+    """Device-resident images (mia_hip_image): an upload reads back bitwise;
+    resampling on the device gives bitwise mia_hip_scale_image's values
+    (down and up, 3D and 2D); a context set from device images evaluates
+    bitwise as one set from host images; size mismatches are errors."""
+    import mia_amd
+    from mia_amd import level, synth
+    for shape, out in (((40, 48, 56), (20, 24, 28)), ((33, 30, 27), (40, 41, 29)), ((60, 70), (30, 35))):
+        img = synth.smoothed_noise(shape, 2.0, 7).astype(np.float32)
+        d = level.DeviceImage(img)
+        assert np.array_equal(d.read(), img)
+        ds = d.scale(out)
+        assert np.array_equal(ds.read(), level.scale_image(img, out))
+    shape = (30, 36, 41)
+    mov, fix = synth.smoothed_noise(shape, 2.0, 11), synth.smoothed_noise(shape, 2.0, 12)
+    cs = tuple(mia_amd.spline_csize(n, 8.0, 3) for n in shape[::-1])
+    p = synth.random_spline_params(cs, 1.0, 3.0, 13)
+    dm, df = level.DeviceImage(mov), level.DeviceImage(fix)
+    with mia_amd.HipContext(0) as c:
+        c.setup(3, [shape], [cs], 3)
+        c.set_images_image(0, dm, df)
+        ca, ga = c.cost_grad(p)
+        with pytest.raises(RuntimeError):
+            c.set_images_image(0, dm.scale((15, 18, 20)), df)
+    with mia_amd.HipContext(0) as c:
+        c.setup(3, [shape], [cs], 3)
+        c.set_images(0, mov, fix)
+        cb, gb = c.cost_grad(p)
+    assert np.array_equal(ca, cb) and np.array_equal(ga, gb)
