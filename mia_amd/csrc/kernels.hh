@@ -761,15 +761,20 @@ __global__ void __launch_bounds__(kBlock) k_minmax_partial(const float *__restri
     }
 }
 
-// clamp range [min(0, min), max] (mia/3d/interpolator.cxx:108-111)
-__global__ void k_minmax_final(const float2 *__restrict__ part, int n, float2 *__restrict__ out)
+// clamp range [min(0, min), max] (mia/3d/interpolator.cxx:108-111), one
+// wave over the block partials (min and max are exact in any order)
+__global__ void __launch_bounds__(64) k_minmax_final(const float2 *__restrict__ part, int n, float2 *__restrict__ out)
 {
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int k = threadIdx.x; k < n; k += 64) {
+        const float2 v = part[k];
+        mn = fminf(mn, v.x);
+        mx = fmaxf(mx, v.y);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
     if (threadIdx.x == 0) {
-        float2 r = part[0];
-        for (int k = 1; k < n; ++k) {
-            r.x = fminf(r.x, part[k].x);
-            r.y = fmaxf(r.y, part[k].y);
-        }
+        float2 r = make_float2(mn, mx);
         if (0.0f < r.x)
             r.x = 0.0f;
         *out = r;
@@ -802,22 +807,50 @@ __global__ void __launch_bounds__(kBlock) k_prefilter_lines(float *__restrict__ 
     double z2n = pow(p, (double)(n - 1));
     double sum = lambda * (double)img[base] + z2n * lambda * (double)img[base + (long long)(n - 1) * stride];
     z2n *= z2n * iz;
-    for (int k = 1; k < n - 1; ++k) {
-        sum += (zn + z2n) * lambda * (double)img[base + (long long)k * stride];
-        zn *= p;
-        z2n *= iz;
+    // each walk loads kU elements (clamped indices, no branch) before it uses
+    // them, then runs the recursion over them in order: with one wave per
+    // SIMD a load at a time would wait out the memory latency every element
+    constexpr int kU = 8;
+    for (int k0 = 1; k0 < n - 1; k0 += kU) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            v[u] = (double)img[base + (long long)min(k0 + u, n - 2) * stride];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (k0 + u < n - 1) {
+                sum += (zn + z2n) * lambda * v[u];
+                zn *= p;
+                z2n *= iz;
+            }
     }
     double c = sum / (1.0 - zn * zn);
     scratch[l] = c;
-    for (int k = 1; k < n; ++k) {
-        c = lambda * (double)img[base + (long long)k * stride] + p * c;
-        scratch[(long long)k * nlines + l] = c;
+    for (int k0 = 1; k0 < n; k0 += kU) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            v[u] = (double)img[base + (long long)min(k0 + u, n - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (k0 + u < n) {
+                c = lambda * v[u] + p * c;
+                scratch[(long long)(k0 + u) * nlines + l] = c;
+            }
     }
     double a = (p / (p * p - 1.0)) * (p * scratch[(long long)(n - 2) * nlines + l] + c);
     img[base + (long long)(n - 1) * stride] = (float)a;
-    for (int k = n - 2; k >= 0; --k) {
-        a = p * (a - scratch[(long long)k * nlines + l]);
-        img[base + (long long)k * stride] = (float)a;
+    for (int k0 = n - 2; k0 >= 0; k0 -= kU) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            v[u] = scratch[(long long)max(k0 - u, 0) * nlines + l];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (k0 - u >= 0) {
+                a = p * (a - v[u]);
+                img[base + (long long)(k0 - u) * stride] = (float)a;
+            }
     }
 }
 
