@@ -312,6 +312,26 @@ __device__ __forceinline__ void axis_index(int s, int n, int *idx)
     }
 }
 
+// kweights' scaled weights (degree 3; degree 1 as kweights), the floor of p
+// returned as a float (an exact integer: positions are clamped to +-1e6)
+template <int DEG>
+__device__ __forceinline__ float kweights_sf(float p, float *w)
+{
+    const float flf = fastfloor16f(p);
+    const float f = p - flf;
+    if constexpr (DEG == 3) {
+        const float fm1 = 1.0f - f;
+        w[3] = f * f * f;
+        w[0] = fm1 * fm1 * fm1;
+        w[2] = fmaf(6.0f, f, fmaf(-2.0f, w[3], w[0]));
+        w[1] = fmaf(6.0f, fm1, fmaf(-2.0f, w[0], w[3]));
+    } else {
+        w[0] = 1.0f - f;
+        w[1] = f;
+    }
+    return flf;
+}
+
 // v clamped to [vmin, vmax] (vmin <= vmax) as one median of three (values
 // interpolated from finite coefficients are never NaN).
 __device__ __forceinline__ float clamp_val(float v, float vmin, float vmax)
@@ -1269,9 +1289,10 @@ __global__ void __launch_bounds__(kBlock)
     };
     if (use_lds) {
         // phase 4: the taps at one base address plus compile-time offsets;
-        // the first tap's byte offset in the box is
-        // 4 (s2 PX PY + s1 PX + s0) - kbox, kbox from the box origin
-        const int kbox = 4 * ((rz0 * TL::PY + ry0) * TL::PX + rx0);
+        // the first tap's box coordinates (a, b, c) are the floors less the
+        // box origin (plus LO): small integers, exact in float, and so is
+        // the element offset c PX PY + b PX + a (below the box size, < 2^24)
+        const float ox = (float)(rx0 + LO), oy = (float)(ry0 + LO), oz = (float)(rz0 + LO);
         // the clamp bounds in vector registers once (not copied per voxel)
         float vlo = cl.x, vhi = cl.y;
         asm("" : "+v"(vlo), "+v"(vhi));
@@ -1279,12 +1300,11 @@ __global__ void __launch_bounds__(kBlock)
         for (int zl = 0; zl < TL::TZ; ++zl) {
             if (zl < nzl) {
                 float wa[NS], wb[NS], wc[NS];
-                const int s0 = kweights<DEG, false, true>(px[zl], wa, nullptr);
-                const int s1 = kweights<DEG, false, true>(py[zl], wb, nullptr);
-                const int s2 = kweights<DEG, false, true>(pz[zl], wc, nullptr);
-                // (24-bit products: the indices are within the image's size)
+                const float a = kweights_sf<DEG>(px[zl], wa) - ox;
+                const float b = kweights_sf<DEG>(py[zl], wb) - oy;
+                const float c = kweights_sf<DEG>(pz[zl], wc) - oz;
                 const unsigned o0 =
-                    (unsigned)(__mul24(s2, 4 * TL::PY * TL::PX) + __mul24(s1, 4 * TL::PX) + s0 * 4 - kbox);
+                    (unsigned)fmaf(c, (float)(TL::PY * TL::PX), fmaf(b, (float)TL::PX, a)) << 2;
                 const float v = box_contract<DEG, TL::PX, TL::PY>(region, o0, wa, wb, wc) * kWScale3<DEG>;
                 store(zl, clamp_val(v, vlo, vhi));
             }
