@@ -363,10 +363,19 @@ def registration_3d(threads):
         c.set_images(0, fix, fix)
         mov = c.warp(0, p)
     transform = "spline:rate=8,penalty=[divcurl:weight=0.01]"
-    t0 = time.perf_counter()
-    params, grid, stats, final = series.nonrigid_register(mov, fix, transform, "image:cost=[ssd]",
-                                                          "lbfgs:maxiter=200", levels=3)
-    wall = time.perf_counter() - t0
+    # run twice: the first call in the process also pays the HIP runtime's
+    # first-use costs (queues, staging buffers, kernel objects); the second is
+    # the steady state a caller registering volume after volume sees
+    walls = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        params, grid, stats, final = series.nonrigid_register(mov, fix, transform, "image:cost=[ssd]",
+                                                              "lbfgs:maxiter=200", levels=3)
+        walls.append(time.perf_counter() - t0)
+        if len(walls) == 2:
+            assert final == first_final, "registration is not reproducible"
+        first_final = final
+    wall = walls[1]
     a, b = series.normalize_pair(mov, fix)
     c_id = 0.5 * float(np.sum((a.astype(np.float64) - b) ** 2))
     log("registration 3D: %.2f s, final cost %.6g (identity %.6g)" % (wall, final, c_id))
@@ -398,7 +407,8 @@ def registration_3d(threads):
                        "cpu_port_evaluations": int(lev[k]) if threads else None,
                        "cpu_port_cost": float(lcost[k]) if threads else None})
     return {"workload": "3D 192x192x160, 3 levels, spline rate 8 + divcurl 0.01, ssd, lbfgs maxiter=200",
-            "seconds": round(wall, 3), "final_cost": final, "identity_cost": c_id,
+            "seconds": round(wall, 4), "seconds_first_call": round(walls[0], 4), "final_cost": final,
+            "identity_cost": c_id,
             "coeff_size": list(grid), "levels": levels,
             "cpu_port": {"seconds": round(cpu_sec, 3), "threads": threads, "kind": "port",
                          "final_cost": cfin.value, "coeff_size": [int(v) for v in ccs],

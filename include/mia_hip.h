@@ -148,9 +148,10 @@ int mia_hip_graph_run(mia_hip_ctx *ctx);
  * lockstep: each iteration is the batched evaluation followed by one
  * optimizer step per instance (one thread block each for instances of up to
  * 2048 parameters; larger ones in segments of 512 parameters over three
- * launches, so that their pair history streams through many CUs), with
- * `iters_per_graph` iterations captured in one HIP graph; the host only
- * replays the graph and checks the instances' done flags.  Same algorithm
+ * launches, so that their pair history streams through many CUs), queued in
+ * chunks of `iters_per_graph` iterations, each chunk ending with a copy of
+ * the instances' done flags that the host checks (see
+ * mia_hip_set_optimize_launch for how a chunk is queued).  Same algorithm
  * as the host `lbfgs` minimizer: memory `memory` (<= 16), `maxeval`
  * evaluations per instance (NLopt's maxeval, which MIA's `maxiter` maps to,
  * addons/nlopt/nlopt.cc:237), first step moving no parameter by more than
@@ -160,11 +161,27 @@ int mia_hip_graph_run(mia_hip_ctx *ctx);
  * with mia_hip_set_penalty are part of every evaluation.  x0: starting
  * parameters of all instances (host); outputs (host, any may be NULL): best
  * parameters, best costs, evaluations per instance, wall time of the
- * iteration loop in seconds (graph capture excluded).  Replaces the
+ * iteration loop in seconds (graph capture and instantiation excluded).
+ * Replaces the
  * minimizer loop of TNonrigidRegister::run for the level
  * (mia/template/nonrigidregister.cxx:261-269). */
 int mia_hip_optimize(mia_hip_ctx *ctx, const double *x0, int maxeval, int memory, double first_step,
                      int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds);
+
+/* How mia_hip_optimize queues its chunks of iterations.
+ * MIA_HIP_LAUNCH_STREAM (the default): the kernels of each chunk are launched
+ * on the context's stream, the next chunk queued while the host waits for the
+ * previous one's done flags (only chunks that start before the maxeval-th
+ * iteration are queued ahead, so no evaluation is spent past maxeval).
+ * MIA_HIP_LAUNCH_GRAPH: one chunk captured as a HIP graph, instantiated once
+ * and launched until every instance is done.  Both run the same kernels in
+ * the same order and give bitwise the same results; on ROCm a graph launch
+ * submits its nodes one by one from the host, so it costs more host time per
+ * kernel than direct launches.  No counterpart in the reference (its
+ * minimizer runs on the host). */
+#define MIA_HIP_LAUNCH_STREAM 0
+#define MIA_HIP_LAUNCH_GRAPH 1
+int mia_hip_set_optimize_launch(mia_hip_ctx *ctx, int mode);
 
 /* Wait for all work on the context stream. */
 int mia_hip_synchronize(mia_hip_ctx *ctx);

@@ -50,6 +50,7 @@ SIGNATURES = {
     "mia_hip_graph_run": (ctypes.c_int, [_vp]),
     "mia_hip_optimize": (ctypes.c_int, [_vp, _dbl_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         _vp, _vp, _vp, _dbl_p]),
+    "mia_hip_set_optimize_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_synchronize": (ctypes.c_int, [_vp]),
     "mia_hip_get_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "mia_hip_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -207,6 +207,13 @@ class HipContext:
                                        ctypes.byref(sec)))
         return x, costs, evals, sec.value
 
+    def set_optimize_launch(self, mode):
+        """How optimize() queues its chunks: "stream" (direct launches, the default) or "graph"."""
+        modes = {"stream": 0, "graph": 1}
+        if mode not in modes:
+            raise ValueError("optimize launch mode must be 'stream' or 'graph', got %r" % (mode,))
+        check(self._L.mia_hip_set_optimize_launch(self._h, modes[mode]))
+
     # -- misc ---------------------------------------------------------------------
     def synchronize(self):
         check(self._L.mia_hip_synchronize(self._h))

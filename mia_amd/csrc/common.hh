@@ -24,6 +24,16 @@ int instance_geometry(const mia_hip_ctx *ctx, int inst, int *dim, uint32_t size[
 // buffer: evaluations of one penalty object must not overlap.
 int divcurl_enqueue(mia_hip_divcurl *p, hipStream_t s, const double *params, double weight, double *wgrad,
                     double *value);
+
+// A non-blocking stream of `device` (the current device): one that an earlier
+// user gave back, else a new one.  Creating a stream can take milliseconds
+// (the runtime may set up a hardware queue), and a registration creates and
+// destroys contexts level by level.
+hipError_t stream_acquire(int device, hipStream_t *s);
+
+// Give a stream back: drained, then kept for stream_acquire (up to eight per
+// device) or destroyed.
+void stream_release(int device, hipStream_t s);
 } // namespace mia_hip_rt
 
 #define HIP_TRY(expr)                                                                                   \

@@ -44,12 +44,15 @@ DeviceCache *device_cache(int device)
     return device >= 0 && device < kMaxCacheDevices ? &caches[device] : nullptr;
 }
 
-// the device's shared stream, created on first use (call with the mutex held
+// the device's shared stream, taken on first use (call with the mutex held
 // and the device current)
 hipError_t cache_stream(DeviceCache &c, hipStream_t *s)
 {
     if (!c.stream) {
-        const hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+        int device = 0;
+        hipError_t e = hipGetDevice(&device);
+        if (e == hipSuccess)
+            e = mia_hip_rt::stream_acquire(device, &c.stream);
         if (e != hipSuccess) {
             c.stream = nullptr;
             return e;

@@ -40,6 +40,44 @@ const char *last_error()
 {
     return g_last_error.c_str();
 }
+
+namespace {
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_idle_streams; // (device, stream)
+constexpr int kIdleStreamsPerDevice = 8;
+} // namespace
+
+hipError_t stream_acquire(int device, hipStream_t *s)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_stream_mu);
+        for (size_t k = g_idle_streams.size(); k-- > 0;)
+            if (g_idle_streams[k].first == device) {
+                *s = g_idle_streams[k].second;
+                g_idle_streams.erase(g_idle_streams.begin() + (long)k);
+                return hipSuccess;
+            }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+void stream_release(int device, hipStream_t s)
+{
+    if (!s)
+        return;
+    (void)hipSetDevice(device);
+    if (hipStreamSynchronize(s) == hipSuccess) {
+        std::lock_guard<std::mutex> lock(g_stream_mu);
+        int kept = 0;
+        for (auto &e : g_idle_streams)
+            kept += e.first == device;
+        if (kept < kIdleStreamsPerDevice) {
+            g_idle_streams.emplace_back(device, s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
+}
 } // namespace mia_hip_rt
 
 namespace {
@@ -162,13 +200,21 @@ struct mia_hip_ctx {
     double *d_scratch = nullptr;
     unsigned *d_cnt = nullptr;
 
+    // pinned host buffers, kept across set-ups and grown when a set-up needs
+    // more (capacities in elements): the graph's parameter, gradient and cost
+    // staging, the optimizer's done flags (two slots), and the staging of
+    // small uploads (a running offset, reset only once the stream is idle)
     double *h_params = nullptr, *h_grad = nullptr, *h_costs = nullptr;
+    size_t cap_params = 0, cap_grad = 0, cap_costs = 0, cap_done = 0;
+    char *h_up = nullptr;
+    size_t cap_up = 0, up_off = 0;
     // on-device optimizer state (allocated on first use, sized for `opt_memory`)
     int opt_memory = 0;
     double *d_oX = nullptr, *d_oG = nullptr, *d_oD = nullptr, *d_oBX = nullptr, *d_oS = nullptr, *d_oY = nullptr,
            *d_obest = nullptr, *d_oSY = nullptr, *d_oYY = nullptr, *d_oPART = nullptr, *d_oPART2 = nullptr;
     int *d_oseg = nullptr;
     int opt_segments = 0; // segments of the optimizer's passes over all instances
+    int opt_launch = MIA_HIP_LAUNCH_STREAM; // how mia_hip_optimize queues its chunks
     int *d_odone = nullptr, *h_done = nullptr;
     OptState *d_ost = nullptr, *d_ost2 = nullptr;
     hipGraph_t graph = nullptr;
@@ -241,20 +287,12 @@ void free_buffers(mia_hip_ctx *c)
     pool_trim(c);
     for (void *p : ptrs)
         pool_release(c, p);
-    if (c->h_params)
-        (void)hipHostFree(c->h_params);
-    if (c->h_grad)
-        (void)hipHostFree(c->h_grad);
-    if (c->h_costs)
-        (void)hipHostFree(c->h_costs);
-    if (c->h_done)
-        (void)hipHostFree(c->h_done);
     c->d_oX = c->d_oG = c->d_oD = c->d_oBX = c->d_oS = c->d_oY = c->d_obest = nullptr;
     c->d_oSY = c->d_oYY = c->d_oPART = c->d_oPART2 = nullptr;
     c->d_oseg = nullptr;
     c->d_ost2 = nullptr;
     c->opt_segments = 0;
-    c->d_odone = c->h_done = nullptr;
+    c->d_odone = nullptr;
     c->d_ost = nullptr;
     c->opt_memory = 0;
     if (c->gexec)
@@ -271,9 +309,68 @@ void free_buffers(mia_hip_ctx *c)
     c->d_cnt = nullptr;
     c->d_pen_on = nullptr;
     c->d_pen_w = c->d_pen_value = c->d_pen_grad = nullptr;
-    c->h_params = c->h_grad = c->h_costs = nullptr;
     c->gexec = nullptr;
     c->graph = nullptr;
+}
+
+// A pinned host buffer of at least `count` elements, kept from earlier
+// calls when large enough (callers make sure no queued copy still uses it).
+template <typename T> int host_reserve(T **p, size_t *cap, size_t count)
+{
+    count = std::max<size_t>(count, 1);
+    if (*p && *cap >= count)
+        return MIA_HIP_OK;
+    if (*p)
+        (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIP_TRY(hipHostMalloc((void **)p, sizeof(T) * count, hipHostMallocDefault));
+    *cap = count;
+    return MIA_HIP_OK;
+}
+
+// Queue an upload of host memory through the context's pinned staging (a
+// copy from pageable memory takes the runtime's slower staged path): the
+// bytes are copied into the staging buffer at its running offset and sent
+// from there on the context's stream.  When the buffer is full the stream is
+// drained first, so that no queued copy still reads what is overwritten.
+int stage_upload(mia_hip_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0)
+        return MIA_HIP_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (c->up_off + need > c->cap_up) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->up_off = 0;
+        if (need > c->cap_up) {
+            int st = host_reserve(&c->h_up, &c->cap_up, std::max<size_t>(need, std::max<size_t>(2 * c->cap_up, 1 << 20)));
+            if (st)
+                return st;
+        }
+    }
+    std::memcpy(c->h_up + c->up_off, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, c->h_up + c->up_off, bytes, hipMemcpyHostToDevice, c->stream));
+    c->up_off += need;
+    return MIA_HIP_OK;
+}
+
+// Read device memory back through the pinned staging, synchronously (the
+// stream is drained, so the staging offset starts again from zero).
+int stage_download(mia_hip_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0)
+        return MIA_HIP_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->up_off = 0;
+    if (bytes > c->cap_up) {
+        int st = host_reserve(&c->h_up, &c->cap_up, std::max<size_t>(bytes, std::max<size_t>(2 * c->cap_up, 1 << 20)));
+        if (st)
+            return st;
+    }
+    HIP_TRY(hipMemcpyAsync(c->h_up, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(dst, c->h_up, bytes);
+    return MIA_HIP_OK;
 }
 
 // Tuned choices per (kind, interpolation degree, size bucket = ceil(log2(voxels
@@ -939,7 +1036,7 @@ int mia_hip_ctx_create(int device, mia_hip_ctx **out)
         return fail(MIA_HIP_ERR_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library targets gfx950");
     mia_hip_ctx *c = new mia_hip_ctx();
     c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = mia_hip_rt::stream_acquire(device, &c->stream);
     if (e != hipSuccess) {
         delete c;
         return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -963,12 +1060,14 @@ int mia_hip_ctx_destroy(mia_hip_ctx *c)
     for (auto e : c->pool)
         (void)hipEventDestroy(e);
     if (c->pen_stream) {
-        (void)hipStreamSynchronize(c->pen_stream);
-        (void)hipStreamDestroy(c->pen_stream);
+        mia_hip_rt::stream_release(c->device, c->pen_stream);
         (void)hipEventDestroy(c->pen_fork);
         (void)hipEventDestroy(c->pen_join);
     }
-    (void)hipStreamDestroy(c->stream);
+    for (void *h : {(void *)c->h_params, (void *)c->h_grad, (void *)c->h_costs, (void *)c->h_done, (void *)c->h_up})
+        if (h)
+            (void)hipHostFree(h);
+    mia_hip_rt::stream_release(c->device, c->stream);
     delete c;
     return MIA_HIP_OK;
 }
@@ -1228,22 +1327,20 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
         free_buffers(c);
         return st;
     }
-    HIP_TRY(hipHostMalloc((void **)&c->h_params, sizeof(double) * std::max<long long>(par, 1), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&c->h_grad, sizeof(double) * std::max<long long>(par, 1), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&c->h_costs, sizeof(double) * n_inst, hipHostMallocDefault));
-    HIP_TRY(hipMemcpyAsync(c->d_inst, c->inst.data(), sizeof(InstDesc) * n_inst, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_wgroups, wg.data(), sizeof(int) * wg.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_s1_prefix, c->s1_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_red_prefix, c->red_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_s1z_prefix, c->s1z_prefix.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice,
-                           c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_redz_prefix, c->redz_prefix.data(), sizeof(int) * (n_inst + 1),
-                           hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_tw, tw.data(), sizeof(float) * tw.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_ts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_cells, cells.data(), sizeof(int) * cells.size(), hipMemcpyHostToDevice, c->stream));
+    if ((st = host_reserve(&c->h_params, &c->cap_params, (size_t)par)) ||
+        (st = host_reserve(&c->h_grad, &c->cap_grad, (size_t)par)) ||
+        (st = host_reserve(&c->h_costs, &c->cap_costs, (size_t)n_inst)))
+        return st;
+    if ((st = stage_upload(c, c->d_inst, c->inst.data(), sizeof(InstDesc) * n_inst)) ||
+        (st = stage_upload(c, c->d_wgroups, wg.data(), sizeof(int) * wg.size())) ||
+        (st = stage_upload(c, c->d_s1_prefix, c->s1_prefix.data(), sizeof(int) * (n_inst + 1))) ||
+        (st = stage_upload(c, c->d_red_prefix, c->red_prefix.data(), sizeof(int) * (n_inst + 1))) ||
+        (st = stage_upload(c, c->d_s1z_prefix, c->s1z_prefix.data(), sizeof(int) * (n_inst + 1))) ||
+        (st = stage_upload(c, c->d_redz_prefix, c->redz_prefix.data(), sizeof(int) * (n_inst + 1))) ||
+        (st = stage_upload(c, c->d_tw, tw.data(), sizeof(float) * tw.size())) ||
+        (st = stage_upload(c, c->d_ts, ts.data(), sizeof(int) * ts.size())) ||
+        (st = stage_upload(c, c->d_cells, cells.data(), sizeof(int) * cells.size())))
+        return st;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n_inst = n_inst;
     return MIA_HIP_OK;
@@ -1343,7 +1440,7 @@ int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double wei
             return st;
     }
     if (p && !c->pen_stream) {
-        HIP_TRY(hipStreamCreateWithFlags(&c->pen_stream, hipStreamNonBlocking));
+        HIP_TRY(mia_hip_rt::stream_acquire(c->device, &c->pen_stream));
         HIP_TRY(hipEventCreateWithFlags(&c->pen_fork, hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->pen_join, hipEventDisableTiming));
     }
@@ -1353,8 +1450,10 @@ int mia_hip_set_penalty(mia_hip_ctx *c, int inst, mia_hip_divcurl *p, double wei
         std::vector<int> on(c->n_inst);
         for (int i = 0; i < c->n_inst; ++i)
             on[i] = c->penalty[i] ? 1 : 0;
-        HIP_TRY(hipMemcpy(c->d_pen_on, on.data(), sizeof(int) * c->n_inst, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(c->d_pen_w, c->penalty_weight.data(), sizeof(double) * c->n_inst, hipMemcpyHostToDevice));
+        int st;
+        if ((st = stage_upload(c, c->d_pen_on, on.data(), sizeof(int) * c->n_inst)) ||
+            (st = stage_upload(c, c->d_pen_w, c->penalty_weight.data(), sizeof(double) * c->n_inst)))
+            return st;
     }
     drop_graph(c);
     return MIA_HIP_OK;
@@ -1545,9 +1644,6 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
                        c->d_ost, c->d_oSY, c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
         for (void *p : old)
             pool_release(c, p);
-        if (c->h_done)
-            (void)hipHostFree(c->h_done);
-        c->h_done = nullptr;
         c->opt_memory = 0;
         const size_t R = (size_t)memory + 1; // ring: memory pairs and the candidate's slot
         if ((st = dev_alloc(c, &c->d_oX, c->n_par)) || (st = dev_alloc(c, &c->d_oG, c->n_par)) ||
@@ -1559,7 +1655,8 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
             (st = dev_alloc(c, &c->d_oPART2, (size_t)kPart * seg[n])) || (st = dev_alloc(c, &c->d_ost2, n)) ||
             (st = dev_alloc(c, &c->d_oseg, n + 1)))
             return st;
-        HIP_TRY(hipHostMalloc((void **)&c->h_done, sizeof(int) * n, hipHostMallocDefault));
+        if ((st = host_reserve(&c->h_done, &c->cap_done, 2 * (size_t)n)))
+            return st;
         c->opt_memory = memory;
         c->opt_segments = seg[n];
     }
@@ -1571,21 +1668,21 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
         o.mode = 0;
     }
     std::vector<int> zeros(n, 0);
-    HIP_TRY(hipMemcpyAsync(c->d_params, x0, sizeof(double) * c->n_par, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_ost, st0.data(), sizeof(OptState) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_odone, zeros.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_oseg, seg.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((st = stage_upload(c, c->d_params, x0, sizeof(double) * c->n_par)) ||
+        (st = stage_upload(c, c->d_ost, st0.data(), sizeof(OptState) * n)) ||
+        (st = stage_upload(c, c->d_odone, zeros.data(), sizeof(int) * n)) ||
+        (st = stage_upload(c, c->d_oseg, seg.data(), sizeof(int) * (n + 1))))
+        return st;
     OptBuffers ob{c->d_oX,   c->d_oG,   c->d_oD,   c->d_oBX,    c->d_oS,     c->d_oY,   c->d_obest, c->d_odone,
                   c->d_ost,  c->d_ost2, c->d_oSY,  c->d_oYY,    c->d_oPART,  c->d_oPART2, c->d_oseg};
-    // capture iters_per_graph iterations + the done flags read-back
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = MIA_HIP_OK;
-    for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
-        rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
-        if (rc == MIA_HIP_OK) {
+    // one chunk: iters_per_graph iterations (evaluation, then the step's
+    // kernels), then the done flags copied to host slot `slot`
+    auto enqueue_chunk = [&](int slot) -> int {
+        int rc = MIA_HIP_OK;
+        for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
+            rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
+            if (rc != MIA_HIP_OK)
+                break;
             hipError_t e = hipSuccess;
             if (n_small > 0) {
                 hipLaunchKernelGGL(k_lbfgs_step_small, dim3(n), dim3(kOptBlock), 0, c->stream,
@@ -1613,59 +1710,136 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
             if (e != hipSuccess)
                 rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs passes: ") + hipGetErrorString(e));
         }
-    }
-    if (rc == MIA_HIP_OK) {
-        hipError_t e = hipMemcpyAsync(c->h_done, c->d_odone, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
-        if (e != hipSuccess)
-            rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-    }
-    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (rc != MIA_HIP_OK) {
-        if (g)
-            (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (e2 != hipSuccess)
-        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
-    hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    if (e3 != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return fail(MIA_HIP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-    }
-    // every iteration performs one evaluation per unfinished instance
-    const int max_launches = (maxeval + iters_per_graph - 1) / iters_per_graph + 1;
-    auto t0 = std::chrono::steady_clock::now();
-    for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
-        hipError_t e = hipGraphLaunch(ge, c->stream);
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
-            break;
+        if (rc == MIA_HIP_OK) {
+            hipError_t e = hipMemcpyAsync(c->h_done + (size_t)slot * n, c->d_odone, sizeof(int) * n,
+                                          hipMemcpyDeviceToHost, c->stream);
+            if (e != hipSuccess)
+                rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
         }
-        bool all = true;
+        return rc;
+    };
+    auto all_done = [&](int slot) {
         for (int i = 0; i < n; ++i)
-            all = all && c->h_done[i];
-        if (all)
-            break;
+            if (!c->h_done[(size_t)slot * n + i])
+                return false;
+        return true;
+    };
+    // every iteration performs one evaluation per unfinished instance, and an
+    // instance finishes at the latest with its maxeval-th evaluation
+    const int max_launches = (maxeval + iters_per_graph - 1) / iters_per_graph + 1;
+    const int sure_launches = max_launches - 1;
+    int rc = MIA_HIP_OK;
+    std::chrono::steady_clock::time_point t0, t1;
+    if (c->opt_launch == MIA_HIP_LAUNCH_STREAM) {
+        // stream launches, one chunk queued ahead while the host reads the
+        // previous chunk's done flags (only chunks that start before the
+        // maxeval-th iteration are queued ahead; a chunk queued after all
+        // instances finished changes nothing: finished instances' steps
+        // return at once)
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        for (auto &e : ev)
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        t0 = std::chrono::steady_clock::now();
+        rc = enqueue_chunk(0);
+        if (rc == MIA_HIP_OK) {
+            hipError_t e = hipEventRecord(ev[0], c->stream);
+            if (e != hipSuccess)
+                rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+        }
+        for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
+            const bool ahead = l + 1 < sure_launches;
+            if (ahead) {
+                rc = enqueue_chunk((l + 1) & 1);
+                if (rc != MIA_HIP_OK)
+                    break;
+                hipError_t e = hipEventRecord(ev[(l + 1) & 1], c->stream);
+                if (e != hipSuccess) {
+                    rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+                    break;
+                }
+            }
+            hipError_t e = hipEventSynchronize(ev[l & 1]);
+            if (e != hipSuccess) {
+                rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize: ") + hipGetErrorString(e));
+                break;
+            }
+            if (all_done(l & 1) || l + 1 == max_launches)
+                break;
+            if (!ahead) {
+                rc = enqueue_chunk((l + 1) & 1);
+                if (rc != MIA_HIP_OK)
+                    break;
+                e = hipEventRecord(ev[(l + 1) & 1], c->stream);
+                if (e != hipSuccess)
+                    rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+            }
+        }
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess && rc == MIA_HIP_OK)
+            rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize: ") + hipGetErrorString(e));
+        t1 = std::chrono::steady_clock::now();
+        for (auto &ev1 : ev)
+            (void)hipEventDestroy(ev1);
+    } else {
+        // one captured graph of a chunk, launched until every instance is done
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_chunk(0);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (rc != MIA_HIP_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e2 != hipSuccess)
+            return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+        hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(MIA_HIP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+        }
+        t0 = std::chrono::steady_clock::now();
+        for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
+            hipError_t e = hipGraphLaunch(ge, c->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) {
+                rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
+                break;
+            }
+            if (all_done(0))
+                break;
+        }
+        t1 = std::chrono::steady_clock::now();
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
     }
-    auto t1 = std::chrono::steady_clock::now();
-    (void)hipGraphExecDestroy(ge);
-    (void)hipGraphDestroy(g);
     if (rc != MIA_HIP_OK)
         return rc;
     if (seconds)
         *seconds = std::chrono::duration<double>(t1 - t0).count();
-    if (x_out)
-        HIP_TRY(hipMemcpy(x_out, c->d_params, sizeof(double) * c->n_par, hipMemcpyDeviceToHost));
-    if (costs_out)
-        HIP_TRY(hipMemcpy(costs_out, c->d_obest, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (x_out && (st = stage_download(c, x_out, c->d_params, sizeof(double) * c->n_par)))
+        return st;
+    if (costs_out && (st = stage_download(c, costs_out, c->d_obest, sizeof(double) * n)))
+        return st;
     if (evals_out) {
         std::vector<OptState> stv(n);
-        HIP_TRY(hipMemcpy(stv.data(), c->d_ost, sizeof(OptState) * n, hipMemcpyDeviceToHost));
+        if ((st = stage_download(c, stv.data(), c->d_ost, sizeof(OptState) * n)))
+            return st;
         for (int i = 0; i < n; ++i)
             evals_out[i] = stv[i].evals;
     }
+    return MIA_HIP_OK;
+}
+
+int mia_hip_set_optimize_launch(mia_hip_ctx *c, int mode)
+{
+    if (!c)
+        return fail(MIA_HIP_ERR_ARG, "null context");
+    if (mode != MIA_HIP_LAUNCH_STREAM && mode != MIA_HIP_LAUNCH_GRAPH)
+        return fail(MIA_HIP_ERR_ARG, "optimize launch: mode must be MIA_HIP_LAUNCH_STREAM or MIA_HIP_LAUNCH_GRAPH");
+    c->opt_launch = mode;
     return MIA_HIP_OK;
 }
 

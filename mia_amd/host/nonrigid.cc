@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <sstream>
+#include <thread>
 
 #include "descr.hh"
 #include "minimizer.hh"
@@ -926,12 +927,28 @@ bool normalize_images(CImage &src, CImage &ref)
     const double sigma = std::sqrt((sum2 - sum * sum / n) / (n - 1.0));
     if (sigma == 0.0)
         return false;
-    // "convert:repn=float,map=linear,b=-mean/sigma,a=1/sigma"
+    // "convert:repn=float,map=linear,b=-mean/sigma,a=1/sigma"; elementwise,
+    // so host threads over contiguous chunks give the same values as one loop
     const double a = 1.0 / sigma, b = -mean / sigma;
-    for (size_t i = 0; i < n1; ++i) {
-        src.data[i] = (float)(a * src.data[i] + b);
-        ref.data[i] = (float)(a * ref.data[i] + b);
+    auto map = [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            src.data[i] = (float)(a * src.data[i] + b);
+            ref.data[i] = (float)(a * ref.data[i] + b);
+        }
+    };
+    const size_t chunk = 1 << 20;
+    const size_t nt = std::min<size_t>({(n1 + chunk - 1) / chunk, 16, std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        map(0, n1);
+        return true;
     }
+    std::vector<std::thread> pool;
+    const size_t per = (n1 + nt - 1) / nt;
+    for (size_t t = 1; t < nt; ++t)
+        pool.emplace_back(map, std::min(n1, t * per), std::min(n1, (t + 1) * per));
+    map(0, std::min(n1, per));
+    for (auto &th : pool)
+        th.join();
     return true;
 }
 
@@ -998,10 +1015,13 @@ bool CNonrigidRegister::minimize_on_device(CSplineTransformation &t, std::vector
 
 PSplineTransformation CNonrigidRegister::run(const CImage &src_in, const CImage &ref_in)
 {
-    if (src_in.size != ref_in.size || src_in.dim != ref_in.dim)
+    return run(std::make_shared<CImage>(src_in), std::make_shared<CImage>(ref_in));
+}
+
+PSplineTransformation CNonrigidRegister::run(PImage src, PImage ref)
+{
+    if (!src || !ref || src->size != ref->size || src->dim != ref->dim)
         throw std::invalid_argument("CNonrigidRegister: src and ref differ in size");
-    auto src = std::make_shared<CImage>(src_in);
-    auto ref = std::make_shared<CImage>(ref_in);
     m_stats.clear();
     if (!normalize_images(*src, *ref))
         return m_creator.create(src->size);

@@ -51,6 +51,8 @@ struct CImage {
     std::vector<float> data;
     CImage() = default;
     CImage(int d, Size3 s) : dim(d), size(s), data(s.product(), 0.0f) {}
+    // a copy of s.product() values at `values` (one pass over fresh memory)
+    CImage(int d, Size3 s, const float *values) : dim(d), size(s), data(values, values + s.product()) {}
 };
 using PImage = std::shared_ptr<CImage>;
 
@@ -355,6 +357,8 @@ public:
     CNonrigidRegister(CFullCostList &costs, const std::string &minimizer, const CSplineTransformCreator &creator,
                       int mg_levels, int idx = -1);
     PSplineTransformation run(const CImage &src, const CImage &ref);
+    // the same on images the registration may keep and normalise in place
+    PSplineTransformation run(PImage src, PImage ref);
     const std::vector<LevelStats> &get_level_stats() const { return m_stats; }
 
 private:

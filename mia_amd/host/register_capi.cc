@@ -26,9 +26,8 @@ extern "C" int mia_host_nonrigid_register(int device, int dim, const uint32_t *s
         return detail::host_fail(MIA_HIP_ERR_ARG, "nonrigid_register: bad arguments");
     try {
         const Size3 s{size[0], size[1], dim == 3 ? size[2] : 1u};
-        CImage a(dim, s), b(dim, s);
-        std::memcpy(a.data.data(), src, sizeof(float) * s.product());
-        std::memcpy(b.data.data(), ref, sizeof(float) * s.product());
+        auto a = std::make_shared<CImage>(dim, s, src);
+        auto b = std::make_shared<CImage>(dim, s, ref);
         CFullCostList costs;
         std::string rest(cost);
         size_t pos;
@@ -40,7 +39,7 @@ extern "C" int mia_host_nonrigid_register(int device, int dim, const uint32_t *s
         } while (pos != std::string::npos);
         auto creator = produce_transform_creator(dim, transform);
         CNonrigidRegister reg(costs, minimizer, *creator, levels);
-        auto t = reg.run(a, b);
+        auto t = reg.run(std::move(a), std::move(b));
         const auto p = t->get_parameters();
         if (n_params)
             *n_params = p.size();

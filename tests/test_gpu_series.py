@@ -195,3 +195,41 @@ def test_device_optimizer_rejects_gdsq():
     with series.SeriesRegistration(frames) as s:
         with pytest.raises(ValueError):
             s.run("gdsq:maxiter=5", series.DEVICE_BATCHED)
+
+
+@pytest.mark.parametrize("ipg", [1, 7, 10])
+def test_optimize_stream_and_graph_launches_agree(ipg):
+    """mia_hip_optimize queued as direct stream launches (next chunk queued
+    ahead) and as one captured graph per chunk: the same kernels in the same
+    order, so parameters, costs and evaluation counts are bitwise equal, for
+    chunk sizes that divide maxeval and that do not, with frames on both sides
+    of the 2048-parameter limit of the one-block step."""
+    import mia_amd
+    from mia_amd import series, synth
+    frames = synth.series_frames_2d(2, (60, 80), 8.0, 4.0, 51) + synth.series_frames_2d(1, (300, 320), 8.0, 4.0, 52)
+    pairs = [series.normalize_pair(m, f) for m, f in frames]
+    shapes = [p[0].shape for p in pairs]
+    cs = [(mia_amd.spline_csize(s[1], 8.0, 2), mia_amd.spline_csize(s[0], 8.0, 2)) for s in shapes]
+    out = {}
+    with mia_amd.HipContext(0) as c:
+        c.setup(2, shapes, cs, 3)
+        for k, p in enumerate(pairs):
+            c.set_images(k, p[0], p[1])
+        with pytest.raises(ValueError):
+            mia_amd._lib.check(c._L.mia_hip_set_optimize_launch(c._h, 2))
+        with pytest.raises(ValueError):
+            c.set_optimize_launch("persistent")
+        for mode in ("graph", "stream"):
+            c.set_optimize_launch(mode)
+            out[mode] = c.optimize(np.zeros(c.n_params_total), maxeval=20, iters_per_graph=ipg)[:3]
+    for a, b in zip(out["stream"], out["graph"]):
+        assert np.array_equal(a, b)
+    assert np.all(out["stream"][2] <= 20)
+    ref = out["stream"]
+    with mia_amd.HipContext(0) as c:
+        c.setup(2, shapes, cs, 3)
+        for k, p in enumerate(pairs):
+            c.set_images(k, p[0], p[1])
+        again = c.optimize(np.zeros(c.n_params_total), maxeval=20, iters_per_graph=20)[:3]
+    for a, b in zip(ref, again):
+        assert np.array_equal(a, b)
