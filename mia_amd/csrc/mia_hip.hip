@@ -391,6 +391,8 @@ struct Tuning {
     bool loaded = false;
     std::string arch, source;
     std::map<std::tuple<int, int, int>, int> table; // (kind, degree, bucket) -> choice
+    // (kind, degree, bucket) -> the tuned case's time of every choice (us), where the table has them
+    std::map<std::tuple<int, int, int>, std::vector<double>> us;
 };
 
 Tuning &tuning()
@@ -468,6 +470,31 @@ bool json_int(const std::string &text, size_t b, size_t e, const char *key, int 
     return true;
 }
 
+// the array of numbers "key": [a, b, ...] in text[b, e); false without one
+bool json_doubles(const std::string &text, size_t b, size_t e, const char *key, std::vector<double> &v)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    const size_t a = text.find(k, b);
+    if (a == std::string::npos || a >= e)
+        return false;
+    const size_t o = text.find('[', a + k.size()), cl = text.find(']', a + k.size());
+    if (o == std::string::npos || cl == std::string::npos || o > cl || cl >= e)
+        return false;
+    v.clear();
+    const char *q = text.c_str() + o + 1, *end = text.c_str() + cl;
+    while (q < end) {
+        char *nx = nullptr;
+        const double x = std::strtod(q, &nx);
+        if (nx == q)
+            break;
+        v.push_back(x);
+        q = nx;
+        while (q < end && (*q == ',' || *q == ' ' || *q == '\n'))
+            ++q;
+    }
+    return !v.empty();
+}
+
 int tune_kind(const std::string &name)
 {
     if (name.empty())
@@ -480,7 +507,8 @@ int tune_kind(const std::string &name)
 
 // the entries of a table written by mia_hip_autotune: objects with "bucket",
 // "degree", "variant" and (optionally) "kernel"; false if unreadable
-bool parse_tuning(const std::string &path, std::string &arch, std::map<std::tuple<int, int, int>, int> &table)
+bool parse_tuning(const std::string &path, std::string &arch, std::map<std::tuple<int, int, int>, int> &table,
+                  std::map<std::tuple<int, int, int>, std::vector<double>> *us = nullptr)
 {
     std::string text;
     if (!read_text(path, text))
@@ -489,6 +517,8 @@ bool parse_tuning(const std::string &path, std::string &arch, std::map<std::tupl
     if (arch.empty())
         return false;
     table.clear();
+    if (us)
+        us->clear();
     size_t pos = 0;
     while ((pos = text.find("\"bucket\"", pos)) != std::string::npos) {
         const size_t b = text.rfind('{', pos), e = text.find('}', pos);
@@ -498,8 +528,12 @@ bool parse_tuning(const std::string &path, std::string &arch, std::map<std::tupl
         const int kind = tune_kind(json_string(text, b, e, "kernel"));
         int d = 0, bk = 0, v = 0;
         if (kind >= 0 && json_int(text, b, e, "degree", d) && json_int(text, b, e, "bucket", bk) &&
-            json_int(text, b, e, "variant", v) && v >= 0 && v < kTuneChoices[kind])
+            json_int(text, b, e, "variant", v) && v >= 0 && v < kTuneChoices[kind]) {
             table[std::make_tuple(kind, d, bk)] = v;
+            std::vector<double> t;
+            if (us && json_doubles(text, b, e, "us", t) && (int)t.size() == kTuneChoices[kind])
+                (*us)[std::make_tuple(kind, d, bk)] = t;
+        }
     }
     return true;
 }
@@ -513,8 +547,11 @@ void load_tuning_locked(Tuning &t, int device, const char *path)
     const std::string p = path ? std::string(path) : default_tuning_path(arch);
     std::string file_arch;
     std::map<std::tuple<int, int, int>, int> table;
-    if (!p.empty() && parse_tuning(p, file_arch, table) && file_arch == arch) {
+    std::map<std::tuple<int, int, int>, std::vector<double>> us;
+    t.us.clear();
+    if (!p.empty() && parse_tuning(p, file_arch, table, &us) && file_arch == arch) {
         t.table = table;
+        t.us = us;
         t.source = p;
     }
     t.arch = arch;
@@ -545,6 +582,45 @@ int tuned_choice(int device, int kind, int degree, long long n)
             best = kv.second;
         }
     return best;
+}
+
+// One 2D warp variant for a whole batch.  Every variant computes the same
+// image (same weights and contraction order; the tile only decides which
+// block computes a pixel), so a batch whose instances fall into size buckets
+// with different tuned variants can run one launch instead of one per
+// variant: the variant with the least tuned time summed over the instances'
+// (nearest) buckets, each bucket's time being that of a batch of like
+// instances.  -1 when some bucket has no timings in the table (then each
+// instance keeps its own bucket's choice).
+int batch_warp2d_choice(int device, int degree, const std::vector<long long> &npix)
+{
+    Tuning &t = tuning();
+    std::lock_guard<std::mutex> lock(t.m);
+    if (!t.loaded)
+        load_tuning_locked(t, device, nullptr);
+    const int kind = MIA_HIP_TUNE_WARP2D;
+    std::vector<double> sum(kTuneChoices[kind], 0.0);
+    for (long long n : npix) {
+        const int bucket = size_bucket(n);
+        const std::vector<double> *best = nullptr;
+        int dist = 1 << 30;
+        for (const auto &kv : t.table)
+            if (std::get<0>(kv.first) == kind && std::get<1>(kv.first) == degree &&
+                std::abs(std::get<2>(kv.first) - bucket) < dist) {
+                dist = std::abs(std::get<2>(kv.first) - bucket);
+                const auto it = t.us.find(kv.first);
+                best = it == t.us.end() ? nullptr : &it->second;
+            }
+        if (!best)
+            return -1;
+        for (int v = 0; v < kTuneChoices[kind]; ++v)
+            sum[v] += (*best)[v];
+    }
+    int v = 0;
+    for (int k = 1; k < kTuneChoices[kind]; ++k)
+        if (sum[k] < sum[v])
+            v = k;
+    return v;
 }
 
 // whether an instance may run the z-cell stage 1 (one x column per thread,
@@ -1151,6 +1227,14 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
     c->penalty.assign(n_inst, nullptr);
     c->penalty_weight.assign(n_inst, 0.0);
     c->inst_variant.assign(n_inst, 0);
+    // 2D: one warp variant for the batch where the tuning table times them
+    int batch_v2 = -1;
+    if (dim == 2 && c->force_variant < 0 && n_inst > 1) {
+        std::vector<long long> npix(n_inst);
+        for (int i = 0; i < n_inst; ++i)
+            npix[i] = (long long)sizes[3 * i] * sizes[3 * i + 1];
+        batch_v2 = batch_warp2d_choice(c->device, degree, npix);
+    }
     std::vector<float> tw;
     std::vector<int> ts, cells;
     long long img = 0, par = 0, part = 0, maxv = 0;
@@ -1227,7 +1311,9 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
         max_nx = std::max(max_nx, d.nx);
         if (dim == 2) {
             // the 2D warp tile and the coefficient window it loads
-            const int v = c->force_variant >= 0 ? c->force_variant : tuned_choice(c->device, MIA_HIP_TUNE_WARP2D, degree, nvox);
+            const int v = c->force_variant >= 0 ? c->force_variant
+                          : batch_v2 >= 0      ? batch_v2
+                                               : tuned_choice(c->device, MIA_HIP_TUNE_WARP2D, degree, nvox);
             const int wx = tile_span(d.nx, w2_tile_x(v), [&](int p) { return ax.start[p]; });
             const int wy = tile_span(d.ny, w2_tile_y(v), [&](int p) { return ay.start[p]; });
             w2_win = std::max(w2_win, wx * wy);
@@ -1685,7 +1771,9 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
                 break;
             hipError_t e = hipSuccess;
             if (n_small > 0) {
-                hipLaunchKernelGGL(k_lbfgs_step_small, dim3(n), dim3(kOptBlock), 0, c->stream,
+                // memory <= kLbRegMem: the stored pairs stay in registers for the whole step
+                auto kstep = memory <= kLbRegMem ? k_lbfgs_step_small<true> : k_lbfgs_step_small<false>;
+                hipLaunchKernelGGL(kstep, dim3(n), dim3(kOptBlock), 0, c->stream,
                                    (const InstDesc *)c->d_inst, c->dim, ob, c->d_params, (const double *)c->d_grad,
                                    (const double *)c->d_costs, memory, maxeval, first_step);
                 e = hipGetLastError();

@@ -782,6 +782,28 @@ __device__ __forceinline__ void lb_load_group(double (*sc)[kLbK], double (*yc)[k
         }
 }
 
+// With memory <= kLbRegMem the one-block step keeps every stored pair in
+// registers for the whole step: read once, oldest first, they serve both the
+// product pass and the direction, which otherwise reads them from memory a
+// second time.
+constexpr int kLbRegMem = 10;
+
+// Stored pairs 0 .. kk - 1 (oldest first, ring slots from head) into ps, py.
+// Every load is issued unconditionally (slots beyond kk repeat the newest
+// stored pair, or slot head when none is stored: in bounds, cached, unused),
+// so that none waits at the end of a branch.
+__device__ __forceinline__ void lb_load_pairs(double (*ps)[kLbK], double (*py)[kLbK], int kk, int head, int memory,
+                                              const double *S, const double *Y, long long n)
+{
+#pragma unroll
+    for (int j = 0; j < kLbRegMem; ++j) {
+        const int jc = j < kk ? j : (kk > 0 ? kk - 1 : 0);
+        const int slot = (head + jc) % memory;
+        lb_load(ps[j], S + (long long)slot * n, n);
+        lb_load(py[j], Y + (long long)slot * n, n);
+    }
+}
+
 // Accepted step, vectors in registers (xt, x, gv, g: this thread's elements of XT, X, GT, G):
 // X <- XT, G <- GT, the new pair s = XT - X, y = GT - G left in sv, yv, and the wave partials of every product of kPart written to part[wave][.] (complete after the
 // caller's barrier).  kk pairs are stored from ring slot head.
@@ -840,6 +862,70 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
             const int j = j0 + 4 * h + (lane >> 4);
             if ((lane & 3) == 3 && j < kk)
                 part[w][kPPair + 4 * (j0 + 4 * h) + (lane >> 2)] = t;
+        }
+    }
+}
+
+// lbfgs_accept_pass with the kk stored pairs already in registers (ps, py,
+// oldest first): the same products, summed in the same order.
+__device__ __forceinline__ void lbfgs_accept_pass_reg(long long n, int kk, double *X, double *G, const double *xt,
+                                                      const double *x, const double *gv, const double *g,
+                                                      double *sv, double *yv, const double (*ps)[kLbK],
+                                                      const double (*py)[kLbK], double (*part)[kPart], double *tbuf)
+{
+    double sy = 0.0, yy = 0.0, ss = 0.0, sg = 0.0, yg = 0.0, gm = 0.0;
+#pragma unroll
+    for (int k = 0; k < kLbK; ++k) {
+        sv[k] = xt[k] - x[k];
+        yv[k] = gv[k] - g[k];
+        sy += sv[k] * yv[k];
+        yy += yv[k] * yv[k];
+        ss += sv[k] * sv[k];
+        sg += sv[k] * gv[k];
+        yg += yv[k] * gv[k];
+        gm = fmax(gm, fabs(gv[k]));
+    }
+    lb_store(X, xt, n);
+    lb_store(G, gv, n);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *tb = tbuf + w * 16 * kTbRow;
+    {
+        const double h[16] = {sy, yy, ss, sg, yg, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const double t = transpose_sum16(h, tb);
+        gm = wave_reduce63<true>(gm);
+        if ((lane & 3) == 3 && lane < 4 * kPGMAX)
+            part[w][lane >> 2] = t;
+        if (lane == 63)
+            part[w][kPGMAX] = gm;
+    }
+    // two pairs per transposed sum (values 4 t + q: pair j0 + t, product q;
+    // the other eight are zero), which keeps the register file within
+    // bounds next to the pairs
+#pragma unroll
+    for (int j0 = 0; j0 < kLbRegMem; j0 += 2) {
+        if (j0 < kk) {
+            double dd[16];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+                if (t < 2 && j0 + t < kLbRegMem) {
+#pragma unroll
+                    for (int k = 0; k < kLbK; ++k) {
+                        d0 += ps[j0 + t][k] * yv[k];
+                        d1 += py[j0 + t][k] * yv[k];
+                        d2 += ps[j0 + t][k] * gv[k];
+                        d3 += py[j0 + t][k] * gv[k];
+                    }
+                }
+                dd[4 * t] = d0;
+                dd[4 * t + 1] = d1;
+                dd[4 * t + 2] = d2;
+                dd[4 * t + 3] = d3;
+            }
+            const double t = transpose_sum16(dd, tb);
+            const int j = j0 + (lane >> 4);
+            if ((lane & 3) == 3 && (lane >> 4) < 2 && j < kk)
+                part[w][kPPair + 4 * j0 + (lane >> 2)] = t;
         }
     }
 }
@@ -937,10 +1023,55 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
 }
 
 
+// lbfgs_compact_direction with the pairs in registers: the stored pairs
+// drop .. kk - 1 (ps, py) and, when a pair was stored by this step, the new
+// one (sv, yv), oldest first, added in the same order.
+__device__ __forceinline__ double lbfgs_compact_direction_reg(long long n, int kk, int drop, bool fresh, int knew,
+                                                              const double *g, const double (*ps)[kLbK],
+                                                              const double (*py)[kLbK], const double *sv,
+                                                              const double *yv, double *D, double *dv, double gamma,
+                                                              const double *cu, const double *cw, double *red,
+                                                              int &rpar)
+{
+#pragma unroll
+    for (int k2 = 0; k2 < kLbK; ++k2)
+        dv[k2] = gamma * g[k2];
+#pragma unroll
+    for (int j = 0; j < kLbRegMem; ++j)
+        if (j >= drop && j < kk) {
+            const double u = cu[j - drop], v = cw[j - drop];
+#pragma unroll
+            for (int k2 = 0; k2 < kLbK; ++k2) {
+                dv[k2] += u * ps[j][k2];
+                dv[k2] -= v * py[j][k2];
+            }
+        }
+    if (fresh) {
+        const double u = cu[knew - 1], v = cw[knew - 1];
+#pragma unroll
+        for (int k2 = 0; k2 < kLbK; ++k2) {
+            dv[k2] += u * sv[k2];
+            dv[k2] -= v * yv[k2];
+        }
+    }
+    double gd = 0.0;
+#pragma unroll
+    for (int k2 = 0; k2 < kLbK; ++k2) {
+        dv[k2] = -dv[k2];
+        const long long e = threadIdx.x + (long long)k2 * kOptBlock;
+        if (e < n)
+            gd += g[k2] * dv[k2];
+    }
+    lb_store(D, dv, n);
+    return block_sum_all(gd, red, rpar);
+}
+
 // One step for an instance of up to kLbRegN parameters, one block each (the
 // other instances return at once: they take k_lbfgs_pass1..3).  Same step as
 // the segmented passes, with the ring of `memory` slots, products summed
-// over the block's waves.
+// over the block's waves.  REG (memory <= kLbRegMem): the stored pairs are read
+// once into registers and serve both passes.
+template <bool REG>
 __global__ void __launch_bounds__(kOptBlock)
     k_lbfgs_step_small(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
                        const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
@@ -962,6 +1093,7 @@ __global__ void __launch_bounds__(kOptBlock)
     double *SYi = ob.SY + (long long)R * R * i, *YYi = ob.YY + (long long)R * R * i;
     bool fresh = false; // a pair was stored by this step
     int drop = 0;       // ... and replaced the oldest one
+    int kold = 0;       // pairs stored before this step (accepted step)
     const long long off = d.par_off;
     double *X = ob.X + off, *G = ob.G + off, *D = ob.D + off, *BX = ob.BX + off;
     double *S = ob.S + (long long)R * off, *Y = ob.Y + (long long)R * off;
@@ -969,6 +1101,8 @@ __global__ void __launch_bounds__(kOptBlock)
     const double *GT = gt + off;
     // this thread's elements of XT, X, GT, G, loaded while the state is read
     double rxt[kLbK], rx[kLbK], rgt[kLbK], rg[kLbK];
+    double sv[kLbK], yv[kLbK];                              // the new pair (accepted step)
+    double ps[REG ? kLbRegMem : 1][kLbK], py[REG ? kLbRegMem : 1][kLbK]; // stored pairs (REG)
     lb_load(rxt, XT, n);
     lb_load(rgt, GT, n);
     lb_load(rx, X, n);
@@ -1020,13 +1154,18 @@ __global__ void __launch_bounds__(kOptBlock)
         __syncthreads();
         const bool armijo = finite && ft <= f0 + 1e-4 * a0 * gd0;
         if (armijo) {
+            const int kk = s.k, head = s.head;
+            if constexpr (REG)
+                lb_load_pairs(ps, py, kk, head, memory, S, Y, n);
             for (int t = threadIdx.x; t < memory * memory; t += kOptBlock) {
                 mSY[t] = SYi[t];
                 mYY[t] = YYi[t];
             }
-            const int kk = s.k, head = s.head;
-            double sv[kLbK], yv[kLbK];
-            lbfgs_accept_pass(n, memory, kk, head, S, Y, X, G, rxt, rx, rgt, rg, sv, yv, part, tbuf);
+            if constexpr (REG)
+                lbfgs_accept_pass_reg(n, kk, X, G, rxt, rx, rgt, rg, sv, yv, ps, py, part, tbuf);
+            else
+                lbfgs_accept_pass(n, memory, kk, head, S, Y, X, G, rxt, rx, rgt, rg, sv, yv, part, tbuf);
+            kold = kk;
             __syncthreads();
             const double sy = part_total(part, kPSY), yy = part_total(part, kPYY), ss = part_total(part, kPSS);
             const double gmax = part_max(part, kPGMAX);
@@ -1093,8 +1232,9 @@ __global__ void __launch_bounds__(kOptBlock)
     if (search) {
         const int k = s.k, head = s.head;
         double dv[kLbK]; // the direction's elements of this thread
-        double sc[kLbP][kLbK], yc[kLbP][kLbK];
-        lb_load_group(sc, yc, 0, k, head, memory, S, Y, n);
+        double sc[REG ? 1 : kLbP][kLbK], yc[REG ? 1 : kLbP][kLbK];
+        if constexpr (!REG)
+            lb_load_group(sc, yc, 0, k, head, memory, S, Y, n);
         double gamma = 1.0;
         if (k > 0) {
             const int sl = (head + k - 1) % memory;
@@ -1103,7 +1243,12 @@ __global__ void __launch_bounds__(kOptBlock)
                 block_compact_coeffs(memory, k, head, drop, fresh, part, mSY, mYY, gamma, cu, cw);
             __syncthreads();
         }
-        double gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
+        double gd;
+        if constexpr (REG)
+            gd = lbfgs_compact_direction_reg(n, kold, drop, fresh, k, rgt, ps, py, sv, yv, D, dv, gamma, cu, cw, red,
+                                             rpar);
+        else
+            gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
             double g2 = 0.0;
 #pragma unroll

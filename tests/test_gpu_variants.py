@@ -235,3 +235,51 @@ def test_autotune_2d_and_stage1_tables(tmp_path):
                 assert np.abs(g1 - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max()
     finally:
         L.mia_hip_load_warp_tuning(0, None)
+
+
+def test_mixed_2d_batch_one_timed_variant(tmp_path):
+    """With per-variant timings in the table, a 2D batch whose instances fall
+    into buckets with different tuned variants runs one warp launch: the
+    variant with the least timing summed over the instances' buckets (here
+    variant 2, which no bucket picks by itself).  Every variant computes the
+    same image, so the batch still matches one context per instance, each
+    with its own bucket's variant, bitwise."""
+    import mia_amd
+    from mia_amd import _lib, autotune
+    L = _lib.load()
+    nv = len(autotune.variants_2d())
+    try:
+        arch = autotune.device_arch(0)
+        rows = []
+        for b, best in ((13, 1), (14, 3), (15, 0), (16, 4)):
+            us = [10.0] * nv
+            us[best] = 7.0
+            us[2] = 8.0
+            rows.append({"kernel": "warp2d", "degree": 3, "bucket": b, "variant": best, "us": us})
+        path = str(tmp_path / "timed.json")
+        with open(path, "w") as f:
+            json.dump({"arch": arch, "table": rows}, f)
+        _lib.check(L.mia_hip_load_warp_tuning(0, path.encode()))
+        shapes = [(70, 101), (110, 130), (181, 170), (250, 251)]   # 2^13 .. 2^16 pixels
+        data = [_inputs(s, 8.0, 70 + k, 1.5, 4.0) for k, s in enumerate(shapes)]
+        with mia_amd.HipContext(0) as c:
+            c.setup(2, shapes, [d[2] for d in data], 3)
+            assert [c.warp_variant(i) for i in range(4)] == [2, 2, 2, 2]
+            for i, d in enumerate(data):
+                c.set_images(i, d[0], d[1])
+            cb, gb = c.cost_grad(np.concatenate([d[3] for d in data]))
+            wb = [c.warp(i, d[3]) for i, d in enumerate(data)]
+        off = 0
+        for i, (s, d) in enumerate(zip(shapes, data)):
+            with mia_amd.HipContext(0) as c1:
+                c1.setup(2, [s], [d[2]], 3)
+                assert c1.warp_variant(0) == [1, 3, 0, 4][i]
+                c1.set_images(0, d[0], d[1])
+                c1s, g1 = c1.cost_grad(d[3])
+                w1 = c1.warp(0, d[3])
+            assert c1s[0] == cb[i]
+            assert np.array_equal(g1, gb[off:off + g1.size])
+            assert np.array_equal(w1, wb[i])
+            off += g1.size
+    finally:
+        L.mia_hip_load_warp_tuning(0, None)
