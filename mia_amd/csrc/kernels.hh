@@ -177,9 +177,30 @@ __device__ __forceinline__ double grad_with_penalty(double sc, double g, const P
 // helpers
 // ---------------------------------------------------------------------------
 
-// Block -> instance by binary search over the per-instance block prefix.
+// Block -> instance: the largest i < n with prefix[i] <= b (prefix is
+// non-decreasing with prefix[0] = 0, so instances without blocks in a launch
+// are skipped).  Every lane of the calling wave must be active and pass the
+// same b; the result is wave-uniform.  Up to 64 instances one load per lane
+// and a ballot, up to 4096 the same over every s-th entry and then within the
+// chunk found (two loads in sequence instead of a binary search's log2(n)),
+// beyond that a binary search.
 __device__ __forceinline__ int find_inst(const int *__restrict__ prefix, int n, int b)
 {
+    const int lane = threadIdx.x & 63;
+    if (n <= 64) {
+        const int v = prefix[min(lane, n - 1)];
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(lane < n && v <= b);
+        return 63 - __builtin_clzll(m);
+    }
+    if (n <= 64 * 64) {
+        const int s = (n + 63) / 64; // chunk length, <= 64
+        const int v = prefix[min(lane * s, n - 1)];
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(lane * s < n && v <= b);
+        const int c0 = (63 - __builtin_clzll(m)) * s;
+        const int u = prefix[min(c0 + lane, n - 1)];
+        const unsigned long long m2 = __builtin_amdgcn_ballot_w64(lane < s && c0 + lane < n && u <= b);
+        return c0 + 63 - __builtin_clzll(m2);
+    }
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -2093,6 +2114,76 @@ __global__ void __launch_bounds__(kBlock)
 // Per instance: SSD sum and mask count in a fixed order, cost and gradient
 // scale (mia/template/ssd.hh:54-99,125-181; weight:
 // mia/template/fullcost.cxx:36-45).  Run by thread block `i` of either kernel.
+// This thread's share of an instance's SSD (and mask count) partials
+// [beg, end) as the finalize adds them: thread t < kBlock takes partials
+// t, t + kBlock, ..., four in flight before they are added (same order; a
+// missing one adds 0).
+template <bool COUNT>
+__device__ __forceinline__ void ssd_thread_share(int beg, int end, const double *__restrict__ ssd_part,
+                                                 const unsigned *__restrict__ cnt_part, double &s, unsigned &c)
+{
+    s = 0.0;
+    c = 0;
+    for (int k0 = beg + (int)threadIdx.x; k0 < end; k0 += 4 * kBlock) {
+        double a[4];
+        unsigned n[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            // unconditional loads from a clamped index (a load under a
+            // branch is waited for at the branch's end)
+            const int k = k0 + u * kBlock;
+            const int kc = min(k, end - 1);
+            const double av = ssd_part[kc];
+            a[u] = k < end ? av : 0.0;
+            if constexpr (COUNT) {
+                const unsigned nv = cnt_part[kc];
+                n[u] = k < end ? nv : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s += a[u];
+            if constexpr (COUNT)
+                c += n[u];
+        }
+    }
+}
+
+// Gradient component e3 of a 2D instance from the stage-1 partial sums: the
+// four y bands that touch its knot added in FP64 in this fixed order (the
+// adjoint of spline.cc:731-791 completed along y), times sc, plus the
+// instance's penalty gradient.  The four partial sums are loaded first
+// (unconditional loads from a clamped band; a missing one then reads as 0).
+__device__ __forceinline__ double reduce2_elem(const InstDesc &d, const float *__restrict__ P, long long e3,
+                                               double sc, const PenaltyIn &pen, int i)
+{
+    int comp, kx, ky;
+    if (2LL * d.cx * d.cy < (1LL << 31)) { // 32-bit index arithmetic (instance-uniform)
+        const unsigned ue = (unsigned)e3 >> 1;
+        comp = (int)(e3 & 1);
+        ky = (int)(ue / (unsigned)d.cx);
+        kx = (int)(ue - (unsigned)ky * (unsigned)d.cx);
+    } else {
+        comp = (int)(e3 % 2);
+        const long long e = e3 / 2;
+        kx = (int)(e % d.cx);
+        ky = (int)(e / d.cx);
+    }
+    float v[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int band = ky - t;
+        const int bc = min(max(band, 0), d.ncy - 1);
+        const float q = P[d.part_off + (long long)bc * 8 * d.cx + (t * d.cx + kx) * 2 + comp];
+        v[t] = band >= 0 && band < d.ncy ? q : 0.0f;
+    }
+    double g = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        g += v[t];
+    return grad_with_penalty(sc, g, pen, i, d.par_off + e3);
+}
+
 __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdIndex si,
                                                   const double *__restrict__ ssd_part,
                                                   const unsigned *__restrict__ cnt_part, int norm, float thr,
@@ -2103,30 +2194,9 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
     __shared__ unsigned redu[kBlock / 64];
     const bool z = si.zmode && d.zcell;
     const int beg = z ? si.zoff + si.s1z[i] : si.s1[i], end = z ? si.zoff + si.s1z[i + 1] : si.s1[i + 1];
-    double s = 0.0;
-    unsigned c = 0;
-    // four partials per thread in flight before they are added (same order;
-    // a missing one adds 0)
-    for (int k0 = beg + threadIdx.x; k0 < end; k0 += 4 * kBlock) {
-        double a[4];
-        unsigned n[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            // unconditional loads from a clamped index (a load under a
-            // branch is waited for at the branch's end)
-            const int k = k0 + u * kBlock;
-            const int kc = min(k, end - 1);
-            const double av = ssd_part[kc];
-            const unsigned nv = cnt_part[kc];
-            a[u] = k < end ? av : 0.0;
-            n[u] = k < end ? nv : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            s += a[u];
-            c += n[u];
-        }
-    }
+    double s;
+    unsigned c;
+    ssd_thread_share<true>(beg, end, ssd_part, cnt_part, s, c);
     s = block_sum(s, red);
     c = block_sum_u(c, redu);
     if (threadIdx.x == 0) {
@@ -2257,34 +2327,7 @@ __global__ void __launch_bounds__(kBlock)
         const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
         if (e3 >= 2 * ncoef)
             return;
-        int comp, kx, ky;
-        if (2 * ncoef < (1LL << 31)) { // 32-bit index arithmetic (block-uniform)
-            const unsigned ue = (unsigned)e3 >> 1;
-            comp = (int)(e3 & 1);
-            ky = (int)(ue / (unsigned)d.cx);
-            kx = (int)(ue - (unsigned)ky * (unsigned)d.cx);
-        } else {
-            comp = (int)(e3 % 2);
-            const long long e = e3 / 2;
-            kx = (int)(e % d.cx);
-            ky = (int)(e / d.cx);
-        }
-        // the four partial sums in flight first (unconditional loads from a
-        // clamped band, a missing one then reads as 0), then the sum in the
-        // fixed t order
-        float v[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int band = ky - t;
-            const int bc = min(max(band, 0), d.ncy - 1);
-            const float q = P[d.part_off + (long long)bc * 8 * d.cx + (t * d.cx + kx) * 2 + comp];
-            v[t] = band >= 0 && band < d.ncy ? q : 0.0f;
-        }
-        double g = 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            g += v[t];
-        grad[d.par_off + e3] = grad_with_penalty(sc, g, pen, i, d.par_off + e3);
+        grad[d.par_off + e3] = reduce2_elem(d, P, e3, sc, pen, i);
     }
 }
 
