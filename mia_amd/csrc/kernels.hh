@@ -1382,6 +1382,11 @@ __global__ void __launch_bounds__(kBlock)
     const int wxn = ts[d.tab_x + min(x0 + TX, d.nx) - 1] + 4 - kx0;
     const int wyn = ts[d.tab_y + min(y0 + TY, d.ny) - 1] + 4 - ky0;
     const double *P = params + d.par_off;
+    // this thread's x table entry and the clamp range, loaded now so that
+    // their latency overlaps the window's (they are used after two barriers)
+    const int tsx = ts[d.tab_x + xc];
+    const float4 wxv = tw[d.tab_x + xc];
+    const float2 cl = clamp[i];
     // the tile's y tables (rows clamped to the image) once into LDS
     if (threadIdx.x < TY) {
         const int yy = min(y0 + (int)threadIdx.x, d.ny - 1);
@@ -1414,8 +1419,7 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     __syncthreads();
-    const int sx = ts[d.tab_x + xc] - kx0;
-    const float4 wxv = tw[d.tab_x + xc];
+    const int sx = tsx - kx0;
     const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
     float px[RPT], py[RPT];
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
@@ -1475,7 +1479,6 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (!xvalid)
         return;
-    const float2 cl = clamp[i];
     float *Wi = W + d.img_off;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
