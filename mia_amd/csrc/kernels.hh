@@ -187,6 +187,8 @@ __device__ __forceinline__ double grad_with_penalty(double sc, double g, const P
 __device__ __forceinline__ int find_inst(const int *__restrict__ prefix, int n, int b)
 {
     const int lane = threadIdx.x & 63;
+    if (n == 1)
+        return 0;
     if (n <= 64) {
         const int v = prefix[min(lane, n - 1)];
         const unsigned long long m = __builtin_amdgcn_ballot_w64(lane < n && v <= b);
@@ -1842,13 +1844,10 @@ __global__ void __launch_bounds__(kBlock, 4)
     const int AS = L.acc_stride;
     float *acc_s = smem + L.acc;
     float4 *xw_s = (float4 *)(smem + L.xw);
-    int *xs_s = (int *)(smem + L.xs);
     int *cl_s = (int *)(smem + L.cl);
     float *S_s = smem + L.S;
-    for (int x = threadIdx.x; x < nx; x += kBlock) {
+    for (int x = threadIdx.x; x < nx; x += kBlock)
         xw_s[x] = tw[d.tab_x + x];
-        xs_s[x] = ts[d.tab_x + x];
-    }
     for (int k = threadIdx.x; k <= cx; k += kBlock)
         cl_s[k] = cells[d.cell_x + k];
     const int nrw = j1 - j0 + 2; // window rows j0 - 1 .. j1 (clamped to the image)

@@ -1718,11 +1718,13 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     // (k_lbfgs_pass1..3): the segment prefix counts none for the small ones
     std::vector<int> seg(n + 1, 0);
     int n_small = 0;
+    bool paired = true;
     for (int i = 0; i < n; ++i) {
         const InstDesc &d = c->inst[i];
         const long long np = (long long)c->dim * d.cx * d.cy * d.cz;
         const bool small = np <= kLbRegN;
         n_small += small;
+        paired = paired && np % 2 == 0 && d.par_off % 2 == 0;
         seg[i + 1] = seg[i] + (small ? 0 : (int)((np + kSeg - 1) / kSeg));
     }
     if (c->opt_memory != memory) {
@@ -1771,8 +1773,12 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
                 break;
             hipError_t e = hipSuccess;
             if (n_small > 0) {
-                // memory <= kLbRegMem: the stored pairs stay in registers for the whole step
-                auto kstep = memory <= kLbRegMem ? k_lbfgs_step_small<true> : k_lbfgs_step_small<false>;
+                // memory <= kLbRegMem: the stored pairs stay in registers for the whole step;
+                // paired: every instance's vectors start 16-byte aligned and have even
+                // lengths, so each thread moves adjacent pairs of elements
+                const bool reg = memory <= kLbRegMem;
+                auto kstep = reg ? (paired ? k_lbfgs_step_small<true, true> : k_lbfgs_step_small<true, false>)
+                                 : (paired ? k_lbfgs_step_small<false, true> : k_lbfgs_step_small<false, false>);
                 hipLaunchKernelGGL(kstep, dim3(n), dim3(kOptBlock), 0, c->stream,
                                    (const InstDesc *)c->d_inst, c->dim, ob, c->d_params, (const double *)c->d_grad,
                                    (const double *)c->d_costs, memory, maxeval, first_step);

@@ -723,12 +723,36 @@ __device__ __forceinline__ double block_max_all(double v, double *red, int &par)
 constexpr int kLbK = 4;
 constexpr long long kLbRegN = (long long)kLbK * kOptBlock;
 
+// This thread's k-th element of a vector: e = thread + k kOptBlock, or with
+// P (every vector of the instance starts 16-byte aligned and has an even
+// length) adjacent pairs, e = 2 thread + (k & 1) + (k >> 1) 2 kOptBlock,
+// loaded and stored two at a time (half the memory instructions, so that a
+// step's loads fit the wave's count of loads in flight)
+template <bool P>
+__device__ __forceinline__ long long lb_e(int k)
+{
+    return P ? 2LL * threadIdx.x + (k & 1) + (long long)(k >> 1) * 2 * kOptBlock
+             : (long long)threadIdx.x + (long long)k * kOptBlock;
+}
+
+template <bool P>
 __device__ __forceinline__ void lb_load(double *v, const double *__restrict__ src, long long n)
 {
+    if constexpr (P) {
 #pragma unroll
-    for (int k = 0; k < kLbK; ++k) {
-        const long long e = threadIdx.x + (long long)k * kOptBlock;
-        v[k] = e < n ? src[e] : 0.0;
+        for (int k = 0; k < kLbK; k += 2) {
+            const long long e = lb_e<P>(k);
+            // an unconditional load from a clamped pair (n is even)
+            const double2 q = *(const double2 *)(src + min(e, n - 2));
+            v[k] = e < n ? q.x : 0.0;
+            v[k + 1] = e < n ? q.y : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k) {
+            const long long e = lb_e<P>(k);
+            v[k] = e < n ? src[e] : 0.0;
+        }
     }
 }
 
@@ -759,17 +783,28 @@ __device__ __forceinline__ double part_max(const double (*part)[kPart], int v)
 }
 
 
+template <bool P>
 __device__ __forceinline__ void lb_store(double *dst, const double *v, long long n)
 {
+    if constexpr (P) {
 #pragma unroll
-    for (int k = 0; k < kLbK; ++k) {
-        const long long e = threadIdx.x + (long long)k * kOptBlock;
-        if (e < n)
-            dst[e] = v[k];
+        for (int k = 0; k < kLbK; k += 2) {
+            const long long e = lb_e<P>(k);
+            if (e < n)
+                *(double2 *)(dst + e) = make_double2(v[k], v[k + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kLbK; ++k) {
+            const long long e = lb_e<P>(k);
+            if (e < n)
+                dst[e] = v[k];
+        }
     }
 }
 
 // Pairs j0 .. j0 + kLbP - 1 (those below kk) from ring slot head into sc, yc
+template <bool P>
 __device__ __forceinline__ void lb_load_group(double (*sc)[kLbK], double (*yc)[kLbK], int j0, int kk, int head,
                                               int memory, const double *S, const double *Y, long long n)
 {
@@ -777,8 +812,8 @@ __device__ __forceinline__ void lb_load_group(double (*sc)[kLbK], double (*yc)[k
     for (int t = 0; t < kLbP; ++t)
         if (j0 + t < kk) {
             const int slot = (head + j0 + t) % memory;
-            lb_load(sc[t], S + (long long)slot * n, n);
-            lb_load(yc[t], Y + (long long)slot * n, n);
+            lb_load<P>(sc[t], S + (long long)slot * n, n);
+            lb_load<P>(yc[t], Y + (long long)slot * n, n);
         }
 }
 
@@ -792,6 +827,7 @@ constexpr int kLbRegMem = 10;
 // Every load is issued unconditionally (slots beyond kk repeat the newest
 // stored pair, or slot head when none is stored: in bounds, cached, unused),
 // so that none waits at the end of a branch.
+template <bool P>
 __device__ __forceinline__ void lb_load_pairs(double (*ps)[kLbK], double (*py)[kLbK], int kk, int head, int memory,
                                               const double *S, const double *Y, long long n)
 {
@@ -799,14 +835,15 @@ __device__ __forceinline__ void lb_load_pairs(double (*ps)[kLbK], double (*py)[k
     for (int j = 0; j < kLbRegMem; ++j) {
         const int jc = j < kk ? j : (kk > 0 ? kk - 1 : 0);
         const int slot = (head + jc) % memory;
-        lb_load(ps[j], S + (long long)slot * n, n);
-        lb_load(py[j], Y + (long long)slot * n, n);
+        lb_load<P>(ps[j], S + (long long)slot * n, n);
+        lb_load<P>(py[j], Y + (long long)slot * n, n);
     }
 }
 
 // Accepted step, vectors in registers (xt, x, gv, g: this thread's elements of XT, X, GT, G):
 // X <- XT, G <- GT, the new pair s = XT - X, y = GT - G left in sv, yv, and the wave partials of every product of kPart written to part[wave][.] (complete after the
 // caller's barrier).  kk pairs are stored from ring slot head.
+template <bool P>
 __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int kk, int head, const double *S,
                                                   const double *Y, double *X, double *G, const double *xt,
                                                   const double *x, const double *gv, const double *g, double *sv,
@@ -824,8 +861,8 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
         yg += yv[k] * gv[k];
         gm = fmax(gm, fabs(gv[k]));
     }
-    lb_store(X, xt, n);
-    lb_store(G, gv, n);
+    lb_store<P>(X, xt, n);
+    lb_store<P>(G, gv, n);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *tb = tbuf + w * 16 * kTbRow;
     {
@@ -839,7 +876,7 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
     }
     for (int j0 = 0; j0 < kk; j0 += kLbP) {
         double sc[kLbP][kLbK], yc[kLbP][kLbK];
-        lb_load_group(sc, yc, j0, kk, head, memory, S, Y, n);
+        lb_load_group<P>(sc, yc, j0, kk, head, memory, S, Y, n);
         double dd[4 * kLbP];
 #pragma unroll
         for (int t = 0; t < kLbP; ++t) {
@@ -868,6 +905,7 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
 
 // lbfgs_accept_pass with the kk stored pairs already in registers (ps, py,
 // oldest first): the same products, summed in the same order.
+template <bool P>
 __device__ __forceinline__ void lbfgs_accept_pass_reg(long long n, int kk, double *X, double *G, const double *xt,
                                                       const double *x, const double *gv, const double *g,
                                                       double *sv, double *yv, const double (*ps)[kLbK],
@@ -885,8 +923,8 @@ __device__ __forceinline__ void lbfgs_accept_pass_reg(long long n, int kk, doubl
         yg += yv[k] * gv[k];
         gm = fmax(gm, fabs(gv[k]));
     }
-    lb_store(X, xt, n);
-    lb_store(G, gv, n);
+    lb_store<P>(X, xt, n);
+    lb_store<P>(G, gv, n);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *tb = tbuf + w * 16 * kTbRow;
     {
@@ -987,6 +1025,7 @@ __device__ __forceinline__ void block_compact_coeffs(int memory, int k, int head
 
 // D <- -H g from the coefficients (all threads; g: this thread's gradient elements, sc / yc: the first
 // group of pairs, already loaded), also left in dv; returns g.D
+template <bool P>
 __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memory, int k, int head,
                                                           const double *S, const double *Y, const double *g,
                                                           double (*sc)[kLbK], double (*yc)[kLbK], double *D,
@@ -998,7 +1037,7 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
         dv[k2] = gamma * g[k2];
     for (int j0 = 0; j0 < k; j0 += kLbP) {
         if (j0 > 0)
-            lb_load_group(sc, yc, j0, k, head, memory, S, Y, n);
+            lb_load_group<P>(sc, yc, j0, k, head, memory, S, Y, n);
 #pragma unroll
         for (int t = 0; t < kLbP; ++t)
             if (j0 + t < k) {
@@ -1014,11 +1053,11 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
 #pragma unroll
     for (int k2 = 0; k2 < kLbK; ++k2) {
         dv[k2] = -dv[k2];
-        const long long e = threadIdx.x + (long long)k2 * kOptBlock;
+        const long long e = lb_e<P>(k2);
         if (e < n)
             gd += g[k2] * dv[k2];
     }
-    lb_store(D, dv, n);
+    lb_store<P>(D, dv, n);
     return block_sum_all(gd, red, rpar);
 }
 
@@ -1026,6 +1065,7 @@ __device__ __forceinline__ double lbfgs_compact_direction(long long n, int memor
 // lbfgs_compact_direction with the pairs in registers: the stored pairs
 // drop .. kk - 1 (ps, py) and, when a pair was stored by this step, the new
 // one (sv, yv), oldest first, added in the same order.
+template <bool P>
 __device__ __forceinline__ double lbfgs_compact_direction_reg(long long n, int kk, int drop, bool fresh, int knew,
                                                               const double *g, const double (*ps)[kLbK],
                                                               const double (*py)[kLbK], const double *sv,
@@ -1058,11 +1098,11 @@ __device__ __forceinline__ double lbfgs_compact_direction_reg(long long n, int k
 #pragma unroll
     for (int k2 = 0; k2 < kLbK; ++k2) {
         dv[k2] = -dv[k2];
-        const long long e = threadIdx.x + (long long)k2 * kOptBlock;
+        const long long e = lb_e<P>(k2);
         if (e < n)
             gd += g[k2] * dv[k2];
     }
-    lb_store(D, dv, n);
+    lb_store<P>(D, dv, n);
     return block_sum_all(gd, red, rpar);
 }
 
@@ -1071,7 +1111,7 @@ __device__ __forceinline__ double lbfgs_compact_direction_reg(long long n, int k
 // the segmented passes, with the ring of `memory` slots, products summed
 // over the block's waves.  REG (memory <= kLbRegMem): the stored pairs are read
 // once into registers and serve both passes.
-template <bool REG>
+template <bool REG, bool P>
 __global__ void __launch_bounds__(kOptBlock)
     k_lbfgs_step_small(const InstDesc *__restrict__ inst, int dim, OptBuffers ob, double *__restrict__ xt,
                        const double *__restrict__ gt, const double *__restrict__ costs, int memory, int maxeval,
@@ -1103,10 +1143,10 @@ __global__ void __launch_bounds__(kOptBlock)
     double rxt[kLbK], rx[kLbK], rgt[kLbK], rg[kLbK];
     double sv[kLbK], yv[kLbK];                              // the new pair (accepted step)
     double ps[REG ? kLbRegMem : 1][kLbK], py[REG ? kLbRegMem : 1][kLbK]; // stored pairs (REG)
-    lb_load(rxt, XT, n);
-    lb_load(rgt, GT, n);
-    lb_load(rx, X, n);
-    lb_load(rg, G, n);
+    lb_load<P>(rxt, XT, n);
+    lb_load<P>(rgt, GT, n);
+    lb_load<P>(rx, X, n);
+    lb_load<P>(rg, G, n);
     const double ft = costs[i];
     if (threadIdx.x == 0)
         s = ob.st[i];
@@ -1119,9 +1159,9 @@ __global__ void __launch_bounds__(kOptBlock)
     bool finish = false;
     bool backtrack = false;
     if (s.phase == 0) {
-        lb_store(X, rxt, n);
-        lb_store(G, rgt, n);
-        lb_store(BX, rxt, n);
+        lb_store<P>(X, rxt, n);
+        lb_store<P>(G, rgt, n);
+        lb_store<P>(BX, rxt, n);
         double gmax = 0.0;
 #pragma unroll
         for (int k = 0; k < kLbK; ++k)
@@ -1144,7 +1184,7 @@ __global__ void __launch_bounds__(kOptBlock)
         const double f0 = s.f, a0 = s.alpha, gd0 = s.gd;
         const bool better = finite && ft < s.best_f;
         if (better)
-            lb_store(BX, rxt, n);
+            lb_store<P>(BX, rxt, n);
         __syncthreads();
         if (threadIdx.x == 0) {
             s.evals += 1;
@@ -1156,15 +1196,15 @@ __global__ void __launch_bounds__(kOptBlock)
         if (armijo) {
             const int kk = s.k, head = s.head;
             if constexpr (REG)
-                lb_load_pairs(ps, py, kk, head, memory, S, Y, n);
+                lb_load_pairs<P>(ps, py, kk, head, memory, S, Y, n);
             for (int t = threadIdx.x; t < memory * memory; t += kOptBlock) {
                 mSY[t] = SYi[t];
                 mYY[t] = YYi[t];
             }
             if constexpr (REG)
-                lbfgs_accept_pass_reg(n, kk, X, G, rxt, rx, rgt, rg, sv, yv, ps, py, part, tbuf);
+                lbfgs_accept_pass_reg<P>(n, kk, X, G, rxt, rx, rgt, rg, sv, yv, ps, py, part, tbuf);
             else
-                lbfgs_accept_pass(n, memory, kk, head, S, Y, X, G, rxt, rx, rgt, rg, sv, yv, part, tbuf);
+                lbfgs_accept_pass<P>(n, memory, kk, head, S, Y, X, G, rxt, rx, rgt, rg, sv, yv, part, tbuf);
             kold = kk;
             __syncthreads();
             const double sy = part_total(part, kPSY), yy = part_total(part, kPYY), ss = part_total(part, kPSS);
@@ -1173,8 +1213,8 @@ __global__ void __launch_bounds__(kOptBlock)
                 fresh = true;
                 drop = kk < memory ? 0 : 1;
                 const int ns = kk < memory ? (head + kk) % memory : head;
-                lb_store(S + (long long)ns * n, sv, n);
-                lb_store(Y + (long long)ns * n, yv, n);
+                lb_store<P>(S + (long long)ns * n, sv, n);
+                lb_store<P>(Y + (long long)ns * n, yv, n);
                 // the new pair's products with the pairs that stay
                 if (threadIdx.x < kk - drop) {
                     const int jo = threadIdx.x + drop;
@@ -1219,11 +1259,11 @@ __global__ void __launch_bounds__(kOptBlock)
             finish = true;
         } else {
             double xv[kLbK], dv[kLbK];
-            lb_load(dv, D, n);
+            lb_load<P>(dv, D, n);
 #pragma unroll
             for (int k = 0; k < kLbK; ++k)
                 xv[k] = rx[k] + an * dv[k];
-            lb_store(XT, xv, n);
+            lb_store<P>(XT, xv, n);
             __syncthreads();
             if (threadIdx.x == 0)
                 s.alpha = an;
@@ -1234,7 +1274,7 @@ __global__ void __launch_bounds__(kOptBlock)
         double dv[kLbK]; // the direction's elements of this thread
         double sc[REG ? 1 : kLbP][kLbK], yc[REG ? 1 : kLbP][kLbK];
         if constexpr (!REG)
-            lb_load_group(sc, yc, 0, k, head, memory, S, Y, n);
+            lb_load_group<P>(sc, yc, 0, k, head, memory, S, Y, n);
         double gamma = 1.0;
         if (k > 0) {
             const int sl = (head + k - 1) % memory;
@@ -1245,20 +1285,20 @@ __global__ void __launch_bounds__(kOptBlock)
         }
         double gd;
         if constexpr (REG)
-            gd = lbfgs_compact_direction_reg(n, kold, drop, fresh, k, rgt, ps, py, sv, yv, D, dv, gamma, cu, cw, red,
+            gd = lbfgs_compact_direction_reg<P>(n, kold, drop, fresh, k, rgt, ps, py, sv, yv, D, dv, gamma, cu, cw, red,
                                              rpar);
         else
-            gd = lbfgs_compact_direction(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
+            gd = lbfgs_compact_direction<P>(n, memory, k, head, S, Y, rgt, sc, yc, D, dv, gamma, cu, cw, red, rpar);
         if (!(gd < 0.0)) { // not a descent direction: restart from steepest descent
             double g2 = 0.0;
 #pragma unroll
             for (int k2 = 0; k2 < kLbK; ++k2) {
                 dv[k2] = -rgt[k2];
-                const long long e = threadIdx.x + (long long)k2 * kOptBlock;
+                const long long e = lb_e<P>(k2);
                 if (e < n)
                     g2 += rgt[k2] * dv[k2];
             }
-            lb_store(D, dv, n);
+            lb_store<P>(D, dv, n);
             gd = block_sum_all(g2, red, rpar);
             if (threadIdx.x == 0) {
                 s.k = 0;
@@ -1281,7 +1321,7 @@ __global__ void __launch_bounds__(kOptBlock)
 #pragma unroll
         for (int k2 = 0; k2 < kLbK; ++k2)
             xv[k2] = rxt[k2] + alpha * dv[k2];
-        lb_store(XT, xv, n);
+        lb_store<P>(XT, xv, n);
         __syncthreads();
         if (threadIdx.x == 0) {
             s.alpha = alpha;
@@ -1291,9 +1331,9 @@ __global__ void __launch_bounds__(kOptBlock)
     }
     if (finish) {
         double bx[kLbK];
-        lb_load(bx, BX, n);
+        lb_load<P>(bx, BX, n);
         __syncthreads();
-        lb_store(XT, bx, n);
+        lb_store<P>(XT, bx, n);
         if (threadIdx.x == 0) {
             s.phase = 2;
             ob.done[i] = 1;
