@@ -188,14 +188,25 @@ __device__ __forceinline__ void seg_partials_out(const double (*wp)[kPart], int 
 
 // Totals over an instance's segments in ascending order (thread v: value v);
 // every block of the instance computes the same totals.
+// The loads go out 16 segments at a time (unconditional, from clamped
+// segment indices) before they are added in ascending order: a load per
+// added term would wait out the memory latency once per segment.
 __device__ __forceinline__ void instance_totals(const double *__restrict__ part, int nseg, int nvals, double *tot)
 {
+    constexpr int U = 16;
     const int v = threadIdx.x;
     if (v < nvals) {
+        const bool mx = v == kPGMAX;
         double t = part[v];
-        for (int sgi = 1; sgi < nseg; ++sgi) {
-            const double x = part[(long long)sgi * kPart + v];
-            t = v == kPGMAX ? fmax(t, x) : t + x;
+        for (int s0 = 1; s0 < nseg; s0 += U) {
+            double x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                x[u] = part[(long long)min(s0 + u, nseg - 1) * kPart + v];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (s0 + u < nseg)
+                    t = mx ? fmax(t, x[u]) : t + x[u];
         }
         tot[v] = t;
     }
