@@ -914,13 +914,54 @@ __device__ __forceinline__ void lbfgs_accept_pass(long long n, int memory, int k
     }
 }
 
+// Block sums of many values per thread for the register-resident step, in a
+// fixed order: each quad of lanes adds its four values (DPP), the quad's
+// first lane stores the sum into st[value][quad] (kQs = 128 quads per
+// block); after a barrier, collect_quad_sums adds each value's quad sums in
+// eight chunks of 16 into part[chunk][value], which part_total / part_max
+// then add in chunk order.  One barrier for any number of values, where a
+// transposed sum of 16 values takes two.
+constexpr int kQs = kOptBlock / 4;
+constexpr int kQsVals = kPPair + 4 * kLbRegMem; // values of a step with kLbRegMem pairs
+
+template <bool MAX>
+__device__ __forceinline__ void quad_stage(double v, double *st, int val)
+{
+    const double id = MAX ? -INFINITY : 0.0;
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dpp64<0xB1, 0xf>(v, id));
+    v = op(v, dpp64<0x4E, 0xf>(v, id));
+    if ((threadIdx.x & 3) == 0)
+        st[val * kQs + (threadIdx.x >> 2)] = v;
+}
+
+__device__ __forceinline__ void collect_quad_sums(const double *st, int nvals, double (*part)[kPart])
+{
+    static_assert(kOptBlock / 64 == 8 && kQs == 8 * 16, "eight chunks of 16 quad sums");
+    for (int t = threadIdx.x; t < nvals * 8; t += kOptBlock) {
+        const int v = t >> 3, c = t & 7;
+        const double *q = st + v * kQs + c * 16;
+        double x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            x[j] = q[j];
+        double s = x[0];
+#pragma unroll
+        for (int j = 1; j < 16; ++j)
+            s = v == kPGMAX ? fmax(s, x[j]) : s + x[j];
+        part[c][v] = s;
+    }
+}
+
 // lbfgs_accept_pass with the kk stored pairs already in registers (ps, py,
-// oldest first): the same products, summed in the same order.
+// oldest first): the same products, each pair's four computed and staged as
+// soon as its loads arrive, summed over the block by quads (quad_stage,
+// collect_quad_sums); part[chunk][.] is complete after the caller's barrier.
 template <bool P>
 __device__ __forceinline__ void lbfgs_accept_pass_reg(long long n, int kk, double *X, double *G, const double *xt,
                                                       const double *x, const double *gv, const double *g,
                                                       double *sv, double *yv, const double (*ps)[kLbK],
-                                                      const double (*py)[kLbK], double (*part)[kPart], double *tbuf)
+                                                      const double (*py)[kLbK], double (*part)[kPart], double *st)
 {
     double sy = 0.0, yy = 0.0, ss = 0.0, sg = 0.0, yg = 0.0, gm = 0.0;
 #pragma unroll
@@ -936,47 +977,30 @@ __device__ __forceinline__ void lbfgs_accept_pass_reg(long long n, int kk, doubl
     }
     lb_store<P>(X, xt, n);
     lb_store<P>(G, gv, n);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double *tb = tbuf + w * 16 * kTbRow;
-    {
-        const double h[16] = {sy, yy, ss, sg, yg, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const double t = transpose_sum16(h, tb);
-        gm = wave_reduce63<true>(gm);
-        if ((lane & 3) == 3 && lane < 4 * kPGMAX)
-            part[w][lane >> 2] = t;
-        if (lane == 63)
-            part[w][kPGMAX] = gm;
-    }
-    // two pairs per transposed sum (values 4 t + q: pair j0 + t, product q;
-    // the other eight are zero), which keeps the register file within
-    // bounds next to the pairs
+    quad_stage<false>(sy, st, kPSY);
+    quad_stage<false>(yy, st, kPYY);
+    quad_stage<false>(ss, st, kPSS);
+    quad_stage<false>(sg, st, kPSG);
+    quad_stage<false>(yg, st, kPYG);
+    quad_stage<true>(gm, st, kPGMAX);
 #pragma unroll
-    for (int j0 = 0; j0 < kLbRegMem; j0 += 2) {
-        if (j0 < kk) {
-            double dd[16];
+    for (int j = 0; j < kLbRegMem; ++j)
+        if (j < kk) {
+            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-                if (t < 2 && j0 + t < kLbRegMem) {
-#pragma unroll
-                    for (int k = 0; k < kLbK; ++k) {
-                        d0 += ps[j0 + t][k] * yv[k];
-                        d1 += py[j0 + t][k] * yv[k];
-                        d2 += ps[j0 + t][k] * gv[k];
-                        d3 += py[j0 + t][k] * gv[k];
-                    }
-                }
-                dd[4 * t] = d0;
-                dd[4 * t + 1] = d1;
-                dd[4 * t + 2] = d2;
-                dd[4 * t + 3] = d3;
+            for (int k = 0; k < kLbK; ++k) {
+                d0 += ps[j][k] * yv[k];
+                d1 += py[j][k] * yv[k];
+                d2 += ps[j][k] * gv[k];
+                d3 += py[j][k] * gv[k];
             }
-            const double t = transpose_sum16(dd, tb);
-            const int j = j0 + (lane >> 4);
-            if ((lane & 3) == 3 && (lane >> 4) < 2 && j < kk)
-                part[w][kPPair + 4 * j0 + (lane >> 2)] = t;
+            quad_stage<false>(d0, st, kPPair + 4 * j);
+            quad_stage<false>(d1, st, kPPair + 4 * j + 1);
+            quad_stage<false>(d2, st, kPPair + 4 * j + 2);
+            quad_stage<false>(d3, st, kPPair + 4 * j + 3);
         }
-    }
+    __syncthreads();
+    collect_quad_sums(st, kPPair + 4 * kk, part);
 }
 
 // Coefficients of the compact direction for k >= 1 pairs from ring slot head (wave 0, lane i for pair i,
@@ -1132,7 +1156,8 @@ __global__ void __launch_bounds__(kOptBlock)
     __shared__ double part[kOptBlock / 64][kPart];
     __shared__ double mSY[kMaxMemory * kMaxMemory], mYY[kMaxMemory * kMaxMemory];
     __shared__ double cu[kMaxMemory], cw[kMaxMemory];
-    __shared__ double tbuf[(kOptBlock / 64) * 16 * kTbRow];
+    __shared__ double tbuf[REG ? 1 : (kOptBlock / 64) * 16 * kTbRow]; // transposed sums (pairs from memory)
+    __shared__ double st[REG ? kQsVals * kQs : 1];                      // quad sums (pairs in registers)
     int rpar = 0; // which half of red the next block reduction uses
     __shared__ OptState s;
     const int i = blockIdx.x;
@@ -1213,7 +1238,7 @@ __global__ void __launch_bounds__(kOptBlock)
                 mYY[t] = YYi[t];
             }
             if constexpr (REG)
-                lbfgs_accept_pass_reg<P>(n, kk, X, G, rxt, rx, rgt, rg, sv, yv, ps, py, part, tbuf);
+                lbfgs_accept_pass_reg<P>(n, kk, X, G, rxt, rx, rgt, rg, sv, yv, ps, py, part, st);
             else
                 lbfgs_accept_pass<P>(n, memory, kk, head, S, Y, X, G, rxt, rx, rgt, rg, sv, yv, part, tbuf);
             kold = kk;
