@@ -233,3 +233,31 @@ def test_optimize_stream_and_graph_launches_agree(ipg):
         again = c.optimize(np.zeros(c.n_params_total), maxeval=20, iters_per_graph=20)[:3]
     for a, b in zip(ref, again):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("memory", [10, 14])
+def test_device_lbfgs_memory_paths_follow_host(memory):
+    """The one-block device step holds up to 10 stored pairs in registers and
+    reads more from memory: both paths against the host L-BFGS (same
+    algorithm, other summation order) over enough evaluations to store more
+    than 10 pairs, on 2D frames (vectors moved in adjacent pairs) and on 3D
+    volumes of an odd parameter count (one element per thread and slot)."""
+    from mia_amd import series
+    frames2 = _frames(3, (60, 110), 61)
+    frames3 = _frames(2, (22, 26), 65, 3)
+    sizes3 = [3 * np.prod([mia_amd_csize3(s) for s in f[1].shape]) for f in frames3]
+    assert any(n % 2 == 1 for n in sizes3)
+    for frames in (frames2, frames3):
+        with series.SeriesRegistration(frames, rate=8.0) as s:
+            desc = "lbfgs:maxiter=28,memory=%d" % memory
+            ph, ch, eh, _ = s.run(desc, series.BATCHED)
+            pd, cd, ed, _ = s.run(desc, series.DEVICE_BATCHED)
+        print("memory", memory, "evaluations", eh, ed, "cost rel", np.abs(cd - ch) / np.abs(ch))
+        assert np.array_equal(eh, ed)
+        np.testing.assert_allclose(cd, ch, rtol=1e-7)
+        assert np.abs(pd - ph).max() <= 1e-6 * max(1.0, np.abs(ph).max())
+
+
+def mia_amd_csize3(n):
+    import mia_amd
+    return mia_amd.spline_csize(n, 8.0, 3)
