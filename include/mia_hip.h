@@ -168,6 +168,24 @@ int mia_hip_graph_run(mia_hip_ctx *ctx);
 int mia_hip_optimize(mia_hip_ctx *ctx, const double *x0, int maxeval, int memory, double first_step,
                      int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds);
 
+/* mia_hip_optimize on n_ctx contexts of one device at once: context k
+ * starts from x0[k] and writes x_out[k], costs_out[k], evals_out[k] (any of
+ * the three arrays, or an entry, may be NULL).  The contexts' chunks of
+ * iterations are queued alternately on two streams that the library keeps
+ * per device, created with the device's first context so that HIP gives
+ * them hardware queues of their own: one context's optimizer step, which
+ * occupies a block per instance, runs beside the other context's
+ * evaluation, which spreads over the whole GPU.  Every context's results are
+ * bitwise those mia_hip_optimize gives it alone (an instance's trajectory
+ * depends on nothing outside it).  Chunks are queued as stream launches
+ * whatever mia_hip_set_optimize_launch chose; with more than two contexts,
+ * contexts k and k + 2 share a stream.  seconds: wall time of the joint
+ * iteration loop.  No counterpart in the reference (its minimizers run on
+ * the host, one registration at a time: src/2dmany2one-nonrigid.cc:185). */
+int mia_hip_optimize_many(mia_hip_ctx *const *ctx, int n_ctx, const double *const *x0, int maxeval, int memory,
+                          double first_step, int iters_per_graph, double *const *x_out, double *const *costs_out,
+                          int *const *evals_out, double *seconds);
+
 /* How mia_hip_optimize queues its chunks of iterations.
  * MIA_HIP_LAUNCH_STREAM (the default): the kernels of each chunk are launched
  * on the context's stream, the next chunk queued while the host waits for the

@@ -24,6 +24,8 @@ extern "C" {
 #define MIA_HOST_PER_FRAME 2     /* one context per frame, frames registered one after the other */
 #define MIA_HOST_DEVICE_BATCHED 3   /* on-device L-BFGS, all frames in one graph per iteration block */
 #define MIA_HOST_DEVICE_PER_FRAME 4 /* on-device L-BFGS, one context (and graph) per frame, one after the other */
+#define MIA_HOST_DEVICE_SPLIT 5     /* on-device L-BFGS, the frames in two batches of about equal pixel counts
+                                       whose iterations overlap (mia_hip_optimize_many); one frame: one batch */
 
 typedef struct mia_host_series mia_host_series;
 

@@ -50,6 +50,9 @@ SIGNATURES = {
     "mia_hip_graph_run": (ctypes.c_int, [_vp]),
     "mia_hip_optimize": (ctypes.c_int, [_vp, _dbl_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         _vp, _vp, _vp, _dbl_p]),
+    "mia_hip_optimize_many": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_dbl_p), ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_dbl_p),
+                                             ctypes.POINTER(_dbl_p), ctypes.POINTER(_c_int_p), _dbl_p]),
     "mia_hip_set_optimize_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mia_hip_synchronize": (ctypes.c_int, [_vp]),
     "mia_hip_get_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),

@@ -207,6 +207,34 @@ class HipContext:
                                        ctypes.byref(sec)))
         return x, costs, evals, sec.value
 
+    @staticmethod
+    def optimize_many(contexts, x0s, maxeval=100, memory=10, first_step=1.0, iters_per_graph=10):
+        """On-device L-BFGS on several contexts of one device at once, their
+        iterations overlapping (mia_hip_optimize_many); returns one
+        (x, costs, evals) per context and the wall time of the joint loop."""
+        if len(contexts) != len(x0s) or not contexts:
+            raise ValueError("need one starting point per context")
+        L = contexts[0]._L
+        n = len(contexts)
+        x0s = [np.ascontiguousarray(x, dtype=np.float64) for x in x0s]
+        for c, x in zip(contexts, x0s):
+            if x.size != c.n_params_total:
+                raise ValueError("expected %d parameters, got %d" % (c.n_params_total, x.size))
+        xs = [np.empty_like(x) for x in x0s]
+        costs = [np.empty(c.n_inst) for c in contexts]
+        evals = [np.empty(c.n_inst, dtype=np.int32) for c in contexts]
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        hs = (ctypes.c_void_p * n)(*[c._h.value for c in contexts])
+        ax0 = (dp * n)(*[_f64p(x) for x in x0s])
+        ax = (dp * n)(*[_f64p(x) for x in xs])
+        ac = (dp * n)(*[_f64p(x) for x in costs])
+        ae = (ip * n)(*[e.ctypes.data_as(ip) for e in evals])
+        sec = ctypes.c_double()
+        check(L.mia_hip_optimize_many(hs, n, ax0, int(maxeval), int(memory), float(first_step), int(iters_per_graph),
+                                      ax, ac, ae, ctypes.byref(sec)))
+        return [(xs[k], costs[k], evals[k]) for k in range(n)], sec.value
+
     def set_optimize_launch(self, mode):
         """How optimize() queues its chunks: "stream" (direct launches, the default) or "graph"."""
         modes = {"stream": 0, "graph": 1}

@@ -6,6 +6,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdlib>
 #include <map>
@@ -78,6 +79,41 @@ void stream_release(int device, hipStream_t s)
     }
     (void)hipStreamDestroy(s);
 }
+
+// Streams of mia_hip_optimize_many, per device: created together with the
+// device's first context (the first streams of the library on that device),
+// so that HIP gives them hardware queues of their own, and kept until the
+// process ends.
+constexpr int kJointStreams = 2;
+namespace {
+std::mutex g_joint_mu;
+std::vector<std::pair<int, std::array<hipStream_t, kJointStreams>>> g_joint;
+} // namespace
+
+hipError_t joint_streams(int device, std::array<hipStream_t, kJointStreams> *out)
+{
+    std::lock_guard<std::mutex> lock(g_joint_mu);
+    for (auto &j : g_joint)
+        if (j.first == device) {
+            if (out)
+                *out = j.second;
+            return hipSuccess;
+        }
+    std::array<hipStream_t, kJointStreams> s{};
+    for (int k = 0; k < kJointStreams; ++k) {
+        hipError_t e = hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (int j = 0; j < k; ++j)
+                (void)hipStreamDestroy(s[j]);
+            return e;
+        }
+    }
+    g_joint.emplace_back(device, s);
+    if (out)
+        *out = s;
+    return hipSuccess;
+}
+
 } // namespace mia_hip_rt
 
 namespace {
@@ -1110,9 +1146,13 @@ int mia_hip_ctx_create(int device, mia_hip_ctx **out)
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(MIA_HIP_ERR_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library targets gfx950");
+    // the joint optimizer's streams first (see joint_streams)
+    hipError_t e = mia_hip_rt::joint_streams(device, nullptr);
+    if (e != hipSuccess)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     mia_hip_ctx *c = new mia_hip_ctx();
     c->device = device;
-    hipError_t e = mia_hip_rt::stream_acquire(device, &c->stream);
+    e = mia_hip_rt::stream_acquire(device, &c->stream);
     if (e != hipSuccess) {
         delete c;
         return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -1699,8 +1739,26 @@ int mia_hip_graph_run(mia_hip_ctx *c)
     return MIA_HIP_OK;
 }
 
-int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, double first_step,
-                     int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds)
+namespace {
+
+// One context's device L-BFGS: the step kernels it needs, its chunk of
+// iterations and its results (mia_hip_optimize, mia_hip_optimize_many).
+struct OptRun {
+    mia_hip_ctx *c = nullptr;
+    int maxeval = 0, memory = 0, ipg = 1;
+    double first_step = 1.0;
+    std::vector<int> seg; // segment prefix of the large instances' passes
+    int n_small = 0;      // instances stepped by one block each
+    bool paired = true;   // every instance's vectors 16-byte aligned, even lengths
+    OptBuffers ob{};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool active = true;
+};
+
+// Optimizer buffers (re)allocated for `memory`, the starting point and the
+// initial state uploaded on the context's stream.
+int opt_run_prepare(OptRun &r, mia_hip_ctx *c, const double *x0, int maxeval, int memory, double first_step,
+                    int iters_per_graph)
 {
     int st = check_ready(c);
     if (st)
@@ -1710,23 +1768,27 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
     if (maxeval < 1 || memory < 1 || memory > kMaxMemory || !(first_step > 0.0) || iters_per_graph < 1)
         return fail(MIA_HIP_ERR_ARG, "optimize: need maxeval >= 1, 1 <= memory <= 16, first_step > 0, "
                                      "iters_per_graph >= 1");
+    r.c = c;
+    r.maxeval = maxeval;
+    r.memory = memory;
+    r.first_step = first_step;
+    r.ipg = iters_per_graph;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const int n = c->n_inst;
     // instances of up to kLbRegN parameters step in one block each
     // (k_lbfgs_step_small), larger ones in segments of kSeg parameters
     // (k_lbfgs_pass1..3): the segment prefix counts none for the small ones
-    std::vector<int> seg(n + 1, 0);
-    int n_small = 0;
-    bool paired = true;
+    r.seg.assign(n + 1, 0);
     for (int i = 0; i < n; ++i) {
         const InstDesc &d = c->inst[i];
         const long long np = (long long)c->dim * d.cx * d.cy * d.cz;
         const bool small = np <= kLbRegN;
-        n_small += small;
-        paired = paired && np % 2 == 0 && d.par_off % 2 == 0;
-        seg[i + 1] = seg[i] + (small ? 0 : (int)((np + kSeg - 1) / kSeg));
+        r.n_small += small;
+        r.paired = r.paired && np % 2 == 0 && d.par_off % 2 == 0;
+        r.seg[i + 1] = r.seg[i] + (small ? 0 : (int)((np + kSeg - 1) / kSeg));
     }
+    const std::vector<int> &seg = r.seg;
     if (c->opt_memory != memory) {
         void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone,
                        c->d_ost, c->d_oSY, c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
@@ -1761,158 +1823,195 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
         (st = stage_upload(c, c->d_odone, zeros.data(), sizeof(int) * n)) ||
         (st = stage_upload(c, c->d_oseg, seg.data(), sizeof(int) * (n + 1))))
         return st;
-    OptBuffers ob{c->d_oX,   c->d_oG,   c->d_oD,   c->d_oBX,    c->d_oS,     c->d_oY,   c->d_obest, c->d_odone,
-                  c->d_ost,  c->d_ost2, c->d_oSY,  c->d_oYY,    c->d_oPART,  c->d_oPART2, c->d_oseg};
-    // one chunk: iters_per_graph iterations (evaluation, then the step's
-    // kernels), then the done flags copied to host slot `slot`
-    auto enqueue_chunk = [&](int slot) -> int {
-        int rc = MIA_HIP_OK;
-        for (int it = 0; it < iters_per_graph && rc == MIA_HIP_OK; ++it) {
-            rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
-            if (rc != MIA_HIP_OK)
-                break;
-            hipError_t e = hipSuccess;
-            if (n_small > 0) {
-                // memory <= kLbRegMem: the stored pairs stay in registers for the whole step;
-                // paired: every instance's vectors start 16-byte aligned and have even
-                // lengths, so each thread moves adjacent pairs of elements
-                const bool reg = memory <= kLbRegMem;
-                auto kstep = reg ? (paired ? k_lbfgs_step_small<true, true> : k_lbfgs_step_small<true, false>)
-                                 : (paired ? k_lbfgs_step_small<false, true> : k_lbfgs_step_small<false, false>);
-                hipLaunchKernelGGL(kstep, dim3(n), dim3(kOptBlock), 0, c->stream,
-                                   (const InstDesc *)c->d_inst, c->dim, ob, c->d_params, (const double *)c->d_grad,
-                                   (const double *)c->d_costs, memory, maxeval, first_step);
-                e = hipGetLastError();
-            }
-            if (e == hipSuccess && seg[n] > 0) {
-                hipLaunchKernelGGL(k_lbfgs_pass1, dim3(seg[n]), dim3(kSegT), 0, c->stream,
+    r.ob = OptBuffers{c->d_oX,  c->d_oG,   c->d_oD,   c->d_oBX,   c->d_oS,    c->d_oY,     c->d_obest, c->d_odone,
+                      c->d_ost, c->d_ost2, c->d_oSY,  c->d_oYY,   c->d_oPART, c->d_oPART2, c->d_oseg};
+    return MIA_HIP_OK;
+}
+
+// One chunk: ipg iterations (evaluation, then the step's kernels), then the
+// done flags copied to host slot `slot`.
+int opt_run_enqueue_chunk(OptRun &r, int slot)
+{
+    mia_hip_ctx *c = r.c;
+    const int n = c->n_inst;
+    const std::vector<int> &seg = r.seg;
+    const OptBuffers &ob = r.ob;
+    int rc = MIA_HIP_OK;
+    for (int it = 0; it < r.ipg && rc == MIA_HIP_OK; ++it) {
+        rc = enqueue_eval(c, c->d_params, c->d_grad, c->d_costs, false);
+        if (rc != MIA_HIP_OK)
+            break;
+        hipError_t e = hipSuccess;
+        if (r.n_small > 0) {
+            // memory <= kLbRegMem: the stored pairs stay in registers for the whole step;
+            // paired: every instance's vectors start 16-byte aligned and have even
+            // lengths, so each thread moves adjacent pairs of elements
+            const bool reg = r.memory <= kLbRegMem;
+            auto kstep = reg ? (r.paired ? k_lbfgs_step_small<true, true> : k_lbfgs_step_small<true, false>)
+                             : (r.paired ? k_lbfgs_step_small<false, true> : k_lbfgs_step_small<false, false>);
+            hipLaunchKernelGGL(kstep, dim3(n), dim3(kOptBlock), 0, c->stream, (const InstDesc *)c->d_inst, c->dim,
+                               ob, c->d_params, (const double *)c->d_grad, (const double *)c->d_costs, r.memory,
+                               r.maxeval, r.first_step);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && seg[n] > 0) {
+            hipLaunchKernelGGL(k_lbfgs_pass1, dim3(seg[n]), dim3(kSegT), 0, c->stream, (const InstDesc *)c->d_inst, n,
+                               c->dim, ob, c->d_params, (const double *)c->d_grad, (const double *)c->d_costs,
+                               r.memory, r.maxeval);
+            e = hipGetLastError();
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_lbfgs_pass2, dim3(seg[n]), dim3(kSegT), 0, c->stream,
                                    (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params,
-                                   (const double *)c->d_grad, (const double *)c->d_costs, memory, maxeval);
+                                   (const double *)c->d_costs, r.memory, r.maxeval, r.first_step);
                 e = hipGetLastError();
-                if (e == hipSuccess) {
-                    hipLaunchKernelGGL(k_lbfgs_pass2, dim3(seg[n]), dim3(kSegT), 0, c->stream,
-                                       (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params,
-                                       (const double *)c->d_costs, memory, maxeval, first_step);
-                    e = hipGetLastError();
-                }
-                if (e == hipSuccess) {
-                    hipLaunchKernelGGL(k_lbfgs_pass3, dim3(seg[n]), dim3(kSegT), 0, c->stream,
-                                       (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params, first_step);
-                    e = hipGetLastError();
-                }
             }
-            if (e != hipSuccess)
-                rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs passes: ") + hipGetErrorString(e));
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_lbfgs_pass3, dim3(seg[n]), dim3(kSegT), 0, c->stream,
+                                   (const InstDesc *)c->d_inst, n, c->dim, ob, c->d_params, r.first_step);
+                e = hipGetLastError();
+            }
         }
-        if (rc == MIA_HIP_OK) {
-            hipError_t e = hipMemcpyAsync(c->h_done + (size_t)slot * n, c->d_odone, sizeof(int) * n,
-                                          hipMemcpyDeviceToHost, c->stream);
-            if (e != hipSuccess)
-                rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-        }
-        return rc;
-    };
-    auto all_done = [&](int slot) {
-        for (int i = 0; i < n; ++i)
-            if (!c->h_done[(size_t)slot * n + i])
-                return false;
-        return true;
-    };
-    // every iteration performs one evaluation per unfinished instance, and an
-    // instance finishes at the latest with its maxeval-th evaluation
-    const int max_launches = (maxeval + iters_per_graph - 1) / iters_per_graph + 1;
+        if (e != hipSuccess)
+            rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs passes: ") + hipGetErrorString(e));
+    }
+    if (rc == MIA_HIP_OK) {
+        hipError_t e = hipMemcpyAsync(c->h_done + (size_t)slot * n, c->d_odone, sizeof(int) * n,
+                                      hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess)
+            rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
+    }
+    return rc;
+}
+
+bool opt_run_all_done(const OptRun &r, int slot)
+{
+    const int n = r.c->n_inst;
+    for (int i = 0; i < n; ++i)
+        if (!r.c->h_done[(size_t)slot * n + i])
+            return false;
+    return true;
+}
+
+// every iteration performs one evaluation per unfinished instance, and an
+// instance finishes at the latest with its maxeval-th evaluation
+int opt_max_launches(int maxeval, int ipg)
+{
+    return (maxeval + ipg - 1) / ipg + 1;
+}
+
+// Stream launches of one or more runs (equal maxeval and chunk length), each
+// on its context's stream: one chunk per run queued ahead while the host reads
+// the previous chunk's done flags (only chunks that start before the
+// maxeval-th iteration are queued ahead; a chunk queued after all of a run's
+// instances finished changes nothing: finished instances' steps return at
+// once).  The runs' chunks are queued alternately, so that on streams of
+// their own one run's optimizer step can run beside another run's evaluation.
+int opt_stream_loop(std::vector<OptRun> &runs, double *seconds)
+{
+    const int max_launches = opt_max_launches(runs[0].maxeval, runs[0].ipg);
     const int sure_launches = max_launches - 1;
     int rc = MIA_HIP_OK;
-    std::chrono::steady_clock::time_point t0, t1;
-    if (c->opt_launch == MIA_HIP_LAUNCH_STREAM) {
-        // stream launches, one chunk queued ahead while the host reads the
-        // previous chunk's done flags (only chunks that start before the
-        // maxeval-th iteration are queued ahead; a chunk queued after all
-        // instances finished changes nothing: finished instances' steps
-        // return at once)
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        for (auto &e : ev)
-            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        t0 = std::chrono::steady_clock::now();
-        rc = enqueue_chunk(0);
-        if (rc == MIA_HIP_OK) {
-            hipError_t e = hipEventRecord(ev[0], c->stream);
-            if (e != hipSuccess)
-                rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-        }
-        for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
-            const bool ahead = l + 1 < sure_launches;
-            if (ahead) {
-                rc = enqueue_chunk((l + 1) & 1);
-                if (rc != MIA_HIP_OK)
-                    break;
-                hipError_t e = hipEventRecord(ev[(l + 1) & 1], c->stream);
-                if (e != hipSuccess) {
-                    rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-                    break;
-                }
-            }
-            hipError_t e = hipEventSynchronize(ev[l & 1]);
-            if (e != hipSuccess) {
-                rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize: ") + hipGetErrorString(e));
-                break;
-            }
-            if (all_done(l & 1) || l + 1 == max_launches)
-                break;
-            if (!ahead) {
-                rc = enqueue_chunk((l + 1) & 1);
-                if (rc != MIA_HIP_OK)
-                    break;
-                e = hipEventRecord(ev[(l + 1) & 1], c->stream);
-                if (e != hipSuccess)
-                    rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-            }
-        }
-        hipError_t e = hipStreamSynchronize(c->stream);
+    auto check = [&](hipError_t e) {
         if (e != hipSuccess && rc == MIA_HIP_OK)
             rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize: ") + hipGetErrorString(e));
-        t1 = std::chrono::steady_clock::now();
-        for (auto &ev1 : ev)
-            (void)hipEventDestroy(ev1);
-    } else {
-        // one captured graph of a chunk, launched until every instance is done
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_chunk(0);
-        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-        if (rc != MIA_HIP_OK) {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e2 != hipSuccess)
-            return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
-        hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (e3 != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            return fail(MIA_HIP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-        }
-        t0 = std::chrono::steady_clock::now();
-        for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
-            hipError_t e = hipGraphLaunch(ge, c->stream);
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) {
-                rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
+        return rc == MIA_HIP_OK;
+    };
+    for (auto &r : runs)
+        for (auto &e : r.ev)
+            if (!check(hipEventCreateWithFlags(&e, hipEventDisableTiming)))
                 break;
-            }
-            if (all_done(0))
+    auto queue = [&](OptRun &r, int slot) {
+        if (rc == MIA_HIP_OK)
+            rc = opt_run_enqueue_chunk(r, slot);
+        if (rc == MIA_HIP_OK)
+            check(hipEventRecord(r.ev[slot], r.c->stream));
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto &r : runs)
+        queue(r, 0);
+    bool any = true;
+    for (int l = 0; l < max_launches && rc == MIA_HIP_OK && any; ++l) {
+        const bool ahead = l + 1 < sure_launches;
+        if (ahead)
+            for (auto &r : runs)
+                if (r.active)
+                    queue(r, (l + 1) & 1);
+        any = false;
+        for (auto &r : runs) {
+            if (!r.active || rc != MIA_HIP_OK)
+                continue;
+            if (!check(hipEventSynchronize(r.ev[l & 1])))
                 break;
+            if (opt_run_all_done(r, l & 1) || l + 1 == max_launches)
+                r.active = false;
+            any = any || r.active;
         }
-        t1 = std::chrono::steady_clock::now();
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
+        if (!ahead && any)
+            for (auto &r : runs)
+                if (r.active)
+                    queue(r, (l + 1) & 1);
     }
-    if (rc != MIA_HIP_OK)
-        return rc;
-    if (seconds)
+    for (auto &r : runs)
+        check(hipStreamSynchronize(r.c->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    for (auto &r : runs)
+        for (auto &e : r.ev)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+    if (rc == MIA_HIP_OK && seconds)
         *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return rc;
+}
+
+// One captured graph of a chunk, launched until every instance is done.
+int opt_graph_loop(OptRun &r, double *seconds)
+{
+    mia_hip_ctx *c = r.c;
+    const int max_launches = opt_max_launches(r.maxeval, r.ipg);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = opt_run_enqueue_chunk(r, 0);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc != MIA_HIP_OK) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e2 != hipSuccess)
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(MIA_HIP_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int l = 0; l < max_launches && rc == MIA_HIP_OK; ++l) {
+        hipError_t e = hipGraphLaunch(ge, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
+            break;
+        }
+        if (opt_run_all_done(r, 0))
+            break;
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (rc == MIA_HIP_OK && seconds)
+        *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return rc;
+}
+
+int opt_run_results(OptRun &r, double *x_out, double *costs_out, int *evals_out)
+{
+    mia_hip_ctx *c = r.c;
+    const int n = c->n_inst;
+    int st;
     if (x_out && (st = stage_download(c, x_out, c->d_params, sizeof(double) * c->n_par)))
         return st;
     if (costs_out && (st = stage_download(c, costs_out, c->d_obest, sizeof(double) * n)))
@@ -1925,6 +2024,72 @@ int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, 
             evals_out[i] = stv[i].evals;
     }
     return MIA_HIP_OK;
+}
+
+// While alive, the listed contexts queue their work on other streams; on
+// destruction those streams are drained and the contexts get their own back.
+struct StreamSwap {
+    std::vector<std::pair<mia_hip_ctx *, hipStream_t>> saved;
+    ~StreamSwap()
+    {
+        for (auto &p : saved) {
+            (void)hipStreamSynchronize(p.first->stream);
+            p.first->stream = p.second;
+        }
+    }
+};
+
+} // namespace
+
+int mia_hip_optimize(mia_hip_ctx *c, const double *x0, int maxeval, int memory, double first_step,
+                     int iters_per_graph, double *x_out, double *costs_out, int *evals_out, double *seconds)
+{
+    std::vector<OptRun> runs(1);
+    int st = opt_run_prepare(runs[0], c, x0, maxeval, memory, first_step, iters_per_graph);
+    if (st)
+        return st;
+    st = c->opt_launch == MIA_HIP_LAUNCH_STREAM ? opt_stream_loop(runs, seconds) : opt_graph_loop(runs[0], seconds);
+    if (st)
+        return st;
+    return opt_run_results(runs[0], x_out, costs_out, evals_out);
+}
+
+int mia_hip_optimize_many(mia_hip_ctx *const *ctx, int n_ctx, const double *const *x0, int maxeval, int memory,
+                          double first_step, int iters_per_graph, double *const *x_out, double *const *costs_out,
+                          int *const *evals_out, double *seconds)
+{
+    if (!ctx || n_ctx < 1 || !x0)
+        return fail(MIA_HIP_ERR_ARG, "optimize_many: need at least one context and its starting point");
+    for (int k = 0; k < n_ctx; ++k) {
+        if (!ctx[k])
+            return fail(MIA_HIP_ERR_ARG, "optimize_many: null context " + std::to_string(k));
+        if (ctx[k]->device != ctx[0]->device)
+            return fail(MIA_HIP_ERR_ARG, "optimize_many: the contexts must be on one device");
+        for (int j = 0; j < k; ++j)
+            if (ctx[j] == ctx[k])
+                return fail(MIA_HIP_ERR_ARG, "optimize_many: context " + std::to_string(k) + " is listed twice");
+    }
+    HIP_TRY(hipSetDevice(ctx[0]->device));
+    std::array<hipStream_t, mia_hip_rt::kJointStreams> js{};
+    HIP_TRY(mia_hip_rt::joint_streams(ctx[0]->device, &js));
+    std::vector<OptRun> runs(n_ctx);
+    int st = MIA_HIP_OK;
+    {
+        StreamSwap swap;
+        for (int k = 0; k < n_ctx && st == MIA_HIP_OK; ++k) {
+            // the context's own stream drained, then its work goes to a joint stream
+            HIP_TRY(hipStreamSynchronize(ctx[k]->stream));
+            swap.saved.emplace_back(ctx[k], ctx[k]->stream);
+            ctx[k]->stream = js[k % mia_hip_rt::kJointStreams];
+            st = opt_run_prepare(runs[k], ctx[k], x0[k], maxeval, memory, first_step, iters_per_graph);
+        }
+        if (st == MIA_HIP_OK)
+            st = opt_stream_loop(runs, seconds);
+        for (int k = 0; k < n_ctx && st == MIA_HIP_OK; ++k)
+            st = opt_run_results(runs[k], x_out ? x_out[k] : nullptr, costs_out ? costs_out[k] : nullptr,
+                                 evals_out ? evals_out[k] : nullptr);
+    }
+    return st;
 }
 
 int mia_hip_set_optimize_launch(mia_hip_ctx *c, int mode)

@@ -166,6 +166,12 @@ struct mia_host_series {
     mia_hip_ctx *batch = nullptr;
     std::vector<mia_hip_ctx *> single;
     std::vector<char> frame_set;
+    // MIA_HOST_DEVICE_SPLIT: the frames in two batches of about equal pixel
+    // counts (none when there is one frame); frame i is instance part_idx[i]
+    // of batch part_of[i]
+    std::vector<mia_hip_ctx *> part;
+    std::vector<std::vector<int>> part_frames;
+    std::vector<int> part_of, part_idx;
 };
 
 bool mia_amd::device_lbfgs_params(const std::string &descr, int &maxeval, int &memory, double &step,
@@ -208,7 +214,9 @@ bool mia_amd::device_lbfgs_params(const std::string &descr, int &maxeval, int &m
 
 // On-device L-BFGS through mia_hip_optimize: all frames in the batched context
 // (one graph covers every frame), or each frame in its own context, one frame
-// after the other.  Only `lbfgs` with gtol = ftolr = 0 runs on the device.
+// after the other; or, through mia_hip_optimize_many, the frames in two
+// batches whose iterations overlap.  Only `lbfgs` with gtol = ftolr = 0 runs
+// on the device.
 static int run_device(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
                       int *evals_out, double *seconds)
 {
@@ -221,7 +229,47 @@ static int run_device(mia_host_series *s, const char *minimizer, int mode, doubl
                                                  : "series: " + err);
     const int chunk = 10;
     int st;
-    if (mode == MIA_HOST_DEVICE_BATCHED) {
+    if (mode == MIA_HOST_DEVICE_SPLIT && !s->part.empty()) {
+        const int np = (int)s->part.size();
+        std::vector<std::vector<double>> x0(np), xo(np), co(np);
+        std::vector<std::vector<int>> eo(np);
+        std::vector<const double *> px0(np);
+        std::vector<double *> pxo(np), pco(np);
+        std::vector<int *> peo(np);
+        for (int p = 0; p < np; ++p) {
+            size_t n = 0, o = 0;
+            const int last = (int)s->part_frames[p].size() - 1;
+            if ((st = mia_hip_n_params(s->part[p], last, &n, &o)))
+                return hip_fail(st);
+            x0[p].assign(o + n, 0.0);
+            xo[p].resize(o + n);
+            co[p].resize(s->part_frames[p].size());
+            eo[p].resize(s->part_frames[p].size());
+            px0[p] = x0[p].data();
+            pxo[p] = xo[p].data();
+            pco[p] = co[p].data();
+            peo[p] = eo[p].data();
+            if ((st = mia_hip_set_cost(s->part[p], 0, 0.0f, 1.0, MIA_HIP_GRAD_REFERENCE)))
+                return hip_fail(st);
+        }
+        if ((st = mia_hip_optimize_many(s->part.data(), np, px0.data(), maxeval, memory, step, chunk, pxo.data(),
+                                        pco.data(), peo.data(), seconds)))
+            return hip_fail(st);
+        for (int i = 0; i < s->n; ++i) {
+            const int p = s->part_of[i], j = s->part_idx[i];
+            size_t n = 0, o = 0;
+            if ((st = mia_hip_n_params(s->part[p], j, &n, &o)))
+                return hip_fail(st);
+            if (params_out)
+                std::copy(xo[p].begin() + (long)o, xo[p].begin() + (long)(o + n), params_out + s->off[i]);
+            if (costs_out)
+                costs_out[i] = co[p][j];
+            if (evals_out)
+                evals_out[i] = eo[p][j];
+        }
+        return MIA_HIP_OK;
+    }
+    if (mode == MIA_HOST_DEVICE_BATCHED || mode == MIA_HOST_DEVICE_SPLIT) {
         std::vector<double> x0(s->n_total, 0.0);
         if ((st = mia_hip_set_cost(s->batch, 0, 0.0f, 1.0, MIA_HIP_GRAD_REFERENCE)))
             return hip_fail(st);
@@ -265,6 +313,9 @@ int mia_host_series_destroy(mia_host_series *s)
         return MIA_HIP_OK;
     if (s->batch)
         mia_hip_ctx_destroy(s->batch);
+    for (auto *c : s->part)
+        if (c)
+            mia_hip_ctx_destroy(c);
     for (auto *c : s->single)
         if (c)
             mia_hip_ctx_destroy(c);
@@ -277,7 +328,9 @@ int mia_host_series_create(int device, int dim, int n_frames, const uint32_t *si
 {
     if (!out || !sizes || n_frames < 1 || (dim != 2 && dim != 3))
         return fail(MIA_HIP_ERR_ARG, "series: need dim 2 or 3, at least one frame and its sizes");
-    auto s = std::unique_ptr<mia_host_series>(new mia_host_series());
+    // on a failure below, the contexts created so far go with the series
+    auto s = std::unique_ptr<mia_host_series, int (*)(mia_host_series *)>(new mia_host_series(),
+                                                                            mia_host_series_destroy);
     s->device = device;
     s->dim = dim;
     s->n = n_frames;
@@ -307,6 +360,40 @@ int mia_host_series_create(int device, int dim, int n_frames, const uint32_t *si
     s->n_total = s->off.back() + s->n_par.back();
     s->single.assign(n_frames, nullptr);
     s->frame_set.assign(n_frames, 0);
+    if (n_frames >= 2) {
+        // two batches of about equal pixel counts: frames by decreasing size,
+        // each to the batch with fewer pixels so far (ties to the first); a
+        // batch keeps its frames in ascending order
+        std::vector<int> order(n_frames);
+        for (int i = 0; i < n_frames; ++i)
+            order[i] = i;
+        auto pixels = [&](int i) { return (double)s->sizes[3 * i] * s->sizes[3 * i + 1] * s->sizes[3 * i + 2]; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pixels(a) > pixels(b); });
+        s->part_frames.assign(2, {});
+        double load[2] = {0.0, 0.0};
+        for (int i : order) {
+            const int p = load[1] < load[0] ? 1 : 0;
+            s->part_frames[p].push_back(i);
+            load[p] += pixels(i);
+        }
+        s->part_of.assign(n_frames, 0);
+        s->part_idx.assign(n_frames, 0);
+        s->part.assign(2, nullptr);
+        for (int p = 0; p < 2; ++p) {
+            auto &fr = s->part_frames[p];
+            std::sort(fr.begin(), fr.end());
+            std::vector<uint32_t> sz, cs;
+            for (size_t j = 0; j < fr.size(); ++j) {
+                s->part_of[fr[j]] = p;
+                s->part_idx[fr[j]] = (int)j;
+                sz.insert(sz.end(), &s->sizes[3 * fr[j]], &s->sizes[3 * fr[j]] + 3);
+                cs.insert(cs.end(), &s->csizes[3 * fr[j]], &s->csizes[3 * fr[j]] + 3);
+            }
+            if ((st = mia_hip_ctx_create(device, &s->part[p])) ||
+                (st = mia_hip_setup(s->part[p], dim, (int)fr.size(), sz.data(), cs.data(), degree)))
+                return hip_fail(st);
+        }
+    }
     *out = s.release();
     return MIA_HIP_OK;
 }
@@ -320,6 +407,8 @@ int mia_host_series_set_frame(mia_host_series *s, int frame, const float *src, c
     normalize_pair(src, ref, nv, a, b);
     int st = mia_hip_set_images(s->batch, frame, a.data(), b.data());
     if (st)
+        return hip_fail(st);
+    if (!s->part.empty() && (st = mia_hip_set_images(s->part[s->part_of[frame]], s->part_idx[frame], a.data(), b.data())))
         return hip_fail(st);
     if (!s->single[frame]) {
         if ((st = mia_hip_ctx_create(s->device, &s->single[frame])))
@@ -353,7 +442,7 @@ int mia_host_series_run(mia_host_series *s, const char *minimizer, int mode, dou
     for (int i = 0; i < s->n; ++i)
         if (!s->frame_set[i])
             return fail(MIA_HIP_ERR_STATE, "series: frame " + std::to_string(i) + " has no images");
-    if (mode == MIA_HOST_DEVICE_BATCHED || mode == MIA_HOST_DEVICE_PER_FRAME)
+    if (mode == MIA_HOST_DEVICE_BATCHED || mode == MIA_HOST_DEVICE_PER_FRAME || mode == MIA_HOST_DEVICE_SPLIT)
         return run_device(s, minimizer, mode, params_out, costs_out, evals_out, seconds);
     std::vector<std::unique_ptr<ReverseMinimizer>> opt(s->n);
     for (int i = 0; i < s->n; ++i) {

@@ -21,6 +21,7 @@ BATCHED = 1
 PER_FRAME = 2
 DEVICE_BATCHED = 3
 DEVICE_PER_FRAME = 4
+DEVICE_SPLIT = 5
 
 _vp = ctypes.c_void_p
 SIGNATURES = {

@@ -33,6 +33,20 @@ def test_ctypes_table_matches_header():
     assert sorted(_lib.SIGNATURES) == _declared(_lib.HEADER)
 
 
+def test_optimize_many_rejects_bad_lists():
+    """mia_hip_optimize_many checks its context list before touching a device:
+    no contexts, a null list or a null context are argument errors."""
+    from mia_amd import _lib
+    L = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    x0 = (dp * 1)()
+    hs = (ctypes.c_void_p * 1)(None)
+    assert L.mia_hip_optimize_many(None, 1, x0, 5, 10, 1.0, 10, None, None, None, None) == -1
+    assert L.mia_hip_optimize_many(hs, 0, x0, 5, 10, 1.0, 10, None, None, None, None) == -1
+    assert L.mia_hip_optimize_many(hs, 1, x0, 5, 10, 1.0, 10, None, None, None, None) == -1
+    assert b"null context 0" in L.mia_hip_last_error()
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_spline_csize_matches_oracle(dim):
     import mia_amd

@@ -144,6 +144,94 @@ def test_device_optimizer_both_search_paths():
     np.testing.assert_allclose(cd, ch, rtol=1e-9)
 
 
+@pytest.mark.parametrize("dim,n,side", [(2, 9, (40, 150)), (3, 3, (24, 40))])
+def test_device_split_equals_batched_and_per_frame(dim, n, side):
+    """The frames in two batches whose iterations overlap on two streams
+    (mia_hip_optimize_many): bitwise the parameters, costs and evaluation
+    counts of one batch and of one context per frame."""
+    from mia_amd import series
+    frames = _frames(n, side, 23, dim)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        ps, cs, es, _ = s.run("lbfgs:maxiter=30", series.DEVICE_SPLIT)
+        pb, cb, eb, _ = s.run("lbfgs:maxiter=30", series.DEVICE_BATCHED)
+        pf, cf, ef, _ = s.run("lbfgs:maxiter=30", series.DEVICE_PER_FRAME)
+        ps2, cs2, es2, _ = s.run("lbfgs:maxiter=30", series.DEVICE_SPLIT)
+    for a, b, c in ((ps, pb, pf), (cs, cb, cf), (es, eb, ef)):
+        assert np.array_equal(a, b)
+        assert np.array_equal(a, c)
+    assert np.array_equal(ps2, ps) and np.array_equal(cs2, cs) and np.array_equal(es2, es)
+    assert np.all(es <= 30)
+
+
+def test_device_split_one_frame():
+    """One frame: the split mode runs it as one batch, with the results of the
+    batched mode."""
+    from mia_amd import series
+    frames = _frames(1, (50, 60), 24)
+    with series.SeriesRegistration(frames, rate=8.0) as s:
+        ps, cs, es, _ = s.run("lbfgs:maxiter=15", series.DEVICE_SPLIT)
+        pb, cb, eb, _ = s.run("lbfgs:maxiter=15", series.DEVICE_BATCHED)
+    assert np.array_equal(ps, pb) and np.array_equal(cs, cb) and np.array_equal(es, eb)
+
+
+def test_optimize_many_equals_each_alone():
+    """mia_hip_optimize_many over three contexts (two share a stream): a 2D
+    batch with frames on both sides of the one-block step's 2048-parameter
+    limit, a 3D volume stepped in segments with the divcurl penalty attached,
+    and a second 2D batch; each context's parameters, costs and evaluation
+    counts are bitwise those of mia_hip_optimize on that context alone, also
+    when the contexts' evaluation budgets end at different iterations."""
+    import mia_amd
+    from mia_amd import level, series, synth
+    fr_a = synth.series_frames_2d(2, (60, 80), 8.0, 4.0, 61) + synth.series_frames_2d(1, (300, 320), 8.0, 4.0, 62)
+    fr_b = synth.series_frames_2d(3, (90, 130), 8.0, 4.0, 63)
+    shape = (40, 36, 44)
+    fix3 = synth.smoothed_noise(shape, 2.0, 64)
+    cs3 = tuple(mia_amd.spline_csize(v, 4.0, 3) for v in shape[::-1])
+    mov3 = synth.spline_warp(fix3, cs3, synth.random_spline_params(cs3, 1.5, 3.0, 65))
+    assert 3 * cs3[0] * cs3[1] * cs3[2] > 2048
+
+    def ctx2d(frames):
+        pairs = [series.normalize_pair(m, f) for m, f in frames]
+        shapes = [p[0].shape for p in pairs]
+        cs = [(mia_amd.spline_csize(v[1], 8.0, 2), mia_amd.spline_csize(v[0], 8.0, 2)) for v in shapes]
+        c = mia_amd.HipContext(0)
+        c.setup(2, shapes, cs, 3)
+        for k, p in enumerate(pairs):
+            c.set_images(k, p[0], p[1])
+        return c
+
+    ca, cb = ctx2d(fr_a), ctx2d(fr_b)
+    c3 = mia_amd.HipContext(0)
+    pen = level.DivcurlPenalty(cs3, tuple(float(v) for v in shape[::-1]))
+    try:
+        c3.setup(3, [shape], [cs3], 3)
+        c3.set_images(0, mov3, fix3)
+        c3.set_penalty(0, pen, 0.01)
+        ctxs = [ca, c3, cb]
+        x0 = [np.zeros(c.n_params_total) for c in ctxs]
+        for maxeval in (25, 8):
+            alone = [c.optimize(x, maxeval=maxeval, iters_per_graph=10)[:3] for c, x in zip(ctxs, x0)]
+            joint, sec = mia_amd.HipContext.optimize_many(ctxs, x0, maxeval=maxeval, iters_per_graph=10)
+            assert sec > 0.0
+            for k in range(3):
+                for a, b in zip(joint[k], alone[k]):
+                    assert np.array_equal(a, b), (maxeval, k)
+                assert np.all(joint[k][2] <= maxeval)
+        # the contexts keep working on their own streams afterwards
+        again = ca.optimize(x0[0], maxeval=25, iters_per_graph=10)[:3]
+        first = mia_amd.HipContext.optimize_many([ca], [x0[0]], maxeval=25)[0][0]
+        for a, b in zip(again, first):
+            assert np.array_equal(a, b)
+        with pytest.raises(ValueError):
+            mia_amd.HipContext.optimize_many([ca, ca], [x0[0], x0[0]], maxeval=5)
+    finally:
+        c3.set_penalty(0, None, 0.0)
+        for c in (ca, cb, c3):
+            c.close()
+        pen.close()
+
+
 def test_device_lbfgs_ring_wraparound():
     """memory = 2: from the third accepted step on, every step drops the
     oldest pair and overwrites its ring slot (and its rows of the pair-product
