@@ -191,9 +191,9 @@ def series_2d(n_frames, minimizer, cpu_threads=0):
     spline rate 8, ssd, registered from the identity with L-BFGS.
 
     Timed (optimisation loop only, set-up excluded):
-      * device_split: on-device L-BFGS, the frames in two batches of about
-        equal pixel counts on two streams, one batch's optimizer step running
-        beside the other's evaluation (median of three runs);
+      * device_split: on-device L-BFGS, the frames in three batches of about
+        equal pixel counts on three streams, one batch's optimizer step
+        running beside another's evaluation (median of three runs);
       * device_batched: on-device L-BFGS, every frame in one batched launch
         sequence per iteration, 10 iterations per chunk (median of three);
       * device_per_frame: the same, one context and graph per frame, frames

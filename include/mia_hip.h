@@ -170,16 +170,16 @@ int mia_hip_optimize(mia_hip_ctx *ctx, const double *x0, int maxeval, int memory
 
 /* mia_hip_optimize on n_ctx contexts of one device at once: context k
  * starts from x0[k] and writes x_out[k], costs_out[k], evals_out[k] (any of
- * the three arrays, or an entry, may be NULL).  The contexts' chunks of
- * iterations are queued alternately on two streams that the library keeps
- * per device, created with the device's first context so that HIP gives
- * them hardware queues of their own: one context's optimizer step, which
- * occupies a block per instance, runs beside the other context's
+ * the three arrays, or an entry, may be NULL).  Each context's iterations
+ * are queued by a host thread of its own on one of three streams that the
+ * library keeps per device, created with the device's first context so that
+ * HIP gives them hardware queues of their own: one context's optimizer step,
+ * which occupies a block per instance, runs beside another context's
  * evaluation, which spreads over the whole GPU.  Every context's results are
  * bitwise those mia_hip_optimize gives it alone (an instance's trajectory
  * depends on nothing outside it).  Chunks are queued as stream launches
- * whatever mia_hip_set_optimize_launch chose; with more than two contexts,
- * contexts k and k + 2 share a stream.  seconds: wall time of the joint
+ * whatever mia_hip_set_optimize_launch chose; with more than three contexts,
+ * contexts k and k + 3 share a stream.  seconds: wall time of the joint
  * iteration loop.  No counterpart in the reference (its minimizers run on
  * the host, one registration at a time: src/2dmany2one-nonrigid.cc:185). */
 int mia_hip_optimize_many(mia_hip_ctx *const *ctx, int n_ctx, const double *const *x0, int maxeval, int memory,

@@ -24,8 +24,9 @@ extern "C" {
 #define MIA_HOST_PER_FRAME 2     /* one context per frame, frames registered one after the other */
 #define MIA_HOST_DEVICE_BATCHED 3   /* on-device L-BFGS, all frames in one graph per iteration block */
 #define MIA_HOST_DEVICE_PER_FRAME 4 /* on-device L-BFGS, one context (and graph) per frame, one after the other */
-#define MIA_HOST_DEVICE_SPLIT 5     /* on-device L-BFGS, the frames in two batches of about equal pixel counts
-                                       whose iterations overlap (mia_hip_optimize_many); one frame: one batch */
+#define MIA_HOST_DEVICE_SPLIT 5     /* on-device L-BFGS, the frames in three batches of about equal pixel counts
+                                       whose iterations overlap (mia_hip_optimize_many); fewer frames than
+                                       three: one batch per frame */
 
 typedef struct mia_host_series mia_host_series;
 

@@ -158,6 +158,9 @@ static void normalize_pair(const float *a, const float *b, size_t n, std::vector
 
 using namespace mia_amd;
 
+// batches of MIA_HOST_DEVICE_SPLIT (one per stream of mia_hip_optimize_many)
+constexpr int kSeriesParts = 3;
+
 struct mia_host_series {
     int device = 0, dim = 2, n = 0, degree = 3;
     std::vector<uint32_t> sizes, csizes;
@@ -166,9 +169,9 @@ struct mia_host_series {
     mia_hip_ctx *batch = nullptr;
     std::vector<mia_hip_ctx *> single;
     std::vector<char> frame_set;
-    // MIA_HOST_DEVICE_SPLIT: the frames in two batches of about equal pixel
-    // counts (none when there is one frame); frame i is instance part_idx[i]
-    // of batch part_of[i]
+    // MIA_HOST_DEVICE_SPLIT: the frames in kSeriesParts batches of about
+    // equal pixel counts (none when there is one frame); frame i is instance
+    // part_idx[i] of batch part_of[i]
     std::vector<mia_hip_ctx *> part;
     std::vector<std::vector<int>> part_frames;
     std::vector<int> part_of, part_idx;
@@ -214,8 +217,8 @@ bool mia_amd::device_lbfgs_params(const std::string &descr, int &maxeval, int &m
 
 // On-device L-BFGS through mia_hip_optimize: all frames in the batched context
 // (one graph covers every frame), or each frame in its own context, one frame
-// after the other; or, through mia_hip_optimize_many, the frames in two
-// batches whose iterations overlap.  Only `lbfgs` with gtol = ftolr = 0 runs
+// after the other; or, through mia_hip_optimize_many, the frames in
+// kSeriesParts batches whose iterations overlap.  Only `lbfgs` with gtol = ftolr = 0 runs
 // on the device.
 static int run_device(mia_host_series *s, const char *minimizer, int mode, double *params_out, double *costs_out,
                       int *evals_out, double *seconds)
@@ -360,26 +363,27 @@ int mia_host_series_create(int device, int dim, int n_frames, const uint32_t *si
     s->n_total = s->off.back() + s->n_par.back();
     s->single.assign(n_frames, nullptr);
     s->frame_set.assign(n_frames, 0);
-    if (n_frames >= 2) {
-        // two batches of about equal pixel counts: frames by decreasing size,
-        // each to the batch with fewer pixels so far (ties to the first); a
-        // batch keeps its frames in ascending order
+    const int n_parts = std::min(n_frames, kSeriesParts);
+    if (n_parts >= 2) {
+        // batches of about equal pixel counts: frames by decreasing size,
+        // each to the batch with the fewest pixels so far (ties to the
+        // first); a batch keeps its frames in ascending order
         std::vector<int> order(n_frames);
         for (int i = 0; i < n_frames; ++i)
             order[i] = i;
         auto pixels = [&](int i) { return (double)s->sizes[3 * i] * s->sizes[3 * i + 1] * s->sizes[3 * i + 2]; };
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pixels(a) > pixels(b); });
-        s->part_frames.assign(2, {});
-        double load[2] = {0.0, 0.0};
+        s->part_frames.assign(n_parts, {});
+        std::vector<double> load(n_parts, 0.0);
         for (int i : order) {
-            const int p = load[1] < load[0] ? 1 : 0;
+            const int p = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             s->part_frames[p].push_back(i);
             load[p] += pixels(i);
         }
         s->part_of.assign(n_frames, 0);
         s->part_idx.assign(n_frames, 0);
-        s->part.assign(2, nullptr);
-        for (int p = 0; p < 2; ++p) {
+        s->part.assign(n_parts, nullptr);
+        for (int p = 0; p < n_parts; ++p) {
             auto &fr = s->part_frames[p];
             std::sort(fr.begin(), fr.end());
             std::vector<uint32_t> sz, cs;

@@ -146,7 +146,7 @@ def test_device_optimizer_both_search_paths():
 
 @pytest.mark.parametrize("dim,n,side", [(2, 9, (40, 150)), (3, 3, (24, 40))])
 def test_device_split_equals_batched_and_per_frame(dim, n, side):
-    """The frames in two batches whose iterations overlap on two streams
+    """The frames in batches whose iterations overlap on streams of their own
     (mia_hip_optimize_many): bitwise the parameters, costs and evaluation
     counts of one batch and of one context per frame."""
     from mia_amd import series
@@ -175,16 +175,17 @@ def test_device_split_one_frame():
 
 
 def test_optimize_many_equals_each_alone():
-    """mia_hip_optimize_many over three contexts (two share a stream): a 2D
-    batch with frames on both sides of the one-block step's 2048-parameter
-    limit, a 3D volume stepped in segments with the divcurl penalty attached,
-    and a second 2D batch; each context's parameters, costs and evaluation
-    counts are bitwise those of mia_hip_optimize on that context alone, also
-    when the contexts' evaluation budgets end at different iterations."""
+    """mia_hip_optimize_many over four contexts (the first and the last share
+    a stream): a 2D batch with frames on both sides of the one-block step's
+    2048-parameter limit, a 3D volume stepped in segments with the divcurl
+    penalty attached, and two more 2D batches; each context's parameters,
+    costs and evaluation counts are bitwise those of mia_hip_optimize on that
+    context alone, for two evaluation budgets."""
     import mia_amd
     from mia_amd import level, series, synth
     fr_a = synth.series_frames_2d(2, (60, 80), 8.0, 4.0, 61) + synth.series_frames_2d(1, (300, 320), 8.0, 4.0, 62)
     fr_b = synth.series_frames_2d(3, (90, 130), 8.0, 4.0, 63)
+    fr_c = synth.series_frames_2d(2, (40, 70), 8.0, 4.0, 66)
     shape = (40, 36, 44)
     fix3 = synth.smoothed_noise(shape, 2.0, 64)
     cs3 = tuple(mia_amd.spline_csize(v, 4.0, 3) for v in shape[::-1])
@@ -201,20 +202,20 @@ def test_optimize_many_equals_each_alone():
             c.set_images(k, p[0], p[1])
         return c
 
-    ca, cb = ctx2d(fr_a), ctx2d(fr_b)
+    ca, cb, cc = ctx2d(fr_a), ctx2d(fr_b), ctx2d(fr_c)
     c3 = mia_amd.HipContext(0)
     pen = level.DivcurlPenalty(cs3, tuple(float(v) for v in shape[::-1]))
     try:
         c3.setup(3, [shape], [cs3], 3)
         c3.set_images(0, mov3, fix3)
         c3.set_penalty(0, pen, 0.01)
-        ctxs = [ca, c3, cb]
+        ctxs = [ca, c3, cb, cc]
         x0 = [np.zeros(c.n_params_total) for c in ctxs]
         for maxeval in (25, 8):
             alone = [c.optimize(x, maxeval=maxeval, iters_per_graph=10)[:3] for c, x in zip(ctxs, x0)]
             joint, sec = mia_amd.HipContext.optimize_many(ctxs, x0, maxeval=maxeval, iters_per_graph=10)
             assert sec > 0.0
-            for k in range(3):
+            for k in range(4):
                 for a, b in zip(joint[k], alone[k]):
                     assert np.array_equal(a, b), (maxeval, k)
                 assert np.all(joint[k][2] <= maxeval)
@@ -227,7 +228,7 @@ def test_optimize_many_equals_each_alone():
             mia_amd.HipContext.optimize_many([ca, ca], [x0[0], x0[0]], maxeval=5)
     finally:
         c3.set_penalty(0, None, 0.0)
-        for c in (ca, cb, c3):
+        for c in (ca, cb, cc, c3):
             c.close()
         pen.close()
 
