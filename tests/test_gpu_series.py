@@ -226,6 +226,15 @@ def test_optimize_many_equals_each_alone():
             assert np.array_equal(a, b)
         with pytest.raises(ValueError):
             mia_amd.HipContext.optimize_many([ca, ca], [x0[0], x0[0]], maxeval=5)
+        # a context that is not ready fails the call before any iteration, and
+        # the others keep their own streams and results
+        with mia_amd.HipContext(0) as bare:
+            bare.setup(2, [(40, 48)], [(mia_amd.spline_csize(48, 8.0, 2), mia_amd.spline_csize(40, 8.0, 2))], 3)
+            with pytest.raises(mia_amd._lib.MiaHipError):
+                mia_amd.HipContext.optimize_many([cb, bare], [x0[2], np.zeros(bare.n_params_total)], maxeval=5)
+        again_b = cb.optimize(x0[2], maxeval=8, iters_per_graph=10)[:3]
+        for a, b in zip(again_b, alone[2]):
+            assert np.array_equal(a, b)
     finally:
         c3.set_penalty(0, None, 0.0)
         for c in (ca, cb, cc, c3):
