@@ -239,7 +239,7 @@ struct mia_hip_ctx {
 
     // pinned host buffers, kept across set-ups and grown when a set-up needs
     // more (capacities in elements): the graph's parameter, gradient and cost
-    // staging, the optimizer's done flags (two slots), and the staging of
+    // staging, the optimizer's done flags, and the staging of
     // small uploads (a running offset, reset only once the stream is idle)
     double *h_params = nullptr, *h_grad = nullptr, *h_costs = nullptr;
     size_t cap_params = 0, cap_grad = 0, cap_costs = 0, cap_done = 0;
@@ -252,7 +252,9 @@ struct mia_hip_ctx {
     int *d_oseg = nullptr;
     int opt_segments = 0; // segments of the optimizer's passes over all instances
     int opt_launch = MIA_HIP_LAUNCH_STREAM; // how mia_hip_optimize queues its chunks
-    int *d_odone = nullptr, *h_done = nullptr;
+    // done flags, written by the optimizer's kernels straight into this
+    // pinned host memory (no copy per chunk; a flag only goes from 0 to 1)
+    int *h_done = nullptr;
     OptState *d_ost = nullptr, *d_ost2 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -316,7 +318,7 @@ void free_buffers(mia_hip_ctx *c)
                     c->d_M, c->d_R, c->d_W,
                     c->d_P, c->d_clamp, c->d_mm, c->d_tw, c->d_ts, c->d_cells, c->d_ssd, c->d_gscale,
                     c->d_params, c->d_grad, c->d_costs, c->d_scratch, c->d_cnt, c->d_oX, c->d_oG, c->d_oD,
-                    c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone, c->d_ost, c->d_oSY,
+                    c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_ost, c->d_oSY,
                     c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2, c->d_pen_on, c->d_pen_w,
                     c->d_pen_value, c->d_pen_grad};
     // blocks the previous set-up released and this one did not reuse go
@@ -329,7 +331,6 @@ void free_buffers(mia_hip_ctx *c)
     c->d_oseg = nullptr;
     c->d_ost2 = nullptr;
     c->opt_segments = 0;
-    c->d_odone = nullptr;
     c->d_ost = nullptr;
     c->opt_memory = 0;
     if (c->gexec)
@@ -1791,7 +1792,7 @@ int opt_run_prepare(OptRun &r, mia_hip_ctx *c, const double *x0, int maxeval, in
     }
     const std::vector<int> &seg = r.seg;
     if (c->opt_memory != memory) {
-        void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_obest, c->d_odone,
+        void *old[] = {c->d_oX, c->d_oG, c->d_oD, c->d_oBX, c->d_oS, c->d_oY, c->d_obest,
                        c->d_ost, c->d_oSY, c->d_oYY, c->d_oPART, c->d_oPART2, c->d_oseg, c->d_ost2};
         for (void *p : old)
             pool_release(c, p);
@@ -1800,13 +1801,11 @@ int opt_run_prepare(OptRun &r, mia_hip_ctx *c, const double *x0, int maxeval, in
         if ((st = dev_alloc(c, &c->d_oX, c->n_par)) || (st = dev_alloc(c, &c->d_oG, c->n_par)) ||
             (st = dev_alloc(c, &c->d_oD, c->n_par)) || (st = dev_alloc(c, &c->d_oBX, c->n_par)) ||
             (st = dev_alloc(c, &c->d_oS, R * c->n_par)) || (st = dev_alloc(c, &c->d_oY, R * c->n_par)) ||
-            (st = dev_alloc(c, &c->d_obest, n)) || (st = dev_alloc(c, &c->d_odone, n)) || (st = dev_alloc(c, &c->d_ost, n)) ||
+            (st = dev_alloc(c, &c->d_obest, n)) || (st = dev_alloc(c, &c->d_ost, n)) ||
             (st = dev_alloc(c, &c->d_oSY, R * R * n)) || (st = dev_alloc(c, &c->d_oYY, R * R * n)) ||
             (st = dev_alloc(c, &c->d_oPART, (size_t)kPart * seg[n])) ||
             (st = dev_alloc(c, &c->d_oPART2, (size_t)kPart * seg[n])) || (st = dev_alloc(c, &c->d_ost2, n)) ||
             (st = dev_alloc(c, &c->d_oseg, n + 1)))
-            return st;
-        if ((st = host_reserve(&c->h_done, &c->cap_done, 2 * (size_t)n)))
             return st;
         c->opt_memory = memory;
         c->opt_segments = seg[n];
@@ -1818,20 +1817,25 @@ int opt_run_prepare(OptRun &r, mia_hip_ctx *c, const double *x0, int maxeval, in
         o.phase = 0;
         o.mode = 0;
     }
-    std::vector<int> zeros(n, 0);
     if ((st = stage_upload(c, c->d_params, x0, sizeof(double) * c->n_par)) ||
         (st = stage_upload(c, c->d_ost, st0.data(), sizeof(OptState) * n)) ||
-        (st = stage_upload(c, c->d_odone, zeros.data(), sizeof(int) * n)) ||
         (st = stage_upload(c, c->d_oseg, seg.data(), sizeof(int) * (n + 1))))
         return st;
-    r.ob = OptBuffers{c->d_oX,  c->d_oG,   c->d_oD,   c->d_oBX,   c->d_oS,    c->d_oY,     c->d_obest, c->d_odone,
+    // the done flags in pinned host memory, cleared here (the stream is idle:
+    // no kernel of an earlier call still writes them), written by the kernels
+    // through their device address
+    if ((st = host_reserve(&c->h_done, &c->cap_done, (size_t)n)))
+        return st;
+    std::memset(c->h_done, 0, sizeof(int) * n);
+    int *done_dev = nullptr;
+    HIP_TRY(hipHostGetDevicePointer((void **)&done_dev, c->h_done, 0));
+    r.ob = OptBuffers{c->d_oX,  c->d_oG,   c->d_oD,   c->d_oBX,   c->d_oS,    c->d_oY,     c->d_obest, done_dev,
                       c->d_ost, c->d_ost2, c->d_oSY,  c->d_oYY,   c->d_oPART, c->d_oPART2, c->d_oseg};
     return MIA_HIP_OK;
 }
 
-// One chunk: ipg iterations (evaluation, then the step's kernels), then the
-// done flags copied to host slot `slot`.
-int opt_run_enqueue_chunk(OptRun &r, int slot)
+// One chunk: ipg iterations (evaluation, then the step's kernels).
+int opt_run_enqueue_chunk(OptRun &r)
 {
     mia_hip_ctx *c = r.c;
     const int n = c->n_inst;
@@ -1875,20 +1879,17 @@ int opt_run_enqueue_chunk(OptRun &r, int slot)
         if (e != hipSuccess)
             rc = fail(MIA_HIP_ERR_DEVICE, std::string("launch lbfgs passes: ") + hipGetErrorString(e));
     }
-    if (rc == MIA_HIP_OK) {
-        hipError_t e = hipMemcpyAsync(c->h_done + (size_t)slot * n, c->d_odone, sizeof(int) * n,
-                                      hipMemcpyDeviceToHost, c->stream);
-        if (e != hipSuccess)
-            rc = fail(MIA_HIP_ERR_DEVICE, hipGetErrorString(e));
-    }
     return rc;
 }
 
-bool opt_run_all_done(const OptRun &r, int slot)
+// Every instance done, as far as the kernels have written their flags (read
+// after a chunk's event: at least that chunk's flags; a kernel of the chunk
+// queued after it may already have set more)
+bool opt_run_all_done(const OptRun &r)
 {
-    const int n = r.c->n_inst;
-    for (int i = 0; i < n; ++i)
-        if (!r.c->h_done[(size_t)slot * n + i])
+    const volatile int *done = r.c->h_done;
+    for (int i = 0; i < r.c->n_inst; ++i)
+        if (!done[i])
             return false;
     return true;
 }
@@ -1920,7 +1921,7 @@ int opt_stream_loop(OptRun &r, double *seconds)
             break;
     auto queue = [&](int slot) {
         if (rc == MIA_HIP_OK)
-            rc = opt_run_enqueue_chunk(r, slot);
+            rc = opt_run_enqueue_chunk(r);
         if (rc == MIA_HIP_OK)
             check(hipEventRecord(r.ev[slot], r.c->stream));
     };
@@ -1932,7 +1933,7 @@ int opt_stream_loop(OptRun &r, double *seconds)
             queue((l + 1) & 1);
         if (!check(hipEventSynchronize(r.ev[l & 1])))
             break;
-        if (opt_run_all_done(r, l & 1) || l + 1 == max_launches)
+        if (opt_run_all_done(r) || l + 1 == max_launches)
             break;
         if (!ahead)
             queue((l + 1) & 1);
@@ -1957,7 +1958,7 @@ int opt_graph_loop(OptRun &r, double *seconds)
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = opt_run_enqueue_chunk(r, 0);
+    int rc = opt_run_enqueue_chunk(r);
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (rc != MIA_HIP_OK) {
         if (g)
@@ -1980,7 +1981,7 @@ int opt_graph_loop(OptRun &r, double *seconds)
             rc = fail(MIA_HIP_ERR_DEVICE, std::string("optimize graph: ") + hipGetErrorString(e));
             break;
         }
-        if (opt_run_all_done(r, 0))
+        if (opt_run_all_done(r))
             break;
     }
     const auto t1 = std::chrono::steady_clock::now();

@@ -64,7 +64,7 @@ struct OptState {
 // partial sums per segment (segment prefix) of passes 1 and 2.
 struct OptBuffers {
     double *X, *G, *D, *BX, *S, *Y, *best_cost;
-    int *done;
+    int *done; // pinned host memory, read by the host between chunks
     OptState *st, *ST2;
     double *SY, *YY, *PART, *PART2;
     const int *seg_prefix;
