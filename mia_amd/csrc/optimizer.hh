@@ -1019,38 +1019,52 @@ __device__ __forceinline__ void block_compact_coeffs(int memory, int k, int head
     const double a = part_total(part, isnew ? kPSG : po + 2);
     const double b = part_total(part, isnew ? kPYG : po + 3);
     const double rinv = 1.0 / mSY[si * memory + si];
-    // w = R^-1 a (back substitution, column by column)
+    // ring slot of stored pair j (j clamped to the stored ones): the table
+    // value each step needs is read one step ahead, so that its LDS latency
+    // overlaps the previous step instead of stalling the dependent chain
+    auto slot = [&](int j) {
+        const int jc = min(max(j, 0), k - 1);
+        return head + jc < memory ? head + jc : head + jc - memory;
+    };
+    // w = R^-1 a (back substitution, column by column: j = k - 1 .. 0)
     double r = a, wi = 0.0;
+    double mcur = mSY[si * memory + slot(k - 1)];
 #pragma unroll
-    for (int j = kMaxMemory - 1; j >= 0; --j)
-        if (j < k) {
-            const int sj = head + j < memory ? head + j : head + j - memory;
+    for (int t = 0; t < kMaxMemory; ++t)
+        if (t < k) {
+            const int j = k - 1 - t;
+            const double mnext = mSY[si * memory + slot(j - 1)];
             const double wj = readlane_d(r * rinv, j);
             if (il == j)
                 wi = wj;
             if (il < j)
-                r -= mSY[si * memory + sj] * wj;
+                r -= mcur * wj;
+            mcur = mnext;
         }
     // t = (Dg + gamma Y^T Y) w - gamma b
     double yw = 0.0;
+    mcur = mYY[si * memory + slot(0)];
 #pragma unroll
     for (int j = 0; j < kMaxMemory; ++j)
         if (j < k) {
-            const int sj = head + j < memory ? head + j : head + j - memory;
-            yw += mYY[si * memory + sj] * readlane_d(wi, j);
+            const double mnext = mYY[si * memory + slot(j + 1)];
+            yw += mcur * readlane_d(wi, j);
+            mcur = mnext;
         }
     // p = R^-T t (forward substitution)
     r = mSY[si * memory + si] * wi + gamma * yw - gamma * b;
     double pi = 0.0;
+    mcur = mSY[slot(0) * memory + si];
 #pragma unroll
     for (int j = 0; j < kMaxMemory; ++j)
         if (j < k) {
-            const int sj = head + j < memory ? head + j : head + j - memory;
+            const double mnext = mSY[slot(j + 1) * memory + si];
             const double pj = readlane_d(r * rinv, j);
             if (il == j)
                 pi = pj;
             if (il > j)
-                r -= mSY[sj * memory + si] * pj;
+                r -= mcur * pj;
+            mcur = mnext;
         }
     if (i < k) {
         cu[i] = pi;
