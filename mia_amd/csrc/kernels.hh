@@ -2033,6 +2033,7 @@ __global__ void __launch_bounds__(kBlock)
     const float2 cl = clamp[i];
     double ssd = 0.0;
     unsigned cnt = 0;
+    const int lane = threadIdx.x & 63;
     for (int x = threadIdx.x; x < nx; x += kBlock) {
         float acc[8];
 #pragma unroll
@@ -2042,27 +2043,61 @@ __global__ void __launch_bounds__(kBlock)
         const float4 wxv = xw_s[x];
         const float wxs[4] = {wxv.x, wxv.y, wxv.z, wxv.w};
         const bool ix = x > 0 && x < nx - 1;
-        const int xm = ix ? x - 1 : x, xp = ix ? x + 1 : x;
-        float wm = 0.0f, wc = 0.0f, wp = 0.0f;
+        // the column beyond the wave for its edge lanes (x - 1 for lane 0,
+        // x + 1 for lane 63, clamped to the row; the others read their own):
+        // the x neighbours come from the neighbouring lanes (DPP)
+        const long long xe = lane == 0 ? max(x - 1, 0) : lane == 63 ? min(x + 1, nx - 1) : x;
+        // one row's residual, force and its y-weighted sums (both modes)
+        auto add_row = [&](float w, float gx, float gy, float r, const float *wys) {
+            const bool on = (thr == 0.0f) || (w > thr);
+            const float dl = on ? w - r : 0.0f;
+            ssd += (double)dl * (double)dl;
+            cnt += on ? 1u : 0u;
+            const float fx = gx * dl, fy = gy * dl;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[2 * t + 0] = fmaf(wys[t], fx, acc[2 * t + 0]);
+                acc[2 * t + 1] = fmaf(wys[t], fy, acc[2 * t + 1]);
+            }
+        };
         if constexpr (MODE == 0) {
-            wc = Wi[(long long)j0 * nx + x];
-            wm = Wi[(long long)max(j0 - 1, 0) * nx + x];
-        }
-#pragma unroll kRowUnroll<MODE>
-        for (int j = j0; j < j1; ++j) {
-            const long long idx = (long long)j * nx + x;
-            const float4 wyv = tw[d.tab_y + j];
-            const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
-            float w, gx, gy;
-            if constexpr (MODE == 0) {
-                const bool iy = j > 0 && j < ny - 1;
-                wp = Wi[(long long)min(j + 1, ny - 1) * nx + x];
-                w = wc;
-                gx = ix ? 0.5f * (Wi[(long long)j * nx + xp] - Wi[(long long)j * nx + xm]) : 0.0f;
-                gy = iy ? 0.5f * (wp - wm) : 0.0f;
-                wm = wc;
-                wc = wp;
-            } else {
+            // rows four at a time: their loads (clamped rows) first, then the
+            // rows in order; W of the column rolls through wm, wc
+            float wc = Wi[(long long)j0 * nx + x];
+            float wm = Wi[(long long)max(j0 - 1, 0) * nx + x];
+            for (int j = j0; j < j1; j += 4) {
+                float wpv[4], ev[4], rv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = min(j + u, j1 - 1);
+                    wpv[u] = Wi[(long long)min(jj + 1, ny - 1) * nx + x];
+                    ev[u] = Wi[(long long)jj * nx + xe];
+                    rv[u] = Ri[(long long)jj * nx + x];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j + u < j1) {
+                        const int jr = j + u;
+                        const float4 wyv = tw[d.tab_y + jr];
+                        const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
+                        const bool iy = jr > 0 && jr < ny - 1;
+                        // every lane takes part in the shifts (a lane left out
+                        // would hand its neighbours the edge value instead of
+                        // its column): the opaque use keeps them out of the
+                        // border lanes' branch below
+                        float wxp = lane_next(wc, ev[u]), wxm = lane_prev(wc, ev[u]);
+                        asm volatile("" : "+v"(wxp), "+v"(wxm));
+                        const float gx = ix ? 0.5f * (wxp - wxm) : 0.0f;
+                        const float gy = iy ? 0.5f * (wpv[u] - wm) : 0.0f;
+                        add_row(wc, gx, gy, rv[u], wys);
+                        wm = wc;
+                        wc = wpv[u];
+                    }
+            }
+        } else {
+            for (int j = j0; j < j1; ++j) {
+                const float4 wyv = tw[d.tab_y + j];
+                const float wys[4] = {wyv.x, wyv.y, wyv.z, wyv.w};
                 float ux = 0.0f, uy = 0.0f;
 #pragma unroll
                 for (int bb = 0; bb < 4; ++bb) {
@@ -2078,34 +2113,36 @@ __global__ void __launch_bounds__(kBlock)
                     uy = fmaf(wys[bb], hy, uy);
                 }
                 float g[2];
-                w = interp2<DEG, true>(Mi, nx, ny, clamp_pos((float)x - ux), clamp_pos((float)j - uy), cl.x, cl.y,
-                                       g);
-                gx = g[0];
-                gy = g[1];
-            }
-            const float r = Ri[idx];
-            const bool on = (thr == 0.0f) || (w > thr);
-            const float dl = on ? w - r : 0.0f;
-            ssd += (double)dl * (double)dl;
-            cnt += on ? 1u : 0u;
-            const float fx = gx * dl, fy = gy * dl;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                acc[2 * t + 0] = fmaf(wys[t], fx, acc[2 * t + 0]);
-                acc[2 * t + 1] = fmaf(wys[t], fy, acc[2 * t + 1]);
+                const float w = interp2<DEG, true>(Mi, nx, ny, clamp_pos((float)x - ux), clamp_pos((float)j - uy),
+                                                   cl.x, cl.y, g);
+                add_row(w, g[0], g[1], Ri[(long long)j * nx + x], wys);
             }
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             acc_s[AS * x + k] = acc[k];
     }
-    const double s = block_sum(ssd, red);
-    const unsigned c = block_sum_u(cnt, redu);
+    // the block's SSD (and, with autothresh, mask count) as block_sum adds
+    // them: wave sums, then the waves in order; their partials share the
+    // barrier that completes the column sums (without autothresh the count
+    // is not used and its partial is 0)
+    const double sw = wave_sum(ssd);
+    const unsigned cw = thr != 0.0f ? wave_sum_u(cnt) : 0u;
+    if (lane == 0) {
+        red[threadIdx.x >> 6] = sw;
+        redu[threadIdx.x >> 6] = cw;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double s = 0.0;
+        unsigned c = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            s += red[k];
+            c += redu[k];
+        }
         ssd_part[b] = s;
         cnt_part[b] = c;
     }
-    __syncthreads();
     contract_x<2>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pout_all + d.part_off + (long long)band * 8 * cx);
 }
 
