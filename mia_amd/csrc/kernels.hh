@@ -1762,13 +1762,27 @@ __global__ void __launch_bounds__(kBlock)
         for (int k = 0; k < 12; ++k)
             acc_s[AS * x + k] = acc[k];
     }
-    const double s = block_sum(ssd, red);
-    const unsigned c = block_sum_u(cnt, redu);
+    // the block's SSD (and, with autothresh, mask count) as block_sum adds
+    // them: wave sums, then the waves in order; their partials share the
+    // barrier that completes the column sums (without autothresh the count
+    // is not used and its partial is 0)
+    const double sw = wave_sum(ssd);
+    const unsigned cw = thr != 0.0f ? wave_sum_u(cnt) : 0u;
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = sw;
+        redu[threadIdx.x >> 6] = cw;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double s = 0.0;
+        unsigned c = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            s += red[k];
+            c += redu[k];
+        }
         ssd_part[b] = s;
         cnt_part[b] = c;
     }
-    __syncthreads();
     contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx,
                   Pout_all + d.part_off + ((long long)z * d.ncy + band) * 12 * cx);
 }
@@ -1967,21 +1981,34 @@ __global__ void __launch_bounds__(kBlock, 4)
             wc[r] = wp[r];
         }
     }
-    const double s = block_sum(ssd, red);
-    const unsigned c = block_sum_u(lane == 0 ? cnt_w : 0u, redu);
-    if (threadIdx.x == 0) {
-        ssd_part[b] = s;
-        cnt_part[b] = c;
+    // the block's SSD and the waves' counts, added as block_sum adds them
+    // (wave sums, then the waves in order) by thread 0 after the first pass's
+    // barrier
+    const double sw = wave_sum(ssd);
+    if (lane == 0) {
+        red[threadIdx.x >> 6] = sw;
+        redu[threadIdx.x >> 6] = cnt_w;
     }
     float *Pb = Pout_all + d.part_off + ((long long)cz * d.ncy + band) * 48 * cx;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        __syncthreads(); // the previous pass is done with acc_s and S_s
+        if (k > 0)
+            __syncthreads(); // the previous pass is done with acc_s and S_s
         if (x < nx)
 #pragma unroll
             for (int m = 0; m < 12; ++m)
                 acc_s[AS * x + m] = a48[12 * k + m];
         __syncthreads();
+        if (k == 0 && threadIdx.x == 0) {
+            double s = 0.0;
+            unsigned c = 0;
+            for (int w = 0; w < NW; ++w) {
+                s += red[w];
+                c += redu[w];
+            }
+            ssd_part[b] = s;
+            cnt_part[b] = c;
+        }
         contract_x<3>(acc_s, AS, xw_s, cl_s, S_s, d.ncx, cx, Pb + 12 * k * cx);
     }
 }
