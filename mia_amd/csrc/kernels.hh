@@ -1450,10 +1450,10 @@ __global__ void __launch_bounds__(kBlock)
     bool use_lds = false;
     const float *Mi = M + d.img_off;
     if constexpr (BOX) {
-        mnx = wave_min(mnx);
-        mny = wave_min(mny);
-        mxx = wave_max(mxx);
-        mxy = wave_max(mxy);
+        // wave extremes by the interleaved DPP chain (the y pair twice: it
+        // takes three of each); positions are clamped, never NaN
+        float mny2 = mny, mxy2 = mxy;
+        wave_min3_max3(mnx, mny, mny2, mxx, mxy, mxy2);
         const int wid = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) {
             red[0][wid] = mnx;
@@ -2261,10 +2261,15 @@ __device__ __forceinline__ void finalize_instance(const InstDesc &d, int i, SsdI
     const bool z = si.zmode && d.zcell;
     const int beg = z ? si.zoff + si.s1z[i] : si.s1[i], end = z ? si.zoff + si.s1z[i + 1] : si.s1[i + 1];
     double s;
-    unsigned c;
-    ssd_thread_share<true>(beg, end, ssd_part, cnt_part, s, c);
+    unsigned c = 0;
+    // the mask count only matters with autothresh
+    if (thr != 0.0f) {
+        ssd_thread_share<true>(beg, end, ssd_part, cnt_part, s, c);
+        c = block_sum_u(c, redu);
+    } else {
+        ssd_thread_share<false>(beg, end, ssd_part, cnt_part, s, c);
+    }
     s = block_sum(s, red);
-    c = block_sum_u(c, redu);
     if (threadIdx.x == 0) {
         const double n = (double)d.nx * d.ny * d.nz;
         double cost, sc;
