@@ -1350,7 +1350,7 @@ __global__ void __launch_bounds__(kBlock)
 template <int DEG, int TX, int TY, bool BOX>
 __global__ void __launch_bounds__(kBlock)
     k_warp2d(const InstDesc *__restrict__ inst, const int *__restrict__ ids, int n_inst,
-             const int *__restrict__ prefix, int boff, int nblocks,
+             const int *__restrict__ prefix, const int4 *__restrict__ tiles, int boff, int nblocks,
              const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const float *__restrict__ M, const float2 *__restrict__ clamp, float *__restrict__ W)
 {
@@ -1370,19 +1370,38 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ float4 wy_s[TY];
     __shared__ int sy_s[TY];
     const int b = boff + xcd_remap(blockIdx.x, nblocks);
-    const int gi = find_inst(prefix, n_inst, b);
-    const int i = ids[gi];
-    const InstDesc d = inst[i];
-    const int lb = b - prefix[gi];
-    const int tiles_x = (d.nx + TX - 1) / TX;
-    const int x0 = (lb % tiles_x) * TX, y0 = (lb / tiles_x) * TY;
+    // the tile's instance, origin and coefficient window: from the tile
+    // table in one load, or derived from the tile prefix and axis tables
+    // (a chain of dependent loads)
+    int i, x0, y0, kx0, ky0, wxn, wyn;
+    InstDesc d;
+    if (tiles) {
+        const int4 t0 = tiles[2 * b], t1 = tiles[2 * b + 1];
+        i = t0.x;
+        x0 = t0.y;
+        y0 = t0.z;
+        kx0 = t0.w;
+        ky0 = t1.x;
+        wxn = t1.y;
+        wyn = t1.z;
+        d = inst[i];
+    } else {
+        const int gi = find_inst(prefix, n_inst, b);
+        i = ids[gi];
+        d = inst[i];
+        const int lb = b - prefix[gi];
+        const int tiles_x = (d.nx + TX - 1) / TX;
+        x0 = (lb % tiles_x) * TX;
+        y0 = (lb / tiles_x) * TY;
+        kx0 = ts[d.tab_x + x0];
+        ky0 = ts[d.tab_y + y0];
+        wxn = ts[d.tab_x + min(x0 + TX, d.nx) - 1] + 4 - kx0;
+        wyn = ts[d.tab_y + min(y0 + TY, d.ny) - 1] + 4 - ky0;
+    }
     const int x = x0 + (threadIdx.x & (TX - 1));
     const int ly = threadIdx.x / TX;
     const bool xvalid = x < d.nx;
     const int xc = min(x, d.nx - 1);
-    const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
-    const int wxn = ts[d.tab_x + min(x0 + TX, d.nx) - 1] + 4 - kx0;
-    const int wyn = ts[d.tab_y + min(y0 + TY, d.ny) - 1] + 4 - ky0;
     const double *P = params + d.par_off;
     // this thread's x table entry and the clamp range, loaded now so that
     // their latency overlaps the window's (they are used after two barriers)

@@ -216,6 +216,7 @@ struct mia_hip_ctx {
         std::vector<int> ids;      // instances of the group, ascending
         std::vector<int> prefix;   // tile prefix over ids
         int ids_off = 0, prefix_off = 0; // offsets into d_wgroups
+        int tiles2d_off = -1;            // 2D: offset of the tile table in d_wgroups (multiple of 4)
     };
     std::vector<WarpGroup> wgroups;
     std::vector<int> inst_variant, inst_group, inst_slot;
@@ -751,6 +752,9 @@ const void *warp3d_kernel(int variant)
 struct WarpLaunch {
     int variant = 0, n = 0, tiles = 0;
     const int *ids = nullptr, *prefix = nullptr; // device: instance ids, tile prefix
+    // 2D: per tile {instance, x0, y0, first x knot, first y knot, window
+    // width, window height, 0} (two int4), or null (the kernel derives them)
+    const int4 *tiles2d = nullptr;
 };
 
 // Everything one cost + gradient evaluation launches with: device pointers
@@ -808,6 +812,7 @@ EvalPlan plan_of(const mia_hip_ctx *c)
         p.warp[g].tiles = G.prefix.back();
         p.warp[g].ids = c->d_wgroups + G.ids_off;
         p.warp[g].prefix = c->d_wgroups + G.prefix_off;
+        p.warp[g].tiles2d = G.tiles2d_off >= 0 ? (const int4 *)(c->d_wgroups + G.tiles2d_off) : nullptr;
     }
     p.inst = c->d_inst;
     p.s1_prefix = c->d_s1_prefix;
@@ -865,8 +870,8 @@ int launch_warp(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const doub
     } else {
         auto kern = warp2d_kernel(pl.degree, G.variant);
         const size_t lds = (size_t)w2_region(G.variant) * sizeof(float) + (size_t)pl.w2_win * sizeof(float2);
-        launch(s, prof, ST_WARP, kern, dim3(nb), lds, pl.inst, G.ids, G.n, G.prefix, boff, nb, d_params, pl.tw,
-               pl.ts, pl.M, pl.clamp, pl.W);
+        launch(s, prof, ST_WARP, kern, dim3(nb), lds, pl.inst, G.ids, G.n, G.prefix, G.tiles2d, boff, nb, d_params,
+               pl.tw, pl.ts, pl.M, pl.clamp, pl.W);
     }
     LAUNCH_CHECK("warp");
     return MIA_HIP_OK;
@@ -1415,6 +1420,27 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
         G.prefix_off = (int)wg.size();
         wg.insert(wg.end(), G.prefix.begin(), G.prefix.end());
     }
+    if (dim == 2)
+        for (auto &G : c->wgroups) {
+            // each tile's instance, origin and coefficient window, as the
+            // kernel would derive them from the tile prefix and the axis tables
+            while (wg.size() % 4)
+                wg.push_back(0);
+            G.tiles2d_off = (int)wg.size();
+            const int tx = w2_tile_x(G.variant), ty = w2_tile_y(G.variant);
+            for (int i : G.ids) {
+                const InstDesc &d = c->inst[i];
+                const int tiles_x = (d.nx + tx - 1) / tx, tiles_y = (d.ny + ty - 1) / ty;
+                for (int lb = 0; lb < tiles_x * tiles_y; ++lb) {
+                    const int x0 = (lb % tiles_x) * tx, y0 = (lb / tiles_x) * ty;
+                    const int kx0 = ts[d.tab_x + x0], ky0 = ts[d.tab_y + y0];
+                    const int wxn = ts[d.tab_x + std::min(x0 + tx, d.nx) - 1] + 4 - kx0;
+                    const int wyn = ts[d.tab_y + std::min(y0 + ty, d.ny) - 1] + 4 - ky0;
+                    const int e[8] = {i, x0, y0, kx0, ky0, wxn, wyn, 0};
+                    wg.insert(wg.end(), e, e + 8);
+                }
+            }
+        }
     c->n_img = img;
     c->n_par = par;
     c->n_part = part;
