@@ -2035,7 +2035,8 @@ __global__ void __launch_bounds__(kBlock, 4)
 // Stage 1, 2D: block = y knot band.
 template <int DEG, int MODE>
 __global__ void __launch_bounds__(kBlock)
-    k_stage1_2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix, int nblocks,
+    k_stage1_2d(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
+                const int4 *__restrict__ blocks, int nblocks,
                 const double *__restrict__ params, const float4 *__restrict__ tw, const int *__restrict__ ts,
                 const int *__restrict__ cells, const float *__restrict__ M, const float2 *__restrict__ clamp,
                 const float *__restrict__ W, const float *__restrict__ R, float thr, float *__restrict__ Pout_all,
@@ -2045,10 +2046,24 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ double red[kBlock / 64];
     __shared__ unsigned redu[kBlock / 64];
     const int b = xcd_remap(blockIdx.x, nblocks);
-    const int i = find_inst(prefix, n_inst, b);
+    // the block's instance, band and rows: from the block table in one load,
+    // or derived from the block prefix and band table (dependent loads)
+    int i, band, j0, j1;
+    if (blocks) {
+        const int4 e = blocks[b];
+        i = e.x;
+        band = e.y;
+        j0 = e.z;
+        j1 = e.w;
+    } else {
+        i = find_inst(prefix, n_inst, b);
+        band = b - prefix[i];
+    }
     const InstDesc d = inst[i];
-    const int band = b - prefix[i];
-    const int j0 = cells[d.cell_y + band], j1 = cells[d.cell_y + band + 1];
+    if (!blocks) {
+        j0 = cells[d.cell_y + band];
+        j1 = cells[d.cell_y + band + 1];
+    }
     const int nx = d.nx, ny = d.ny, cx = d.cx;
     const Stage1Layout L = stage1_layout(2, nx, cx, d.ncx, MODE == 1);
     const int AS = L.acc_stride;

@@ -219,6 +219,7 @@ struct mia_hip_ctx {
         int tiles2d_off = -1;            // 2D: offset of the tile table in d_wgroups (multiple of 4)
     };
     std::vector<WarpGroup> wgroups;
+    int s1tab_off = -1; // 2D: offset in d_wgroups of the stage-1 block table (multiple of 4)
     std::vector<int> inst_variant, inst_group, inst_slot;
     int force_variant = -1; // -1: tuned per size bucket
     int force_s1 = -1;      // stage-1 layout of 3D reference gradients: -1 tuned, 0 per slice, 1 z cell
@@ -771,6 +772,7 @@ struct EvalPlan {
     WarpLaunch warp[kMaxTuneChoices];
     const InstDesc *inst = nullptr;
     const int *s1_prefix = nullptr, *red_prefix = nullptr, *s1z_prefix = nullptr, *redz_prefix = nullptr;
+    const int4 *s1tab2d = nullptr; // 2D stage 1: per block {instance, band, first row, end row}, or null
     const float *M = nullptr, *R = nullptr;
     float *W = nullptr, *P = nullptr;
     const float4 *tw = nullptr;
@@ -814,6 +816,7 @@ EvalPlan plan_of(const mia_hip_ctx *c)
         p.warp[g].prefix = c->d_wgroups + G.prefix_off;
         p.warp[g].tiles2d = G.tiles2d_off >= 0 ? (const int4 *)(c->d_wgroups + G.tiles2d_off) : nullptr;
     }
+    p.s1tab2d = c->s1tab_off >= 0 ? (const int4 *)(c->d_wgroups + c->s1tab_off) : nullptr;
     p.inst = c->d_inst;
     p.s1_prefix = c->d_s1_prefix;
     p.red_prefix = c->d_red_prefix;
@@ -904,15 +907,17 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
     const size_t lds = stage1_layout(pl.dim, pl.max_nx, pl.max_cx, pl.max_ncx, exact).bytes;
     if (any_slice) {
         const int nb = pl.s1_blocks;
-        decltype(&k_stage1_3d<3, 0>) kern;
-        if (pl.dim == 3)
-            kern = pl.degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
-                                  : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
-        else
-            kern = pl.degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
-                                  : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
-        launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1_prefix, nb, d_params, pl.tw, pl.ts,
-               pl.cells, pl.M, pl.clamp, (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd, pl.cnt);
+        if (pl.dim == 3) {
+            auto kern = pl.degree == 3 ? (exact ? k_stage1_3d<3, 1> : k_stage1_3d<3, 0>)
+                                       : (exact ? k_stage1_3d<1, 1> : k_stage1_3d<1, 0>);
+            launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1_prefix, nb, d_params, pl.tw, pl.ts,
+                   pl.cells, pl.M, pl.clamp, (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd, pl.cnt);
+        } else {
+            auto kern = pl.degree == 3 ? (exact ? k_stage1_2d<3, 1> : k_stage1_2d<3, 0>)
+                                       : (exact ? k_stage1_2d<1, 1> : k_stage1_2d<1, 0>);
+            launch(s, prof, ST_STAGE1, kern, dim3(nb), lds, pl.inst, n, pl.s1_prefix, pl.s1tab2d, nb, d_params,
+                   pl.tw, pl.ts, pl.cells, pl.M, pl.clamp, (const float *)pl.W, pl.R, pl.thr, pl.P, pl.ssd, pl.cnt);
+        }
         LAUNCH_CHECK("stage1");
     }
     if (n_z > 0) {
@@ -1419,6 +1424,20 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
         wg.insert(wg.end(), G.ids.begin(), G.ids.end());
         G.prefix_off = (int)wg.size();
         wg.insert(wg.end(), G.prefix.begin(), G.prefix.end());
+    }
+    c->s1tab_off = -1;
+    if (dim == 2) {
+        // stage 1: each block's {instance, y band, first row, end row}
+        while (wg.size() % 4)
+            wg.push_back(0);
+        c->s1tab_off = (int)wg.size();
+        for (int i = 0; i < n_inst; ++i) {
+            const InstDesc &d = c->inst[i];
+            for (int band = 0; band < c->s1_prefix[i + 1] - c->s1_prefix[i]; ++band) {
+                const int e[4] = {i, band, cells[d.cell_y + band], cells[d.cell_y + band + 1]};
+                wg.insert(wg.end(), e, e + 4);
+            }
+        }
     }
     if (dim == 2)
         for (auto &G : c->wgroups) {
