@@ -2347,7 +2347,7 @@ __global__ void __launch_bounds__(kBlock)
 template <int DIM, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
     k_reduce(const InstDesc *__restrict__ inst, int n_inst, const int *__restrict__ prefix,
-             const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
+             const int2 *__restrict__ blocks, const float *__restrict__ P, const float4 *__restrict__ tw, const int *__restrict__ ts,
              const int *__restrict__ cells, SsdIndex si, int nfin, const double *__restrict__ ssd_part,
              const unsigned *__restrict__ cnt_part, int norm, float thr, double weight, double *__restrict__ costs,
              const double *__restrict__ gscale, double *__restrict__ grad, PenaltyIn pen)
@@ -2358,7 +2358,17 @@ __global__ void __launch_bounds__(kBlock)
         return;
     }
     b -= nfin;
-    const int i = find_inst(prefix, n_inst, b);
+    // the block's instance and its block within the instance: from the block
+    // table in one load (2D), or from the block prefix
+    int i, lb;
+    if (blocks) {
+        const int2 e = blocks[b];
+        i = e.x;
+        lb = e.y;
+    } else {
+        i = find_inst(prefix, n_inst, b);
+        lb = b - prefix[i];
+    }
     const InstDesc d = inst[i];
     if (si.zmode && d.zcell)
         return;
@@ -2373,7 +2383,7 @@ __global__ void __launch_bounds__(kBlock)
         // the quarters are added in order through LDS
         __shared__ double part[kBlock / 64][64];
         const int lane = threadIdx.x & 63, g4 = threadIdx.x >> 6;
-        const long long e3 = (long long)(b - prefix[i]) * 64 + lane;
+        const long long e3 = (long long)lb * 64 + lane;
         double g = 0.0;
         if (e3 < 3 * ncoef) {
             const int comp = (int)(e3 % 3);
@@ -2429,7 +2439,7 @@ __global__ void __launch_bounds__(kBlock)
                 sc, ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane], pen, i, d.par_off + e3);
         return;
     } else {
-        const long long e3 = (long long)(b - prefix[i]) * kBlock + threadIdx.x;
+        const long long e3 = (long long)lb * kBlock + threadIdx.x;
         if (e3 >= 2 * ncoef)
             return;
         grad[d.par_off + e3] = reduce2_elem(d, P, e3, sc, pen, i);

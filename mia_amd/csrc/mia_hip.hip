@@ -220,6 +220,7 @@ struct mia_hip_ctx {
     };
     std::vector<WarpGroup> wgroups;
     int s1tab_off = -1; // 2D: offset in d_wgroups of the stage-1 block table (multiple of 4)
+    int redtab_off = -1; // 2D: offset in d_wgroups of the reduce block table (multiple of 2)
     std::vector<int> inst_variant, inst_group, inst_slot;
     int force_variant = -1; // -1: tuned per size bucket
     int force_s1 = -1;      // stage-1 layout of 3D reference gradients: -1 tuned, 0 per slice, 1 z cell
@@ -773,6 +774,7 @@ struct EvalPlan {
     const InstDesc *inst = nullptr;
     const int *s1_prefix = nullptr, *red_prefix = nullptr, *s1z_prefix = nullptr, *redz_prefix = nullptr;
     const int4 *s1tab2d = nullptr; // 2D stage 1: per block {instance, band, first row, end row}, or null
+    const int2 *redtab2d = nullptr; // 2D reduce: per gradient block {instance, block of the instance}, or null
     const float *M = nullptr, *R = nullptr;
     float *W = nullptr, *P = nullptr;
     const float4 *tw = nullptr;
@@ -817,6 +819,7 @@ EvalPlan plan_of(const mia_hip_ctx *c)
         p.warp[g].tiles2d = G.tiles2d_off >= 0 ? (const int4 *)(c->d_wgroups + G.tiles2d_off) : nullptr;
     }
     p.s1tab2d = c->s1tab_off >= 0 ? (const int4 *)(c->d_wgroups + c->s1tab_off) : nullptr;
+    p.redtab2d = c->redtab_off >= 0 ? (const int2 *)(c->d_wgroups + c->redtab_off) : nullptr;
     p.inst = c->d_inst;
     p.s1_prefix = c->d_s1_prefix;
     p.red_prefix = c->d_red_prefix;
@@ -948,9 +951,10 @@ int enqueue_plan(const EvalPlan &pl, hipStream_t s, mia_hip_ctx *prof, const dou
             kern = fused ? k_reduce<3, true> : k_reduce<3, false>;
         else
             kern = fused ? k_reduce<2, true> : k_reduce<2, false>;
-        launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.red_prefix, (const float *)pl.P, pl.tw, pl.ts,
-               pl.cells, si, nfin, (const double *)pl.ssd, (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight,
-               d_costs, (const double *)pl.gscale, d_grad, pl.pen);
+        launch(s, prof, ST_REDUCE, kern, dim3(nb), 0, pl.inst, n, pl.red_prefix, pl.dim == 2 ? pl.redtab2d : nullptr,
+               (const float *)pl.P, pl.tw, pl.ts, pl.cells, si, nfin, (const double *)pl.ssd,
+               (const unsigned *)pl.cnt, pl.norm, pl.thr, pl.weight, d_costs, (const double *)pl.gscale, d_grad,
+               pl.pen);
         LAUNCH_CHECK("reduce");
         nfin = 0;
     }
@@ -1426,7 +1430,17 @@ int setup_impl(mia_hip_ctx *c, int dim, int n_inst, const uint32_t *sizes, const
         wg.insert(wg.end(), G.prefix.begin(), G.prefix.end());
     }
     c->s1tab_off = -1;
+    c->redtab_off = -1;
     if (dim == 2) {
+        // reduce: each gradient block's {instance, block of the instance}
+        while (wg.size() % 2)
+            wg.push_back(0);
+        c->redtab_off = (int)wg.size();
+        for (int i = 0; i < n_inst; ++i)
+            for (int k = 0; k < c->red_prefix[i + 1] - c->red_prefix[i]; ++k) {
+                wg.push_back(i);
+                wg.push_back(k);
+            }
         // stage 1: each block's {instance, y band, first row, end row}
         while (wg.size() % 4)
             wg.push_back(0);
